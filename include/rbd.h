@@ -1,0 +1,159 @@
+/*
+ * rbd.h -- C ABI of the MI355X batched rigid-body-dynamics engine (libpinocchio_amd.so).
+ *
+ * The engine evaluates the reference's dynamics algorithms over a batch of B
+ * configurations of one model in a single call, on one GPU.  Each batched entry point
+ * replaces a loop of the reference's single-configuration algorithm over the columns of
+ * Eigen matrices, i.e. what pinocchio's rneaInParallel / abaInParallel do with OpenMP:
+ *
+ *   rbd_rnea              rneaInParallel            algorithm/parallel/rnea.hpp:39-83
+ *                         (per column: rnea         algorithm/rnea.hxx:117-161, 773)
+ *   rbd_aba               abaInParallel             algorithm/parallel/aba.hpp:40-84
+ *                         (per column: aba WORLD    algorithm/aba.hxx:242-293, 926-948)
+ *   rbd_crba              crba                      algorithm/crba.hxx:498-548, 602-617
+ *   rbd_cholesky_decompose cholesky::decompose      algorithm/cholesky.hxx:17-62
+ *   rbd_cholesky_solve    cholesky::solve           algorithm/cholesky.hxx:466-520
+ *   rbd_rnea_derivatives  computeRNEADerivatives    algorithm/rnea-derivatives.hxx:472-541, 756-766
+ *   rbd_aba_derivatives   computeABADerivatives     algorithm/aba-derivatives.hxx:380-453, 900-923
+ *   rbd_multi_rnea_aba    (no counterpart: one launch over instances of several models)
+ *
+ * Conventions (identical to the reference): 6-vectors are [linear; angular]; free-flyer
+ * configuration is [x, y, z, qx, qy, qz, qw]; joint 0 is the universe; parents[i] < i.
+ *
+ * Array layout.  A batched vector argument (q: nq x B, v/a/tau/ddq: nv x B) is a matrix
+ * with one configuration per column.  By default it is column-major like Eigen:
+ * element (k, b) at p[k + b * ld] with ld >= rows.  With RBD_LAYOUT_BATCH the batch index
+ * is fastest: element (k, b) at p[k * ld + b] with ld >= B (the engine's native layout).
+ * A batched matrix argument (nv x nv per configuration) uses element index k = r + c * nv
+ * (column-major inside the configuration) in the same two layouts.
+ *
+ * Pointers are device pointers (hipMalloc'd or torch tensors) unless RBD_HOST_PTRS is
+ * set, in which case the engine stages through its own device buffers and synchronises.
+ * Without RBD_ASYNC every call synchronises its stream before returning, like the
+ * reference's synchronous calls; with RBD_ASYNC it returns once the work is enqueued.
+ *
+ * Return value: 0 on success, otherwise an RBD_E* code; rbd_last_error() then gives a
+ * message (thread-local).  Shape errors mirror the reference's std::invalid_argument
+ * conditions (algorithm/parallel/rnea.hpp:53-68, algorithm/parallel/aba.hpp:54-69).
+ */
+#ifndef RBD_H_
+#define RBD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RBD_OK = 0,
+  RBD_EINVAL = 1,      /* bad argument or shape (std::invalid_argument in the reference) */
+  RBD_EUNSUPPORTED = 2,/* joint type / topology / dtype the engine does not handle */
+  RBD_EDEVICE = 3,     /* HIP runtime error */
+  RBD_ENOMEM = 4
+};
+
+enum { RBD_F32 = 0, RBD_F64 = 1 };
+
+/* joint types */
+enum { RBD_JOINT_UNIVERSE = 0, RBD_JOINT_FREEFLYER = 1, RBD_JOINT_RX = 2, RBD_JOINT_RY = 3, RBD_JOINT_RZ = 4 };
+
+/* flags */
+#define RBD_HOST_PTRS 0x1u     /* arguments are host pointers */
+#define RBD_ASYNC 0x2u         /* do not synchronise before returning */
+#define RBD_LAYOUT_BATCH 0x4u  /* batch index fastest (see header comment) */
+#define RBD_UPPER_ONLY 0x8u    /* matrix outputs: write the upper triangle only (reference's crba) */
+
+/* Flattened model (multibody/model.hpp:98-220).  All arrays have njoints entries, entry 0
+ * being the universe.  placement[i] = R (row major, 9) then translation (3), the joint
+ * placement in the parent frame; inertia[i] = mass, lever c (3), rotational inertia about
+ * the centre of mass as (xx, xy, yy, xz, yz, zz).  armature has nv entries (may be NULL),
+ * gravity 6 (may be NULL for (0, 0, -9.81, 0, 0, 0)).  The engine copies what it needs. */
+typedef struct rbd_model_desc {
+  int32_t njoints;
+  const int32_t* parents;
+  const int32_t* jtype;
+  const double* placement;
+  const double* inertia;
+  const double* armature;
+  const double* gravity;
+} rbd_model_desc;
+
+typedef struct rbd_ctx rbd_ctx;
+typedef struct rbd_model rbd_model;
+
+const char* rbd_last_error(void);
+const char* rbd_version(void);
+
+/* Context: owns a HIP stream on `device` and the staging buffers.  Replaces the
+ * ModelPool + OpenMP thread team of the reference (multibody/pool/model.hpp:19-165,
+ * algorithm/parallel/omp.hpp:12-16); no global state is mutated. */
+int rbd_ctx_create(int device, rbd_ctx** out);
+int rbd_ctx_destroy(rbd_ctx* ctx);
+/* The stream the context enqueues on (a hipStream_t). */
+void* rbd_ctx_stream(rbd_ctx* ctx);
+int rbd_ctx_synchronize(rbd_ctx* ctx);
+
+/* Model upload (ModelTpl::addJoint, multibody/model.hxx:61-170, flattened).  Fails with
+ * RBD_EUNSUPPORTED when the engine has no kernels for the model's topology. */
+int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* desc, rbd_model** out);
+int rbd_model_destroy(rbd_model* model);
+/* nq, nv, njoints and whether the model is DFS-compact (crba.hxx:573-595). */
+int rbd_model_dims(const rbd_model* model, int32_t* nq, int32_t* nv, int32_t* njoints, int32_t* compact);
+
+/* tau = rnea(q, v, a) per configuration.  Replaces rneaInParallel. */
+int rbd_rnea(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+             const void* v, int64_t ldv, const void* a, int64_t lda, void* tau, int64_t ldtau, uint32_t flags);
+
+/* ddq = aba(q, v, tau) per configuration.  Replaces abaInParallel (WORLD convention; the
+ * result is convention-independent up to rounding, the engine sweeps in local frames). */
+int rbd_aba(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+            const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq, int64_t ldddq, uint32_t flags);
+
+/* M = crba(q) per configuration: joint-space inertia matrix with armature on the
+ * diagonal.  Symmetric by default; RBD_UPPER_ONLY writes the upper triangle only and
+ * leaves the rest untouched (the reference's behaviour). */
+int rbd_crba(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+             void* M, int64_t ldM, uint32_t flags);
+
+/* FP64 tree-sparse factorisation M = U D U^T of each M (upper triangle read; dtype is
+ * the dtype of M).  U is written as the nnz_upper strictly-upper structural non-zeros in
+ * double, column by column (see rbd_model_cholesky_pattern), and D as nv doubles.  Both
+ * use the layout given by flags with element index = position in the packed list. */
+int rbd_cholesky_decompose(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* M,
+                           int64_t ldM, double* U, int64_t ldU, double* D, int64_t ldD, uint32_t flags);
+/* Number of strictly-upper structural non-zeros and their (row, col) list, in packed order. */
+int rbd_model_cholesky_pattern(const rbd_model* model, int32_t* nnz, int32_t* rows, int32_t* cols);
+/* x <- M^{-1} x with a factor from rbd_cholesky_decompose (x has dtype `dtype`). */
+int rbd_cholesky_solve(rbd_ctx* ctx, const rbd_model* model, int64_t batch, const double* U, int64_t ldU,
+                       const double* D, int64_t ldD, int dtype, void* x, int64_t ldx, uint32_t flags);
+
+/* computeRNEADerivatives: dtau_dq, dtau_dv, dtau_da (= M) per configuration; any output
+ * may be NULL.  Structurally zero blocks are written as zeros.  Requires gravity with a
+ * zero angular part, like the reference (rnea-derivatives.hxx:494-496). */
+int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                         int64_t ldq, const void* v, int64_t ldv, const void* a, int64_t lda, void* dtau_dq,
+                         void* dtau_dv, void* dtau_da, int64_t ldout, uint32_t flags);
+
+/* computeABADerivatives: ddq_dq, ddq_dv, ddq_dtau (= M^{-1}) per configuration, and
+ * optionally ddq (may be NULL).  FP32 only. */
+int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                        int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq,
+                        void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags);
+
+/* Mixed-model batch: instances of up to 8 models in one launch, each instance running
+ * RNEA (tau from q, v, a) and ABA (ddq from q, v, tau_in).  Instance k uses model
+ * models[model_id[k]]; its vectors start at offset_q[k] / offset_v[k] elements in the
+ * packed float32 device arrays q, v, a, tau_in, tau_out, ddq_out (each instance's
+ * vector contiguous).  Device pointers only.  The per-model tile width is chosen by the
+ * autotuner on first use. */
+int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t nmodels, const rbd_model* const* models, int64_t ninstances,
+                       const int32_t* model_id, const int64_t* offset_q, const int64_t* offset_v,
+                       const float* q, const float* v, const float* a, const float* tau_in, float* tau_out,
+                       float* ddq_out, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBD_H_ */

@@ -1,0 +1,393 @@
+// oracle_capi.cpp -- C entry points of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+//
+// Batched wrappers in the shape of rneaInParallel / abaInParallel
+// (algorithm/parallel/rnea.hpp:39-83, algorithm/parallel/aba.hpp:40-84): an OpenMP
+// `parallel for schedule(static)` over configurations with one Data workspace per thread.
+// All arrays are column-major with one configuration per column (Eigen's default):
+// q is nq x B, v/a/tau/ddq are nv x B, matrices are nv x nv per configuration stored
+// back to back.  Inputs and outputs are double; the `_s` suffix selects the scalar type
+// the algorithm runs in (0 = double, 1 = float, 2 = long double), so the float rounding
+// of the reference algorithm itself can be measured.
+#include <omp.h>
+
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "rbd_oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+thread_local std::string g_err;
+
+template<class S, class Fn>
+int runBatch(const ModelDesc* d, int64_t B, int nthreads, Fn&& fn) {
+  try {
+    if (!d) throw std::invalid_argument("null model");
+    if (B < 0) throw std::invalid_argument("negative batch");
+    const Model<S> model(*d);
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    std::string err;
+#pragma omp parallel num_threads(nt)
+    {
+      Data<S> data(model);
+#pragma omp for schedule(static)
+      for (int64_t b = 0; b < B; ++b) {
+        try {
+          fn(model, data, b);
+        } catch (const std::exception& e) {
+#pragma omp critical
+          err = e.what();
+        }
+      }
+    }
+    if (!err.empty()) throw std::runtime_error(err);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+template<class S> struct Buf {
+  std::vector<S> x;
+  const S* load(const double* src, int n) {
+    x.resize(n);
+    for (int k = 0; k < n; ++k) x[k] = S(src[k]);
+    return x.data();
+  }
+};
+template<class S> inline void store(const S* src, double* dst, int n) {
+  for (int k = 0; k < n; ++k) dst[k] = double(src[k]);
+}
+template<class S> inline void storeMat(const Mat<S>& m, double* dst) {
+  for (size_t k = 0; k < m.a.size(); ++k) dst[k] = double(m.a[k]);
+}
+
+template<class S>
+int rneaT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a, double* tau, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv, ba;
+    static thread_local std::vector<S> out;
+    out.resize(m.nv);
+    rnea(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * m.nv, m.nv), ba.load(a + b * m.nv, m.nv), out.data());
+    store(out.data(), tau + b * m.nv, m.nv);
+  });
+}
+template<class S>
+int abaT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau, double* ddq, int conv,
+         int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv, bt;
+    static thread_local std::vector<S> out;
+    out.resize(m.nv);
+    const S* qq = bq.load(q + b * m.nq, m.nq);
+    const S* vv = bv.load(v + b * m.nv, m.nv);
+    const S* tt = bt.load(tau + b * m.nv, m.nv);
+    if (conv == 0) abaLocal(m, data, qq, vv, tt, out.data());
+    else abaWorld(m, data, qq, vv, tt, out.data());
+    store(out.data(), ddq + b * m.nv, m.nv);
+  });
+}
+template<class S> int crbaT(const ModelDesc* d, int64_t B, const double* q, double* M, int symmetrize, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    if (!m.isCompact()) throw std::invalid_argument("CRBA requires a compact (DFS-ordered) model");
+    static thread_local Buf<S> bq;
+    crbaWorld(m, data, bq.load(q + b * m.nq, m.nq));
+    if (symmetrize)
+      for (int c = 0; c < m.nv; ++c)
+        for (int r = c + 1; r < m.nv; ++r) data.M(r, c) = data.M(c, r);
+    storeMat(data.M, M + b * m.nv * m.nv);
+  });
+}
+template<class S>
+int choleskyT(const ModelDesc* d, int64_t B, const double* M, double* U, double* D, double* Minv, double* x, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    const int nv = m.nv;
+    const double* Mb = M + b * nv * nv;
+    for (int k = 0; k < nv * nv; ++k) data.M.a[k] = S(Mb[k]);
+    choleskyDecompose(m, data);
+    if (U) storeMat(data.Ufac, U + b * nv * nv);
+    if (D) store(data.D.data(), D + b * nv, nv);
+    if (Minv) {
+      Mat<S> mi(nv, nv);
+      choleskyComputeMinv(m, data, mi);
+      for (int c = 0; c < nv; ++c)
+        for (int r = c + 1; r < nv; ++r) mi(r, c) = mi(c, r);
+      storeMat(mi, Minv + b * nv * nv);
+    }
+    if (x) {
+      std::vector<S> xs(nv);
+      for (int k = 0; k < nv; ++k) xs[k] = S(x[b * nv + k]);
+      choleskySolve(m, data, xs.data());
+      store(xs.data(), x + b * nv, nv);
+    }
+  });
+}
+template<class S>
+int rneaDerivT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a, double* dq,
+               double* dv, double* M, double* tau, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv, ba;
+    const int nv = m.nv;
+    rneaDerivatives(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * nv, nv), ba.load(a + b * nv, nv));
+    if (dq) storeMat(data.dtau_dq, dq + b * nv * nv);
+    if (dv) storeMat(data.dtau_dv, dv + b * nv * nv);
+    if (M) storeMat(data.M, M + b * nv * nv);
+    if (tau) store(data.tau.data(), tau + b * nv, nv);
+  });
+}
+template<class S>
+int abaDerivT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau, double* dq,
+              double* dv, double* Minv, double* ddq, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv, bt;
+    const int nv = m.nv;
+    Mat<S> Xq(nv, nv), Xv(nv, nv), Mi(nv, nv);
+    abaDerivatives(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * nv, nv), bt.load(tau + b * nv, nv), Xq, Xv,
+                   Mi);
+    if (dq) storeMat(Xq, dq + b * nv * nv);
+    if (dv) storeMat(Xv, dv + b * nv * nv);
+    if (Minv) storeMat(Mi, Minv + b * nv * nv);
+    if (ddq) store(data.ddq.data(), ddq + b * nv, nv);
+  });
+}
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+int orc_dims(const ModelDesc* d, int* nq, int* nv, int* compact) {
+  try {
+    const Model<double> m(*d);
+    *nq = m.nq;
+    *nv = m.nv;
+    if (compact) *compact = m.isCompact() ? 1 : 0;
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Topology helpers as the reference's Data holds them (data.hxx:196-315).
+int orc_topology(const ModelDesc* d, int* idx_q, int* idx_v, int* nv_subtree, int* parents_fromRow,
+                 int* nvSubtree_fromRow) {
+  try {
+    const Model<double> m(*d);
+    for (int i = 0; i < m.njoints; ++i) {
+      idx_q[i] = m.idx_q[i];
+      idx_v[i] = m.idx_v[i];
+      nv_subtree[i] = m.nvSubtree[i];
+    }
+    for (int k = 0; k < m.nv; ++k) {
+      parents_fromRow[k] = m.parents_fromRow[k];
+      nvSubtree_fromRow[k] = m.nvSubtree_fromRow[k];
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int orc_rnea(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a, double* tau, int scalar,
+             int nthreads) {
+#define CALL(S) rneaT<S>(d, B, q, v, a, tau, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_aba(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau, double* ddq,
+            int convention, int scalar, int nthreads) {
+#define CALL(S) abaT<S>(d, B, q, v, tau, ddq, convention, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_crba(const ModelDesc* d, int64_t B, const double* q, double* M, int symmetrize, int scalar, int nthreads) {
+#define CALL(S) crbaT<S>(d, B, q, M, symmetrize, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+// M: full (or upper) nv x nv per config.  Any of U, D, Minv, x may be null; x is solved in place.
+int orc_cholesky(const ModelDesc* d, int64_t B, const double* M, double* U, double* D, double* Minv, double* x,
+                 int scalar, int nthreads) {
+#define CALL(S) choleskyT<S>(d, B, M, U, D, Minv, x, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_rnea_derivatives(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a,
+                         double* dtau_dq, double* dtau_dv, double* M, double* tau, int scalar, int nthreads) {
+#define CALL(S) rneaDerivT<S>(d, B, q, v, a, dtau_dq, dtau_dv, M, tau, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_aba_derivatives(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau,
+                        double* ddq_dq, double* ddq_dv, double* Minv, double* ddq, int scalar, int nthreads) {
+#define CALL(S) abaDerivT<S>(d, B, q, v, tau, ddq_dq, ddq_dv, Minv, ddq, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_integrate(const ModelDesc* d, int64_t B, const double* q, const double* v, double* qout) {
+  return runBatch<double>(d, B, 1, [&](const Model<double>& m, Data<double>&, int64_t b) {
+    integrate(m, q + b * m.nq, v + b * m.nv, qout + b * m.nq);
+  });
+}
+
+// Kinetic energy 0.5 v^T M v and potential energy -sum m g.c (energy.hpp), per config.
+int orc_energy(const ModelDesc* d, int64_t B, const double* q, const double* v, double* T, double* V, int scalar) {
+  if (scalar == 2)
+    return runBatch<long double>(d, B, 1, [&](const Model<long double>& m, Data<long double>& data, int64_t b) {
+      Buf<long double> bq, bv;
+      const long double* qq = bq.load(q + b * m.nq, m.nq);
+      if (T) T[b] = double(kineticEnergy(m, data, qq, bv.load(v + b * m.nv, m.nv)));
+      if (V) V[b] = double(potentialEnergy(m, data, qq));
+    });
+  return runBatch<double>(d, B, 1, [&](const Model<double>& m, Data<double>& data, int64_t b) {
+    if (T) T[b] = kineticEnergy(m, data, q + b * m.nq, v + b * m.nv);
+    if (V) V[b] = potentialEnergy(m, data, q + b * m.nq);
+  });
+}
+
+}  // extern "C"
+
+// ---- algorithmic operation counts (one configuration, flop_count.hpp) -----------------
+#include "flop_count.hpp"
+
+extern "C" int orc_flops(const ModelDesc* d, int alg, const double* q, const double* v, const double* x,
+                         long long* out) {
+  try {
+    using F = oracle::Flop;
+    const Model<F> m(*d);
+    Data<F> data(m);
+    std::vector<F> qq(q, q + m.nq), vv(v, v + m.nv), xx(x, x + m.nv), res(m.nv);
+    long long& n = F::count();
+    n = 0;
+    switch (alg) {
+      case 0: rnea(m, data, qq.data(), vv.data(), xx.data(), res.data()); break;
+      case 1: abaWorld(m, data, qq.data(), vv.data(), xx.data(), res.data()); break;
+      case 2: abaLocal(m, data, qq.data(), vv.data(), xx.data(), res.data()); break;
+      case 3: crbaWorld(m, data, qq.data()); break;
+      case 4: {
+        crbaWorld(m, data, qq.data());
+        n = 0;
+        choleskyDecompose(m, data);
+        break;
+      }
+      case 5: rneaDerivatives(m, data, qq.data(), vv.data(), xx.data()); break;
+      case 6: {
+        Mat<F> a(m.nv, m.nv), b(m.nv, m.nv), c(m.nv, m.nv);
+        abaDerivatives(m, data, qq.data(), vv.data(), xx.data(), a, b, c);
+        break;
+      }
+      case 7: {
+        crbaWorld(m, data, qq.data());
+        choleskyDecompose(m, data);
+        n = 0;
+        choleskySolve(m, data, xx.data());
+        break;
+      }
+      default: throw std::invalid_argument("unknown algorithm id");
+    }
+    *out = n;
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// ---- CPU baseline timing: the batch loop of rneaInParallel / abaInParallel, timed in C --
+// alg: 0 rnea, 1 aba (WORLD), 2 crba, 3 crba + cholesky::decompose, 4 aba + crba +
+// decompose (one step of the headline workload), 5 computeRNEADerivatives,
+// 6 computeABADerivatives.  Each thread owns one Data (ModelPool semantics) and writes
+// results into it; inputs are the caller's column-major double arrays.
+#include <chrono>
+extern "C" int orc_time(const ModelDesc* d, int alg, int64_t B, const double* q, const double* v, const double* x,
+                        int nthreads, int reps, double* seconds) {
+  try {
+    const Model<double> m(*d);
+    if ((alg >= 2 && alg <= 6) && !m.isCompact()) throw std::invalid_argument("model is not compact");
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    std::vector<double> times;
+    for (int r = 0; r < reps; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel num_threads(nt)
+      {
+        Data<double> data(m);
+        std::vector<double> out(m.nv);
+        Mat<double> A(m.nv, m.nv), Bm(m.nv, m.nv), C(m.nv, m.nv);
+#pragma omp for schedule(static)
+        for (int64_t b = 0; b < B; ++b) {
+          const double* qb = q + b * m.nq;
+          const double* vb = v + b * m.nv;
+          const double* xb = x + b * m.nv;
+          switch (alg) {
+            case 0: rnea(m, data, qb, vb, xb, out.data()); break;
+            case 1: abaWorld(m, data, qb, vb, xb, out.data()); break;
+            case 2: crbaWorld(m, data, qb); break;
+            case 3: crbaWorld(m, data, qb); choleskyDecompose(m, data); break;
+            case 4:
+              abaWorld(m, data, qb, vb, xb, out.data());
+              crbaWorld(m, data, qb);
+              choleskyDecompose(m, data);
+              break;
+            case 5: rneaDerivatives(m, data, qb, vb, xb); break;
+            case 6: abaDerivatives(m, data, qb, vb, xb, A, Bm, C); break;
+          }
+        }
+      }
+      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    std::sort(times.begin(), times.end());
+    *seconds = times[times.size() / 2];
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}

@@ -1,0 +1,345 @@
+"""pinocchio_amd -- MI355X batched rigid-body dynamics behind pinocchio's batch API.
+
+Python binding of libpinocchio_amd.so (C ABI in include/rbd.h).  Two levels:
+
+* the drop-in mirror of the reference's Python API for the batched path
+  (bindings/python/algorithm/parallel/{rnea,aba}.cpp): ``ModelPool``,
+  ``rneaInParallel(num_threads, pool, q, v, a)``, ``abaInParallel(num_threads, pool, q, v, tau)``
+  taking and returning numpy matrices with one configuration per column;
+* ``Engine``: device-resident calls on torch tensors (no host round trip), used by the
+  benchmarks and the GPU parity tests.
+
+There is no CPU fallback: if the library is missing or no GPU is present the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import models  # noqa: F401
+from .models import ModelDesc
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpinocchio_amd.so")
+
+RBD_F32, RBD_F64 = 0, 1
+RBD_HOST_PTRS, RBD_ASYNC, RBD_LAYOUT_BATCH, RBD_UPPER_ONLY = 0x1, 0x2, 0x4, 0x8
+_ERRORS = {1: ValueError, 2: NotImplementedError, 3: RuntimeError, 4: MemoryError}
+
+_lib = None
+
+
+def lib():
+    """Load libpinocchio_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C pinocchio_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.rbd_last_error.restype = ctypes.c_char_p
+        L.rbd_version.restype = ctypes.c_char_p
+        L.rbd_ctx_stream.restype = ctypes.c_void_p
+        L.rbd_ctx_stream.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().rbd_last_error().decode()
+        raise _ERRORS.get(rc, RuntimeError)(msg)
+
+
+def _i64(x):
+    return ctypes.c_int64(int(x))
+
+
+class Context:
+    """A HIP stream on one GPU (rbd_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().rbd_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @property
+    def stream(self) -> int:
+        return lib().rbd_ctx_stream(self.handle)
+
+    def synchronize(self):
+        _check(lib().rbd_ctx_synchronize(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.rbd_ctx_destroy(self.handle)
+            self.handle = None
+
+
+class Model:
+    """A model uploaded to a context (rbd_model)."""
+
+    def __init__(self, ctx: Context, desc: ModelDesc):
+        self.ctx = ctx
+        self.desc = desc
+        self._cdesc = desc.cstruct()
+        h = ctypes.c_void_p()
+        _check(lib().rbd_model_create(ctx.handle, ctypes.byref(self._cdesc), ctypes.byref(h)))
+        self.handle = h
+        nq, nv, nj, compact = (ctypes.c_int32() for _ in range(4))
+        _check(lib().rbd_model_dims(h, ctypes.byref(nq), ctypes.byref(nv), ctypes.byref(nj), ctypes.byref(compact)))
+        self.nq, self.nv, self.njoints, self.compact = nq.value, nv.value, nj.value, bool(compact.value)
+
+    def cholesky_pattern(self):
+        n = ctypes.c_int32()
+        _check(lib().rbd_model_cholesky_pattern(self.handle, ctypes.byref(n), None, None))
+        rows = np.zeros(n.value, np.int32)
+        cols = np.zeros(n.value, np.int32)
+        _check(lib().rbd_model_cholesky_pattern(self.handle, ctypes.byref(n), rows.ctypes.data_as(ctypes.c_void_p),
+                                                cols.ctypes.data_as(ctypes.c_void_p)))
+        return rows, cols
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.rbd_model_destroy(self.handle)
+            self.handle = None
+
+
+# ------------------------------------------------------------------------------------
+# Device-resident engine over torch tensors
+# ------------------------------------------------------------------------------------
+def _dtype_code(t):
+    import torch
+    if t.dtype == torch.float32:
+        return RBD_F32
+    if t.dtype == torch.float64:
+        return RBD_F64
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Engine:
+    """Batched dynamics on device tensors.
+
+    layout="col": tensors are (B, rows), one configuration per row of memory (the
+    Eigen column-per-configuration matrix, column major).  layout="batch": tensors are
+    (rows, B) with the batch index fastest (the engine's native layout).  Matrix outputs
+    are (B, nv, nv) for "col" (element [b, c, r] = M_b(r, c)) and (nv*nv, B) for "batch"
+    (row r + c*nv).
+    """
+
+    def __init__(self, model: Model, layout: str = "col"):
+        assert layout in ("col", "batch")
+        self.model = model
+        self.layout = layout
+
+    # helpers -----------------------------------------------------------------------
+    def _flags(self, async_=False):
+        f = RBD_LAYOUT_BATCH if self.layout == "batch" else 0
+        return f | (RBD_ASYNC if async_ else 0)
+
+    def _vec(self, t, rows, name):
+        # a dimension of extent 1 may carry any stride
+        if self.layout == "col":
+            ok = t.dim() == 2 and t.shape[1] == rows and (t.stride(1) == 1 or rows == 1)
+            assert ok, f"{name}: expected (B, {rows}) rows-contiguous"
+            return t.shape[0], (t.stride(0) if t.shape[0] > 1 else rows)
+        ok = t.dim() == 2 and t.shape[0] == rows and (t.stride(1) == 1 or t.shape[1] == 1)
+        assert ok, f"{name}: expected ({rows}, B) contiguous"
+        return t.shape[1], (t.stride(0) if rows > 1 else max(t.shape[1], 1))
+
+    def _new_vec(self, like, rows, B):
+        import torch
+        if self.layout == "col":
+            return torch.empty((B, rows), dtype=like.dtype, device=like.device)
+        return torch.empty((rows, B), dtype=like.dtype, device=like.device)
+
+    def _new_mat(self, like, B, dtype=None):
+        import torch
+        nv = self.model.nv
+        dt = dtype or like.dtype
+        if self.layout == "col":
+            return torch.zeros((B, nv, nv), dtype=dt, device=like.device)
+        return torch.zeros((nv * nv, B), dtype=dt, device=like.device)
+
+    def _mat_ld(self, t):
+        nv = self.model.nv
+        return nv * nv if self.layout == "col" else t.stride(0)
+
+    # algorithms --------------------------------------------------------------------
+    def rnea(self, q, v, a, out=None, async_=False):
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        _, lda = self._vec(a, m.nv, "a")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        _, ldo = self._vec(out, m.nv, "tau")
+        _check(lib().rbd_rnea(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v), _i64(ldv),
+                              _ptr(a), _i64(lda), _ptr(out), _i64(ldo), ctypes.c_uint32(self._flags(async_))))
+        return out
+
+    def aba(self, q, v, tau, out=None, async_=False):
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        _, ldt = self._vec(tau, m.nv, "tau")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        _, ldo = self._vec(out, m.nv, "ddq")
+        _check(lib().rbd_aba(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v), _i64(ldv),
+                             _ptr(tau), _i64(ldt), _ptr(out), _i64(ldo), ctypes.c_uint32(self._flags(async_))))
+        return out
+
+    def _mat_arg(self, t, B, name):
+        nv = self.model.nv
+        if self.layout == "col":
+            assert t.shape == (B, nv, nv) and t.is_contiguous(), f"{name}: expected contiguous (B, nv, nv)"
+            return nv * nv
+        assert t.dim() == 2 and t.shape[0] == nv * nv and t.shape[1] == B and t.stride(1) == 1, f"{name}: (nv*nv, B)"
+        return t.stride(0) if B > 0 else max(B, 1)
+
+    def crba(self, q, out=None, upper_only=False, async_=False):
+        """Joint-space inertia matrix per configuration (symmetric unless upper_only)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        out = self._new_mat(q, B) if out is None else out
+        ldm = self._mat_arg(out, B, "M")
+        flags = self._flags(async_) | (RBD_UPPER_ONLY if upper_only else 0)
+        _check(lib().rbd_crba(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(out), _i64(ldm),
+                              ctypes.c_uint32(flags)))
+        return out
+
+    def cholesky_decompose(self, M, async_=False):
+        """FP64 factor M = U D U^T; returns (U_packed, D): (B, nnz) and (B, nv) for "col",
+        (nnz, B) and (nv, B) for "batch"."""
+        import torch
+        m = self.model
+        B = M.shape[0] if self.layout == "col" else M.shape[1]
+        ldm = self._mat_arg(M, B, "M")
+        nnz = len(self.model.cholesky_pattern()[0])
+        if self.layout == "col":
+            U = torch.zeros((B, max(nnz, 1)), dtype=torch.float64, device=M.device)
+            D = torch.zeros((B, m.nv), dtype=torch.float64, device=M.device)
+            ldu, ldd = max(nnz, 1), m.nv
+        else:
+            U = torch.zeros((max(nnz, 1), B), dtype=torch.float64, device=M.device)
+            D = torch.zeros((m.nv, B), dtype=torch.float64, device=M.device)
+            ldu = ldd = max(B, 1)
+        _check(lib().rbd_cholesky_decompose(m.ctx.handle, m.handle, _i64(B), _dtype_code(M), _ptr(M), _i64(ldm),
+                                            _ptr(U), _i64(ldu), _ptr(D), _i64(ldd),
+                                            ctypes.c_uint32(self._flags(async_))))
+        return U, D
+
+    def cholesky_solve(self, U, D, x, async_=False):
+        """x <- M^-1 x in place with a factor from cholesky_decompose."""
+        m = self.model
+        B, ldx = self._vec(x, m.nv, "x")
+        ldu = U.stride(0) if self.layout == "col" else U.stride(0)
+        ldd = D.stride(0)
+        _check(lib().rbd_cholesky_solve(m.ctx.handle, m.handle, _i64(B), _ptr(U), _i64(ldu), _ptr(D), _i64(ldd),
+                                        _dtype_code(x), _ptr(x), _i64(ldx), ctypes.c_uint32(self._flags(async_))))
+        return x
+
+    def rnea_derivatives(self, q, v, a, async_=False):
+        """(dtau_dq, dtau_dv, M) per configuration (computeRNEADerivatives)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        _, lda = self._vec(a, m.nv, "a")
+        outs = [self._new_mat(q, B) for _ in range(3)]
+        ldo = self._mat_arg(outs[0], B, "out")
+        _check(lib().rbd_rnea_derivatives(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v),
+                                          _i64(ldv), _ptr(a), _i64(lda), *[_ptr(o) for o in outs], _i64(ldo),
+                                          ctypes.c_uint32(self._flags(async_))))
+        return tuple(outs)
+
+    def aba_derivatives(self, q, v, tau, async_=False):
+        """(ddq_dq, ddq_dv, Minv, ddq) per configuration (computeABADerivatives)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        _, ldt = self._vec(tau, m.nv, "tau")
+        outs = [self._new_mat(q, B) for _ in range(3)]
+        ldo = self._mat_arg(outs[0], B, "out")
+        ddq = self._new_vec(q, m.nv, B)
+        _, ldd = self._vec(ddq, m.nv, "ddq")
+        _check(lib().rbd_aba_derivatives(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v),
+                                         _i64(ldv), _ptr(tau), _i64(ldt), *[_ptr(o) for o in outs], _i64(ldo),
+                                         _ptr(ddq), _i64(ldd), ctypes.c_uint32(self._flags(async_))))
+        return outs[0], outs[1], outs[2], ddq
+
+    @staticmethod
+    def mats_to_numpy(t, layout, nv):
+        """Device matrix output -> (B, nv, nv) numpy array with [b] the configuration's matrix."""
+        a = t.double().cpu().numpy()
+        if layout == "col":
+            return a.transpose(0, 2, 1)  # [b, c, r] -> [b, r, c]
+        B = a.shape[1]
+        return a.reshape(nv, nv, B).transpose(2, 1, 0)  # [c, r, b] -> [b, r, c]
+
+
+# ------------------------------------------------------------------------------------
+# Drop-in mirror of the reference's Python batch API
+# ------------------------------------------------------------------------------------
+class ModelPool:
+    """Counterpart of pinocchio.ModelPool (multibody/pool/model.hpp:19-165): holds the
+    model once on the GPU instead of one copy per OpenMP thread; `size` is kept for the
+    reference's argument checks."""
+
+    def __init__(self, model_desc: ModelDesc, size: int = 1, device: int = 0):
+        if size < 1:
+            raise ValueError("The pool size must be positive")
+        self.ctx = Context(device)
+        self.model = Model(self.ctx, model_desc)
+        self._size = size
+
+    def size(self) -> int:
+        return self._size
+
+
+def _host_call(fn_name, num_threads, pool, q, v, x, dtype):
+    """Shared argument checks of algorithm/parallel/{rnea,aba}.hpp:53-69 and the call."""
+    m = pool.model
+    if pool.size() == 0:
+        raise ValueError("The pool should have at least one element")
+    if num_threads > pool.size():
+        raise ValueError("The number of threads should be lower than the pool size")
+    q = np.asfortranarray(np.asarray(q, dtype=dtype))
+    v = np.asfortranarray(np.asarray(v, dtype=dtype))
+    x = np.asfortranarray(np.asarray(x, dtype=dtype))
+    if q.ndim != 2 or q.shape[0] != m.nq:
+        raise ValueError("The configuration matrix has wrong number of rows")
+    if v.ndim != 2 or v.shape[0] != m.nv:
+        raise ValueError("The velocity matrix has wrong number of rows")
+    if x.ndim != 2 or x.shape[0] != m.nv:
+        raise ValueError("The third input matrix has wrong number of rows")
+    B = q.shape[1]
+    if v.shape[1] != B or x.shape[1] != B:
+        raise ValueError("The input matrices do not have the same number of columns")
+    out = np.zeros((m.nv, B), dtype=dtype, order="F")
+    code = RBD_F64 if dtype == np.float64 else RBD_F32
+    if B == 0:
+        return out.astype(np.float64)
+    f = getattr(lib(), fn_name)
+    _check(f(pool.ctx.handle, m.handle, _i64(B), code, q.ctypes.data_as(ctypes.c_void_p), _i64(m.nq),
+             v.ctypes.data_as(ctypes.c_void_p), _i64(m.nv), x.ctypes.data_as(ctypes.c_void_p), _i64(m.nv),
+             out.ctypes.data_as(ctypes.c_void_p), _i64(m.nv), ctypes.c_uint32(RBD_HOST_PTRS)))
+    return out.astype(np.float64)
+
+
+def rneaInParallel(num_threads, pool, q, v, a, dtype=np.float64):
+    """tau = rnea(q[:, i], v[:, i], a[:, i]) for each column (algorithm/parallel/rnea.hpp:39-83)."""
+    return _host_call("rbd_rnea", num_threads, pool, q, v, a, dtype)
+
+
+def abaInParallel(num_threads, pool, q, v, tau, dtype=np.float32):
+    """ddq = aba(q[:, i], v[:, i], tau[:, i]) for each column (algorithm/parallel/aba.hpp:40-84).
+
+    The engine's forward dynamics computes in FP32 (inputs are rounded, the result is
+    returned as float64 like the reference's Eigen::MatrixXd)."""
+    return _host_call("rbd_aba", num_threads, pool, q, v, tau, dtype)

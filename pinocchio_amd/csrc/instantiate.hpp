@@ -1,0 +1,93 @@
+// instantiate.hpp -- per-topology kernel instantiation and launcher tables.
+#pragma once
+
+#include "ops.hpp"
+#include "k_dyn.hpp"
+#include "k_crba.hpp"
+#include "k_deriv.hpp"
+#include "_gen/topologies.inc"
+
+namespace rbd {
+
+inline unsigned gridFor(int64_t B) { return (unsigned)((B + kTile - 1) / kTile); }
+
+template<class TP, class S>
+void launchRnea(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> out) {
+  hipLaunchKernelGGL((k_rnea<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, out);
+}
+template<class TP, class S>
+void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
+  hipLaunchKernelGGL((k_aba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, tau, out);
+}
+template<class TP, class S>
+void launchCrba(hipStream_t st, const S* mc, int64_t B, CView<S> q, View<S> M, int upper) {
+  hipLaunchKernelGGL((k_crba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, M, upper);
+}
+template<class TP, class S> void launchChol(hipStream_t st, int64_t B, CView<S> M, View<double> U, View<double> D) {
+  hipLaunchKernelGGL((k_chol<TP, S, double>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
+}
+template<class TP> void launchChol32(hipStream_t st, int64_t B, CView<float> M, View<float> U, View<float> D) {
+  hipLaunchKernelGGL((k_chol<TP, float, float>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
+}
+template<class TP, class S>
+void launchCholSolve(hipStream_t st, int64_t B, CView<double> U, CView<double> D, View<S> x) {
+  hipLaunchKernelGGL((k_chol64_solve<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, U, D, x);
+}
+template<class TP, class S>
+void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
+                 View<S> M, int mask) {
+  hipLaunchKernelGGL((k_rnead<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, dq, dv, M, mask);
+}
+template<class TP, class S>
+void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv) {
+  hipLaunchKernelGGL((k_abad_solve<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, U, D, Xq, Xv, Minv);
+}
+template<class TP> constexpr int nnzUpper() {
+  int n = 0;
+  for (int c = 0; c < TP::T.nv; ++c)
+    for (int x = TP::T.dofParent[c]; x >= 0; x = TP::T.dofParent[x]) ++n;
+  return n;
+}
+
+template<class TP> TopoOps makeOpsBase() {
+  TopoOps o{};
+  o.name = TP::name;
+  o.njoints = TP::NJ;
+  o.parent = TP::parent;
+  o.type = TP::type;
+  o.nq = TP::T.nq;
+  o.nv = TP::T.nv;
+  o.compact = TP::T.compact ? 1 : 0;
+  return o;
+}
+
+// Kernel families, one translation unit each so that a topology's kernels compile in
+// parallel.  "dyn" (RNEA, ABA) exists for every topology; the others need a compact
+// (DFS-ordered) tree.
+template<class TP> void add_dyn(TopoOps& o) {
+  o.rnea_f32 = &launchRnea<TP, float>;
+  o.rnea_f64 = &launchRnea<TP, double>;
+  o.aba_f32 = &launchAba<TP, float>;
+}
+template<class TP> void add_crba(TopoOps& o) {
+  o.crba_f32 = &launchCrba<TP, float>;
+  o.crba_f64 = &launchCrba<TP, double>;
+}
+template<class TP> void add_chol(TopoOps& o) {
+  o.chol_f32 = &launchChol<TP, float>;
+  o.chol_f64 = &launchChol<TP, double>;
+  o.cholsolve_f32 = &launchCholSolve<TP, float>;
+  o.cholsolve_f64 = &launchCholSolve<TP, double>;
+  o.chol32_f32 = &launchChol32<TP>;
+  o.nnzU = nnzUpper<TP>();
+}
+template<class TP> void add_rnead(TopoOps& o) { o.rnead_f32 = &launchRneaD<TP, float>; }
+template<class TP> void add_abads(TopoOps& o) { o.abad_solve_f32 = &launchAbaDSolve<TP, float>; }
+
+}  // namespace rbd
+
+// one kernel family of one topology
+#define RBD_FILL(NAME, FAM) \
+  namespace rbd {           \
+  void fill_##NAME##_##FAM(TopoOps& o) { add_##FAM<Topo_##NAME>(o); } \
+  }

@@ -1,0 +1,285 @@
+// k_crba.hpp -- batched joint-space inertia matrix (CRBA) and tree-sparse Cholesky.
+//
+// CRBA follows crbaWorldConvention (crba.hxx:20-99, 498-548): world-frame joint axes
+// J_c = oMi.act(S_c) and composite inertias Ycrb (additive in the world frame).  Instead of
+// the reference's row blocks M[i, subtree(i)] = J_i^T Ycrb_i J[:, subtree] (which keep
+// a 6 x nv Ag matrix), the column of each dof c is produced when its subtree closes:
+// F_c = Ycrb_j J_c and M[r, c] = J_r . F_c for every ancestor-or-self dof r -- the same
+// products, grouped so that only the open path (oMi per joint) is live.
+//
+// cholesky::decompose (cholesky.hxx:17-62) runs in FP64 as a right-looking elimination
+// over the tree pattern: column j (from nv-1 down to 0) yields D_j = A_jj and
+// U_rj = A_rj / D_j for its ancestor rows, then updates A_rr' -= U_rj A_r'j for ancestor
+// pairs.  A entries are loaded lazily on first touch, so the live set is the pattern of
+// the open path, not the whole matrix.
+#pragma once
+
+#include "kernels_common.hpp"
+
+namespace rbd {
+
+// world-frame axis of dof k (0..nvj-1) of joint i with world transform oM
+template<int TY, class S> __device__ __forceinline__ Mot<S> worldAxis(const Xform<S>& oM, int k) {
+  if constexpr (TY == JT_FF) {
+    if (k < 3) return {col(oM.R, k), {S(0), S(0), S(0)}};
+    const V3<S> w = col(oM.R, k - 3);
+    return {cross(oM.p, w), w};
+  } else {
+    const V3<S> w = col(oM.R, TY - JT_RX);
+    return {cross(oM.p, w), w};
+  }
+}
+
+// composite inertia of body j expressed in the world (origin form)
+template<class S> __device__ __forceinline__ CInertia<S> worldInertia(const S* __restrict__ mc, int j, const Xform<S>& oM) {
+  return transportUp(oM, mcComposite(mc, j));
+}
+
+template<class TP> struct CrbaPattern {
+  static constexpr auto& T = TP::T;
+  static constexpr int NV = T.nv;
+  // structural non-zero of M: r ancestor-or-self of c (dof tree)
+  static constexpr bool nz(int r, int c) {
+    if (r > c) return nz(c, r);
+    for (int x = c; x >= 0; x = T.dofParent[x])
+      if (x == r) return true;
+    return false;
+  }
+};
+
+template<class S> __device__ __forceinline__ void stMat(View<S> g, int64_t b, int nv, int r, int c, S x) {
+  g.p[(int64_t)(r + c * nv) * g.rs + b * g.bs] = x;
+}
+
+// ---------------------------------------------------------------------------------------
+// CRBA: M(q), symmetric (or upper row blocks only), armature on the diagonal
+// ---------------------------------------------------------------------------------------
+template<class TP, class S>
+__global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_t B, CView<S> gq, View<S> gM,
+                                             int upperOnly) {
+  constexpr auto& T = TP::T;
+  constexpr int NJ = TP::NJ, NQ = T.nq, NV = T.nv;
+  __shared__ S sq[NQ * kPitch];
+  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
+  stageIn<NQ>(sq, gq, b0, nb);
+  __syncthreads();
+  const int L = laneId();
+  const int64_t b = b0 + L;
+  const bool live = L < nb;
+
+  Xform<S> oM[NJ];
+  CInertia<S> Yacc[NJ];
+
+  static_for<0, T.nev>([&](auto E) {
+    constexpr int e = E;
+    stepFence();
+    const S* __restrict__ mc = stepConsts(mc_in);
+    constexpr int i = TP::T.evJoint[e];
+    constexpr int p = TP::T.parent[i];
+    constexpr int ty = TP::T.type[i];
+    constexpr int iq = TP::T.iq[i], iv = TP::T.iv[i], nvi = TP::T.nvj[i];
+    if constexpr (TP::T.evKind[e] == EV_PRE) {
+      Xform<S> X;
+      if constexpr (ty == JT_FF) {
+        S q7[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) q7[k] = sq[(iq + k) * kPitch + L];
+        X = freeflyerX(mc, i, q7, false);
+      } else {
+        S sn, cs;
+        sincosT(sq[iq * kPitch + L], &sn, &cs);
+        X = revoluteX(mc, i, ty - JT_RX, sn, cs);
+      }
+      if constexpr (p > 0) oM[i] = compose(oM[p], X);
+      else oM[i] = X;
+    } else {
+      CInertia<S> Y = worldInertia(mc, i, oM[i]);
+      if constexpr (TP::T.nchild[i] > 0) cadd(Y, Yacc[i]);
+#pragma unroll
+      for (int k = 0; k < nvi; ++k) {
+        const int c = iv + k;
+        const Frc<S> F = cmul(Y, worldAxis<ty>(oM[i], k));
+        // own block (all dofs of this joint)
+#pragma unroll
+        for (int k2 = 0; k2 < nvi; ++k2) {
+          const int r = iv + k2;
+          S m = dot(worldAxis<ty>(oM[i], k2), F);
+          if (r == c) m += mc[mcArmature<NJ>() + c];
+          if (live && (!upperOnly || r <= c || ty == JT_FF)) stMat(gM, b, NV, r, c, m);
+        }
+        // ancestors
+        static_for<1, NJ>([&](auto A) {
+          constexpr int a = A;
+          if constexpr (a != i && TP::T.pos[a] < TP::T.pos[i] && TP::T.pos[a] + TP::T.subsize[a] > TP::T.pos[i]) {
+            constexpr int tya = TP::T.type[a];
+#pragma unroll
+            for (int k2 = 0; k2 < TP::T.nvj[a]; ++k2) {
+              const int r = TP::T.iv[a] + k2;
+              const S m = dot(worldAxis<tya>(oM[a], k2), F);
+              if (live) {
+                stMat(gM, b, NV, r, c, m);
+                if (!upperOnly) stMat(gM, b, NV, c, r, m);
+              }
+            }
+          }
+        });
+      }
+      if constexpr (p > 0) {
+        if constexpr (TP::T.childIdx[i] == 0) Yacc[p] = Y;
+        else cadd(Yacc[p], Y);
+      }
+    }
+  });
+  // structural zeros
+  if (live && !upperOnly) {
+    static_for<0, NV>([&](auto C) {
+      static_for<0, NV>([&](auto R) {
+        if constexpr (!CrbaPattern<TP>::nz(R, C)) stMat(gM, b, NV, R, C, S(0));
+      });
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 tree-sparse Cholesky M = U D U^T (cholesky::decompose), M read from its upper
+// structural entries (any dtype), U written packed (strictly-upper pattern, column by
+// column, rows from the root down) and D as nv doubles.
+// ---------------------------------------------------------------------------------------
+template<class TP> struct CholPattern {
+  static constexpr auto& T = TP::T;
+  static constexpr int NV = T.nv;
+  static constexpr int depthOf(int c) {
+    int d = 0;
+    for (int x = T.dofParent[c]; x >= 0; x = T.dofParent[x]) ++d;
+    return d;
+  }
+  // packed index of (r, c), r a strict ancestor of c; rows listed from the root down
+  static constexpr int packed(int r, int c) {
+    int base = 0;
+    for (int k = 0; k < c; ++k) base += depthOf(k);
+    int pos = 0;
+    for (int x = T.dofParent[c]; x >= 0; x = T.dofParent[x])
+      if (x > r) ++pos;
+    return base + (depthOf(c) - 1 - pos);
+  }
+  static constexpr bool anc(int r, int c) {  // r strict ancestor of c
+    for (int x = T.dofParent[c]; x >= 0; x = T.dofParent[x])
+      if (x == r) return true;
+    return false;
+  }
+  // first column (in elimination order nv-1 .. 0) that touches A[r][c] (r <= c):
+  // the largest k >= c in the subtree of c (for r < c, k must also descend from r)
+  static constexpr int firstTouch(int r, int c) {
+    int best = c;
+    for (int k = NV - 1; k > c; --k)
+      if (anc(c, k)) {
+        best = k;
+        break;
+      }
+    (void)r;
+    return best;
+  }
+};
+
+// SC is the arithmetic (and output) type of the factor: double for cholesky::decompose,
+// float inside the ABA-derivative pipeline.
+template<class TP, class SIN, class SC>
+__global__ __launch_bounds__(64) void k_chol(int64_t B, CView<SIN> gM, View<SC> gU, View<SC> gD) {
+  constexpr auto& T = TP::T;
+  constexpr int NV = T.nv;
+  using P = CholPattern<TP>;
+  const int L = laneId();
+  const int64_t b = int64_t(blockIdx.x) * kTile + L;
+  if (b >= B) return;
+  SC A[NV][NV];  // only pattern entries (r <= c, r ancestor-or-self of c) are materialised
+  auto ldM = [&](int r, int c) { return SC(gM.p[(int64_t)(r + c * NV) * gM.rs + b * gM.bs]); };
+  static_for<0, NV>([&](auto JJ) {
+    constexpr int j = NV - 1 - JJ;
+    stepFence();
+    // lazily load the column-j entries and the pairs this column updates
+    if constexpr (P::firstTouch(j, j) == j) A[j][j] = ldM(j, j);
+    static_for<0, NV>([&](auto RR) {
+      constexpr int r = RR;
+      if constexpr (P::anc(r, j) && P::firstTouch(r, j) == j) A[r][j] = ldM(r, j);
+    });
+    const SC Dj = A[j][j];
+    const SC Dinv = SC(1) / Dj;
+    gD.p[(int64_t)j * gD.rs + b * gD.bs] = Dj;
+    SC Uc[NV];
+    static_for<0, NV>([&](auto RR) {
+      constexpr int r = RR;
+      if constexpr (P::anc(r, j)) {
+        Uc[r] = A[r][j] * Dinv;
+        constexpr int pk = P::packed(r, j);
+        gU.p[(int64_t)pk * gU.rs + b * gU.bs] = Uc[r];
+      }
+    });
+    // rank-1 update of the ancestor block: A[r][r2] -= U[r] * A[r2][j]  (r <= r2)
+    static_for<0, NV>([&](auto R2) {
+      constexpr int r2 = R2;
+      if constexpr (P::anc(r2, j)) {
+        // A[r2][r2] and A[r][r2] for r ancestor of r2 (both ancestors of j)
+        if constexpr (P::firstTouch(r2, r2) == j) A[r2][r2] = ldM(r2, r2);
+        A[r2][r2] -= Uc[r2] * A[r2][j];
+        static_for<0, NV>([&](auto R1) {
+          constexpr int r1 = R1;
+          if constexpr (P::anc(r1, r2)) {
+            if constexpr (P::firstTouch(r1, r2) == j) A[r1][r2] = ldM(r1, r2);
+            A[r1][r2] -= Uc[r1] * A[r2][j];
+          }
+        });
+      }
+    });
+  });
+}
+
+// ---------------------------------------------------------------------------------------
+// x <- M^-1 x with a packed FP64 factor (cholesky::solve = Uiv, D^-1, Utiv,
+// cholesky.hxx:193-317, 466-520); arithmetic in FP64, x in S.
+// ---------------------------------------------------------------------------------------
+template<class TP, class S>
+__global__ __launch_bounds__(64) void k_chol64_solve(int64_t B, CView<double> gU, CView<double> gD, View<S> gx) {
+  constexpr auto& T = TP::T;
+  constexpr int NV = T.nv;
+  using P = CholPattern<TP>;
+  const int L = laneId();
+  const int64_t b = int64_t(blockIdx.x) * kTile + L;
+  if (b >= B) return;
+  double x[NV];
+  S* __restrict__ px = gx.p + b * gx.bs;
+  const double* __restrict__ pU = gU.p + b * gU.bs;
+  const double* __restrict__ pD = gD.p + b * gD.bs;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) x[k] = double(px[k * gx.rs]);
+  static_for<0, NV>([&](auto KK) {
+    constexpr int k = NV - 1 - KK;
+    stepFence();
+    const double* __restrict__ pUk = laneLaunder(pU);
+    static_for<k + 1, NV>([&](auto EE) {
+      constexpr int e2 = EE;
+      if constexpr (P::anc(k, e2)) {
+        constexpr int pk = P::packed(k, e2);
+        x[k] -= pUk[pk * gU.rs] * x[e2];
+      }
+    });
+  });
+#pragma unroll
+  for (int k = 0; k < NV; ++k) x[k] /= pD[k * gD.rs];
+  static_for<0, NV>([&](auto KK) {
+    constexpr int k = KK;
+    stepFence();
+    const double* __restrict__ pUk = laneLaunder(pU);
+    static_for<k + 1, NV>([&](auto EE) {
+      constexpr int e2 = EE;
+      if constexpr (P::anc(k, e2)) {
+        constexpr int pk = P::packed(k, e2);
+        x[e2] -= pUk[pk * gU.rs] * x[k];
+      }
+    });
+  });
+#pragma unroll
+  for (int k = 0; k < NV; ++k) px[k * gx.rs] = S(x[k]);
+}
+
+}  // namespace rbd

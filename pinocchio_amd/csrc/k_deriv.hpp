@@ -1,0 +1,330 @@
+// k_deriv.hpp -- batched analytical derivatives of RNEA and ABA.
+//
+// computeRNEADerivatives (rnea-derivatives.hxx:242-541) works in the world frame with
+// per-dof column data J, dV/dq, dA/dq, dA/dv and per-body composite quantities Ycrb,
+// doYcrb (= Ycrb.variation(ov) + [oh x*], inertia.hpp:750-787 and
+// rnea-derivatives.hxx:324-333) and of.  The reference fills row blocks
+// (idx_v, subtree) and column blocks (subtree+, idx_v) when it reaches the ancestor; here
+// every entry pairing an ancestor dof r with a dof c is produced when c's subtree closes,
+// from c's column forces (dFda_c, dFdq_c, dFdv_c, dYtJ_c) and r's column data, so the
+// column forces never have to be stored:
+//   dtau_dq[r][c] = J_r . (dFdq_c + J_c x* of_c)   dtau_dq[c][r] = dFda_c . dAdq_r + dYtJ_c . dVdq_r
+//   dtau_dv[r][c] = J_r . dFdv_c                    dtau_dv[c][r] = dFda_c . dAdv_r + dYtJ_c . J_r
+//   M[r][c] = M[c][r] = J_r . dFda_c
+// and the entries inside one joint's block use dFdq_c before the J_c x* of_c term, as the
+// reference does.
+//
+// computeABADerivatives (aba-derivatives.hxx:380-453) returns ddq_dq = -Minv dtau_dq,
+// ddq_dv = -Minv dtau_dv, ddq_dtau = Minv with dtau evaluated at a = ddq (the identity
+// its own Bwd2 pass implements).  The engine runs ABA (k_aba), this RNEA-derivative
+// kernel at a = ddq, then k_abad_solve, which factors M = U D U^T over the tree pattern in
+// registers and applies M^-1 to the 2 nv derivative columns and the nv unit columns.
+// FP32 throughout: an FP32 tree factor of the FP32 M gives Minv within 3.2e-6 of FP64 on
+// the humanoid, against 2.8e-5 for the reference's FP32 Fcrb sweep.
+#pragma once
+
+#include "k_crba.hpp"
+
+namespace rbd {
+
+// time variation of a world-frame body inertia (origin form m, h, Io) along ov, plus the
+// [oh x*] terms: the blocks (LIN,ANG), (ANG,LIN), (ANG,ANG); (LIN,LIN) is zero.
+template<class S> struct DY {
+  R3<S> LA, AL, AA;
+};
+template<class S> RBD_HD R3<S> skewM(V3<S> v) { return {S(0), -v.z, v.y, v.z, S(0), -v.x, -v.y, v.x, S(0)}; }
+// [u]x [w]x = w u^T - (u.w) I
+template<class S> RBD_HD R3<S> skewSq(V3<S> u, V3<S> w) {
+  const S d = dot(u, w);
+  return {w.x * u.x - d, w.x * u.y, w.x * u.z, w.y * u.x, w.y * u.y - d, w.y * u.z, w.z * u.x, w.z * u.y, w.z * u.z - d};
+}
+template<class S> RBD_HD R3<S> add(const R3<S>& a, const R3<S>& b) {
+  return {a.m00 + b.m00, a.m01 + b.m01, a.m02 + b.m02, a.m10 + b.m10, a.m11 + b.m11,
+          a.m12 + b.m12, a.m20 + b.m20, a.m21 + b.m21, a.m22 + b.m22};
+}
+template<class S> RBD_HD R3<S> sub(const R3<S>& a, const R3<S>& b) {
+  return {a.m00 - b.m00, a.m01 - b.m01, a.m02 - b.m02, a.m10 - b.m10, a.m11 - b.m11,
+          a.m12 - b.m12, a.m20 - b.m20, a.m21 - b.m21, a.m22 - b.m22};
+}
+template<class S> RBD_HD R3<S> transpose(const R3<S>& a) {
+  return {a.m00, a.m10, a.m20, a.m01, a.m11, a.m21, a.m02, a.m12, a.m22};
+}
+template<class S> RBD_HD R3<S> scaleM(S s, const R3<S>& a) {
+  return {s * a.m00, s * a.m01, s * a.m02, s * a.m10, s * a.m11, s * a.m12, s * a.m20, s * a.m21, s * a.m22};
+}
+template<class S> RBD_HD DY<S> inertiaVariation(const CInertia<S>& Y, const Mot<S>& v, const Frc<S>& oh) {
+  const R3<S> Io = {Y.xx, Y.xy, Y.xz, Y.xy, Y.yy, Y.yz, Y.xz, Y.yz, Y.zz};
+  const R3<S> W = skewM(v.a);
+  // inertia.hpp:750-787 in origin form: m c = h
+  R3<S> LA = sub(add(scaleM(-Y.m, skewM(v.l)), skewSq(Y.h, v.a)), skewSq(v.a, Y.h));
+  R3<S> AA = sub(sub(scaleM(S(-1), skewSq(v.l, Y.h)), skewSq(Y.h, v.l)), mul(Io, W));
+  AA = add(AA, mul(W, Io));
+  const R3<S> AL = transpose(LA);
+  const R3<S> Fl = skewM(-oh.l), Fa = skewM(-oh.a);
+  return {add(LA, Fl), add(AL, Fl), add(AA, Fa)};
+}
+template<class S> RBD_HD void dyAdd(DY<S>& a, const DY<S>& b) {
+  a.LA = add(a.LA, b.LA);
+  a.AL = add(a.AL, b.AL);
+  a.AA = add(a.AA, b.AA);
+}
+// D x with D = [[0, LA], [AL, AA]]
+template<class S> RBD_HD Frc<S> dyMul(const DY<S>& D, const Mot<S>& x) {
+  return {mul(D.LA, x.a), mul(D.AL, x.l) + mul(D.AA, x.a)};
+}
+// D^T x
+template<class S> RBD_HD Frc<S> dyMulT(const DY<S>& D, const Mot<S>& x) {
+  return {mulT(D.AL, x.a), mulT(D.LA, x.l) + mulT(D.AA, x.a)};
+}
+template<class S> RBD_HD S dotFF(const Frc<S>& a, const Mot<S>& b) { return dot(a.l, b.l) + dot(a.a, b.a); }
+
+template<class S> struct ColData {
+  Mot<S> J, dVdq, dAdq, dAdv;
+};
+
+// ---------------------------------------------------------------------------------------
+// computeRNEADerivatives: dtau_dq, dtau_dv, M (any may be skipped) and optionally tau
+// ---------------------------------------------------------------------------------------
+template<class TP, class S>
+__global__ __launch_bounds__(64) void k_rnead(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
+                                              CView<S> ga, View<S> gdq, View<S> gdv, View<S> gM, int writeMask) {
+  constexpr auto& T = TP::T;
+  constexpr int NJ = TP::NJ, NQ = T.nq, NV = T.nv;
+  __shared__ S sq[NQ * kPitch];
+  __shared__ S sv[NV * kPitch];
+  __shared__ S sa[NV * kPitch];
+  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
+  stageIn<NQ>(sq, gq, b0, nb);
+  stageIn<NV>(sv, gv, b0, nb);
+  stageIn<NV>(sa, ga, b0, nb);
+  __syncthreads();
+  const int L = laneId();
+  const int64_t b = b0 + L;
+  const bool live = L < nb;
+  const bool wq = live && (writeMask & 1), wv = live && (writeMask & 2), wM = live && (writeMask & 4);
+  const S* gr = mc_in + mcGravity<NJ, NV>();
+  const Mot<S> g = {{gr[0], gr[1], gr[2]}, {gr[3], gr[4], gr[5]}};
+  const Mot<S> a0 = {{-g.l.x, -g.l.y, -g.l.z}, {-g.a.x, -g.a.y, -g.a.z}};
+
+  Xform<S> oM[NJ];
+  Mot<S> ov[NJ], oa[NJ];  // world velocity and gravity-shifted acceleration
+  ColData<S> cd[NJ];      // revolute joints only
+  CInertia<S> Yacc[NJ];
+  DY<S> DYacc[NJ];
+  Frc<S> Facc[NJ];
+
+  // column data of dof k of joint a (the root free-flyer's is derived on the fly)
+  auto colData = [&](auto A, int k) -> ColData<S> {
+    constexpr int a = decltype(A)::value;
+    if constexpr (TP::T.type[a] == JT_FF) {
+      const Mot<S> J = worldAxis<JT_FF>(oM[a], k);
+      return {J, zeroMot<S>(), crossMM(a0, J), crossMM(ov[a], J)};
+    } else {
+      return cd[a];
+    }
+  };
+
+  static_for<0, T.nev>([&](auto E) {
+    constexpr int e = E;
+    stepFence();
+    const S* __restrict__ mc = stepConsts(mc_in);
+    constexpr int i = TP::T.evJoint[e];
+    constexpr int p = TP::T.parent[i];
+    constexpr int ty = TP::T.type[i];
+    constexpr int iq = TP::T.iq[i], iv = TP::T.iv[i], nvi = TP::T.nvj[i];
+    if constexpr (TP::T.evKind[e] == EV_PRE) {
+      if constexpr (ty == JT_FF) {
+        static_assert(p == 0, "free-flyer joints are supported at the root only");
+        S q7[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) q7[k] = sq[(iq + k) * kPitch + L];
+        oM[i] = freeflyerX(mc, i, q7, false);
+        Mot<S> vJ = zeroMot<S>(), aJ = zeroMot<S>();
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const Mot<S> J = worldAxis<JT_FF>(oM[i], k);
+          vJ = vJ + scale(sv[(iv + k) * kPitch + L], J);
+          aJ = aJ + scale(sa[(iv + k) * kPitch + L], J);
+        }
+        ov[i] = vJ;
+        oa[i] = a0 + aJ;
+      } else {
+        S sn, cs;
+        sincosT(sq[iq * kPitch + L], &sn, &cs);
+        const Xform<S> X = revoluteX(mc, i, ty - JT_RX, sn, cs);
+        if constexpr (p > 0) oM[i] = compose(oM[p], X);
+        else oM[i] = X;
+        const Mot<S> J = worldAxis<ty>(oM[i], 0);
+        const S qd = sv[iv * kPitch + L], qdd = sa[iv * kPitch + L];
+        const Mot<S> vJ = scale(qd, J);
+        if constexpr (p > 0) {
+          ov[i] = ov[p] + vJ;
+          oa[i] = oa[p] + crossMM(ov[p], vJ) + scale(qdd, J);
+          const Mot<S> dVdq = crossMM(ov[p], J);
+          cd[i] = {J, dVdq, crossMM(oa[p], J) + crossMM(ov[p], dVdq), crossMM(ov[i], J) + dVdq};
+        } else {
+          ov[i] = vJ;
+          oa[i] = a0 + scale(qdd, J);
+          cd[i] = {J, zeroMot<S>(), crossMM(a0, J), crossMM(ov[i], J)};
+        }
+      }
+    } else {
+      // ---- CLOSE(i): composite quantities of the subtree ----
+      CInertia<S> Y = worldInertia(mc, i, oM[i]);
+      const Frc<S> oh = cmul(Y, ov[i]);
+      Frc<S> of = cmul(Y, oa[i]) + crossMF(ov[i], oh);
+      DY<S> DYc = inertiaVariation(Y, ov[i], oh);
+      if constexpr (TP::T.nchild[i] > 0) {
+        cadd(Y, Yacc[i]);
+        dyAdd(DYc, DYacc[i]);
+        of = of + Facc[i];
+      }
+#pragma unroll
+      for (int k = 0; k < nvi; ++k) {
+        const int c = iv + k;
+        const ColData<S> C = colData(IC<i>{}, k);
+        const Frc<S> dFda = cmul(Y, C.J);
+        Frc<S> dFdq = cmul(Y, C.dAdq);
+        if constexpr (p > 0) dFdq = dFdq + dyMul(DYc, C.dVdq);
+        const Frc<S> dFdv = dyMul(DYc, C.J) + cmul(Y, C.dAdv);
+        const Frc<S> dYtJ = dyMulT(DYc, C.J);
+        // entries inside this joint's block
+#pragma unroll
+        for (int k2 = 0; k2 < nvi; ++k2) {
+          const int r = iv + k2;
+          const Mot<S> Jr = colData(IC<i>{}, k2).J;
+          if (wq) stMat(gdq, b, NV, r, c, dot(Jr, dFdq));
+          if (wv) stMat(gdv, b, NV, r, c, dot(Jr, dFdv));
+          if (wM) stMat(gM, b, NV, r, c, dot(Jr, dFda) + (r == c ? mc[mcArmature<NJ>() + c] : S(0)));
+        }
+        const Frc<S> dFdqA = dFdq + crossMF(C.J, of);
+        // pairs with every ancestor dof
+        static_for<1, NJ>([&](auto A) {
+          constexpr int a = A;
+          if constexpr (a != i && TP::T.pos[a] < TP::T.pos[i] && TP::T.pos[a] + TP::T.subsize[a] > TP::T.pos[i]) {
+#pragma unroll
+            for (int k2 = 0; k2 < TP::T.nvj[a]; ++k2) {
+              const int r = TP::T.iv[a] + k2;
+              const ColData<S> R = colData(A, k2);
+              if (wq) {
+                stMat(gdq, b, NV, r, c, dot(R.J, dFdqA));
+                stMat(gdq, b, NV, c, r, dot(R.dAdq, dFda) + dot(R.dVdq, dYtJ));
+              }
+              if (wv) {
+                stMat(gdv, b, NV, r, c, dot(R.J, dFdv));
+                stMat(gdv, b, NV, c, r, dot(R.dAdv, dFda) + dot(R.J, dYtJ));
+              }
+              if (wM) {
+                const S m = dot(R.J, dFda);
+                stMat(gM, b, NV, r, c, m);
+                stMat(gM, b, NV, c, r, m);
+              }
+            }
+          }
+        });
+      }
+      if constexpr (p > 0) {
+        if constexpr (TP::T.childIdx[i] == 0) {
+          Yacc[p] = Y;
+          DYacc[p] = DYc;
+          Facc[p] = of;
+        } else {
+          cadd(Yacc[p], Y);
+          dyAdd(DYacc[p], DYc);
+          Facc[p] = Facc[p] + of;
+        }
+      }
+    }
+  });
+  if (live) {
+    static_for<0, NV>([&](auto C) {
+      static_for<0, NV>([&](auto R) {
+        if constexpr (!CrbaPattern<TP>::nz(R, C)) {
+          if (wq) stMat(gdq, b, NV, R, C, S(0));
+          if (wv) stMat(gdv, b, NV, R, C, S(0));
+          if (wM) stMat(gM, b, NV, R, C, S(0));
+        }
+      });
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// ABA-derivative solve: with the packed tree factor of M (from k_chol<.., S, S>), reads
+// dtau_dq (from gXq) and dtau_dv (from gXv); writes -Minv dtau_dq, -Minv dtau_dv in place
+// and Minv into gMinv.  The whole factor is held in registers across the 3 nv solves.
+// ---------------------------------------------------------------------------------------
+template<class TP, class S>
+__global__ __launch_bounds__(64) void k_abad_solve(int64_t B, CView<S> gU, CView<S> gD, View<S> gXq, View<S> gXv,
+                                                   View<S> gMinv) {
+  constexpr auto& T = TP::T;
+  constexpr int NV = T.nv;
+  using P = CholPattern<TP>;
+  const int L = laneId();
+  const int64_t b = int64_t(blockIdx.x) * kTile + L;
+  if (b >= B) return;
+  auto ld = [&](View<S> g, int r, int c) { return g.p[(int64_t)(r + c * NV) * g.rs + b * g.bs]; };
+
+  S U[NV][NV];
+  S Dinv[NV];
+  {
+    const S* __restrict__ pU = gU.p + b * gU.bs;
+    const S* __restrict__ pD = gD.p + b * gD.bs;
+    static_for<0, NV>([&](auto CC) {
+      constexpr int c = CC;
+      Dinv[c] = S(1) / pD[c * gD.rs];
+      static_for<0, NV>([&](auto RR) {
+        constexpr int r = RR;
+        if constexpr (P::anc(r, c)) {
+          constexpr int pk = P::packed(r, c);
+          U[r][c] = pU[pk * gU.rs];
+        }
+      });
+    });
+  }
+
+  // x <- M^-1 x  (cholesky::solve: Uiv, D^-1, Utiv, cholesky.hxx:466-520)
+  auto solve = [&](S (&x)[NV]) {
+    static_for<0, NV>([&](auto KK) {
+      constexpr int k = NV - 1 - KK;
+      stepFence();
+      static_for<k + 1, NV>([&](auto EE) {
+        constexpr int e2 = EE;
+        if constexpr (P::anc(k, e2)) x[k] -= U[k][e2] * x[e2];
+      });
+    });
+#pragma unroll
+    for (int k = 0; k < NV; ++k) x[k] *= Dinv[k];
+    static_for<0, NV>([&](auto KK) {
+      constexpr int k = KK;
+      stepFence();
+      static_for<k + 1, NV>([&](auto EE) {
+        constexpr int e2 = EE;
+        if constexpr (P::anc(k, e2)) x[e2] -= U[k][e2] * x[k];
+      });
+    });
+  };
+
+#pragma unroll 1
+  for (int c = 0; c < NV; ++c) {
+    S x[NV];
+#pragma unroll
+    for (int r = 0; r < NV; ++r) x[r] = ld(gXq, r, c);
+    solve(x);
+#pragma unroll
+    for (int r = 0; r < NV; ++r) gXq.p[(int64_t)(r + c * NV) * gXq.rs + b * gXq.bs] = -x[r];
+#pragma unroll
+    for (int r = 0; r < NV; ++r) x[r] = ld(gXv, r, c);
+    solve(x);
+#pragma unroll
+    for (int r = 0; r < NV; ++r) gXv.p[(int64_t)(r + c * NV) * gXv.rs + b * gXv.bs] = -x[r];
+#pragma unroll
+    for (int r = 0; r < NV; ++r) x[r] = (r == c) ? S(1) : S(0);
+    solve(x);
+#pragma unroll
+    for (int r = 0; r < NV; ++r) gMinv.p[(int64_t)(r + c * NV) * gMinv.rs + b * gMinv.bs] = x[r];
+  }
+}
+
+}  // namespace rbd

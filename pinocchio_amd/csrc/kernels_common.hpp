@@ -1,0 +1,229 @@
+// kernels_common.hpp -- shared pieces of the per-lane sweep kernels: batch I/O staging
+// through LDS, model-constant access, joint transforms.
+//
+// One lane evaluates one configuration.  A workgroup is one 64-lane wave that owns a
+// tile of 64 consecutive configurations; its inputs are copied into LDS with coalesced
+// loads (whatever the caller's layout) and read back per lane as the sweep needs them,
+// so no input occupies a register longer than its use.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spatial.hpp"
+#include "topo.hpp"
+
+namespace rbd {
+
+constexpr int kTile = 64;   // configurations per wave
+constexpr int kPitch = 65;  // LDS row pitch (odd: conflict-free column access)
+
+// Strided batch view: element (k, b) at p[k * rs + b * bs].
+template<class S> struct View {
+  S* p;
+  int64_t rs, bs;
+};
+template<class S> struct CView {
+  const S* p;
+  int64_t rs, bs;
+};
+
+__device__ __forceinline__ int laneId() { return threadIdx.x & 63; }
+
+// Scheduling fence between per-joint steps of an unrolled sweep: keeps the machine
+// scheduler from hoisting the next joint's constant loads across the whole tree, which
+// would stretch live ranges far beyond the open path.
+__device__ __forceinline__ void stepFence() {
+#ifndef RBD_NO_FENCE
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
+// Model-constant pointer made opaque to the optimiser at the point of use: uniform loads
+// through it cannot be merged with, or hoisted above, those of earlier steps, so a
+// joint's constants occupy scalar registers only during that joint's step.
+template<class S> __device__ __forceinline__ const S* stepConsts(const S* mc) {
+#ifndef RBD_NO_LAUNDER
+  asm volatile("" : "+s"(mc));
+#endif
+  return mc;
+}
+
+// Per-lane pointer made opaque at the point of use, so loads through it are neither
+// merged with nor reused from (CSE) earlier steps: values re-read later stay out of the
+// register file in between.
+template<class S> __device__ __forceinline__ S* laneLaunder(S* p) {
+  asm volatile("" : "+v"(p));
+  return p;
+}
+
+// Copy rows [0, R) of the tile starting at configuration b0 into lds[k * kPitch + b].
+// Lanes past the batch end get zeros (keeps every lane on the fast math paths).
+template<int R, class S>
+__device__ __forceinline__ void stageIn(S* __restrict__ lds, CView<S> g, int64_t b0, int nb) {
+  const int t = laneId();
+  if (g.rs == 1) {  // configuration-contiguous (column major, Eigen default)
+    for (int e = t; e < R * kTile; e += kTile) {
+      const int b = e / R, k = e - b * R;
+      lds[k * kPitch + b] = b < nb ? g.p[(b0 + b) * g.bs + k] : S(0);
+    }
+  } else {  // batch-fastest or general strides
+    for (int e = t; e < R * kTile; e += kTile) {
+      const int k = e >> 6, b = e & 63;
+      lds[k * kPitch + b] = b < nb ? g.p[k * g.rs + (b0 + b) * g.bs] : S(0);
+    }
+  }
+}
+template<int R, class S>
+__device__ __forceinline__ void stageOut(const S* __restrict__ lds, View<S> g, int64_t b0, int nb) {
+  const int t = laneId();
+  if (g.rs == 1) {
+    for (int e = t; e < R * kTile; e += kTile) {
+      const int b = e / R, k = e - b * R;
+      if (b < nb) g.p[(b0 + b) * g.bs + k] = lds[k * kPitch + b];
+    }
+  } else {
+    for (int e = t; e < R * kTile; e += kTile) {
+      const int k = e >> 6, b = e & 63;
+      if (b < nb) g.p[k * g.rs + (b0 + b) * g.bs] = lds[k * kPitch + b];
+    }
+  }
+}
+
+// ---- model constants ----------------------------------------------------------------
+template<class S> __device__ __forceinline__ R3<S> mcRot(const S* __restrict__ mc, int j) {
+  const S* r = mc + j * MC_REC + MC_R;
+  return {r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]};
+}
+template<class S> __device__ __forceinline__ V3<S> mcPos(const S* __restrict__ mc, int j) {
+  const S* r = mc + j * MC_REC + MC_P;
+  return {r[0], r[1], r[2]};
+}
+template<class S> __device__ __forceinline__ Body<S> mcBody(const S* __restrict__ mc, int j) {
+  const S* r = mc + j * MC_REC;
+  return {r[MC_M], {r[MC_C], r[MC_C + 1], r[MC_C + 2]}, r[MC_I], r[MC_I + 1], r[MC_I + 2], r[MC_I + 3], r[MC_I + 4],
+          r[MC_I + 5]};
+}
+template<class S> __device__ __forceinline__ Sym6<S> mcSym6(const S* __restrict__ mc, int j) {
+  const S* r = mc + j * MC_REC + MC_Y6;
+  Sym6<S> o;
+#pragma unroll
+  for (int k = 0; k < 21; ++k) o.a[k] = r[k];
+  return o;
+}
+template<class S> __device__ __forceinline__ CInertia<S> mcComposite(const S* __restrict__ mc, int j) {
+  const S* r = mc + j * MC_REC + MC_CI;
+  return {r[0], {r[1], r[2], r[3]}, r[4], r[5], r[6], r[7], r[8], r[9]};
+}
+
+// ---- joint transforms ---------------------------------------------------------------
+// Branch-free sine and cosine of a joint angle.  The library routine carries a
+// large-argument reduction path whose registers, inlined once per joint, would be live
+// across the whole sweep; this one is straight-line code:
+//   k = rint(x * 2/pi), r = x - k pi/2 (Cody-Waite, pi/2 split in three parts, FMA),
+//   sin r, cos r by their Taylor series on |r| <= pi/4 (truncation below 3e-9 in FP32
+//   with terms to r^9 / r^10, below 1e-19 in FP64 with terms to r^17 / r^18),
+//   then the quadrant k mod 4 selects and signs the pair.
+// Accurate to a few ulp for |x| up to ~1e4 (FP32) / ~1e8 (FP64), far beyond joint ranges.
+template<class S> struct TrigConst;
+template<> struct TrigConst<float> {
+  static constexpr float twoOverPi = 0.636619772367581343f;
+  static constexpr float c1 = 1.5707963705062866f, c2 = -4.371138828673793e-08f, c3 = -1.7151245100058819e-15f;
+  static constexpr int nsin = 4, ncos = 5;
+};
+template<> struct TrigConst<double> {
+  static constexpr double twoOverPi = 0.6366197723675814;
+  static constexpr double c1 = 1.5707963267948966, c2 = 6.123233995736766e-17, c3 = -1.4973849048591698e-33;
+  static constexpr int nsin = 8, ncos = 9;
+};
+template<class S> RBD_HD void sincosT(S x, S* s, S* c) {
+  using K = TrigConst<S>;
+  const S k = rint(x * K::twoOverPi);
+  S r = fma(-k, K::c1, x);
+  r = fma(-k, K::c2, r);
+  r = fma(-k, K::c3, r);
+  const S r2 = r * r;
+  // sin r = r (1 - r^2/3! + r^4/5! - ...), cos r = 1 - r^2/2! + r^4/4! - ...  (Horner)
+  S ps = S(0), pc = S(0);
+  {
+    // coefficients (-1)^n / (2n+1)!  and  (-1)^n / (2n)!  for n = 1..N, built exactly
+    S fs[10], fc[10];
+    S f = S(1);
+#pragma unroll
+    for (int n = 1; n <= 9; ++n) {
+      fc[n] = ((n & 1) ? S(-1) : S(1)) / (f * S(2 * n - 1) * S(2 * n));
+      f = f * S(2 * n - 1) * S(2 * n);
+      fs[n] = ((n & 1) ? S(-1) : S(1)) / (f * S(2 * n + 1));
+    }
+#pragma unroll
+    for (int n = K::nsin; n >= 1; --n) ps = fma(ps, r2, fs[n]);
+#pragma unroll
+    for (int n = K::ncos; n >= 1; --n) pc = fma(pc, r2, fc[n]);
+  }
+  const S sr = fma(r * r2, ps, r);
+  const S cr = fma(r2, pc, S(1));
+  const int q = int(k) & 3;
+  const S s0 = (q & 1) ? cr : sr;
+  const S c0 = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -s0 : s0;
+  *c = ((q + 1) & 2) ? -c0 : c0;
+}
+
+// liMi = jointPlacement * jointTransform(q) for a revolute joint (joint-revolute.hpp:793-820)
+template<class S> __device__ __forceinline__ Xform<S> revoluteX(const S* __restrict__ mc, int j, int ax, S sn, S cs) {
+  return {rotAbout(mcRot(mc, j), ax, sn, cs), mcPos(mc, j)};
+}
+// liMi for a free-flyer joint (joint-free-flyer.hpp:333-375); q7 = [x, y, z, qx, qy, qz, qw]
+template<class S>
+__device__ __forceinline__ Xform<S> freeflyerX(const S* __restrict__ mc, int j, const S* q7, bool identityPlacement) {
+  const Xform<S> J{quatToRot(q7[3], q7[4], q7[5], q7[6]), {q7[0], q7[1], q7[2]}};
+  if (identityPlacement) return J;
+  return compose(Xform<S>{mcRot(mc, j), mcPos(mc, j)}, J);
+}
+
+template<class S> __device__ __forceinline__ Mot<S> axisMot(int ax, S w) {
+  Mot<S> m = zeroMot<S>();
+  if (ax == 0) m.a.x = w;
+  else if (ax == 1) m.a.y = w;
+  else m.a.z = w;
+  return m;
+}
+
+// 6x6 SPD inverse in double through an LLT (PerformStYSInversion,
+// joint-common-operations.hpp:22-45, for the free-flyer D = S^T Ia S block).
+__device__ __forceinline__ void spdSolve6(const double (&A)[6][6], const double (&b)[6], double (&x)[6]) {
+  double L[6][6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double s = A[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+    const double ljj = sqrt(s);
+    const double inv = 1.0 / ljj;
+    L[j][j] = ljj;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      double t = A[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+      L[i][j] = t * inv;
+    }
+  }
+  double y[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double t = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
+    y[i] = t / L[i][i];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double t = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) t -= L[k][i] * x[k];
+    x[i] = t / L[i][i];
+  }
+}
+
+}  // namespace rbd

@@ -1,0 +1,51 @@
+// ops.hpp -- launcher table of one compiled topology (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rbd {
+
+template<class S> struct View;
+template<class S> struct CView;
+
+template<class S>
+using LaunchVec3 = void (*)(hipStream_t, const S* mc, int64_t B, CView<S> in0, CView<S> in1, CView<S> in2,
+                            View<S> out);
+template<class S> using LaunchCrba = void (*)(hipStream_t, const S* mc, int64_t B, CView<S> q, View<S> M, int upper);
+template<class S>
+using LaunchChol = void (*)(hipStream_t, int64_t B, CView<S> M, View<double> U, View<double> D);
+template<class S>
+using LaunchCholSolve = void (*)(hipStream_t, int64_t B, CView<double> U, CView<double> D, View<S> x);
+template<class S>
+using LaunchRneaD = void (*)(hipStream_t, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq,
+                             View<S> dv, View<S> M, int mask);
+// FP32 factor for the ABA-derivative pipeline
+using LaunchChol32 = void (*)(hipStream_t, int64_t B, CView<float> M, View<float> U, View<float> D);
+template<class S>
+using LaunchAbaDSolve = void (*)(hipStream_t, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv,
+                                 View<S> Minv);
+
+struct TopoOps {
+  const char* name;
+  int njoints;
+  const int* parent;
+  const int* type;
+  int nq, nv, compact;
+  LaunchVec3<float> rnea_f32;
+  LaunchVec3<double> rnea_f64;
+  LaunchVec3<float> aba_f32;
+  LaunchVec3<double> aba_f64;
+  LaunchCrba<float> crba_f32;
+  LaunchCrba<double> crba_f64;
+  LaunchChol<float> chol_f32;
+  LaunchChol<double> chol_f64;
+  LaunchCholSolve<float> cholsolve_f32;
+  LaunchCholSolve<double> cholsolve_f64;
+  LaunchRneaD<float> rnead_f32;
+  LaunchChol32 chol32_f32;
+  LaunchAbaDSolve<float> abad_solve_f32;
+  int nnzU;  // strictly-upper structural non-zeros of M
+};
+
+}  // namespace rbd

@@ -1,0 +1,603 @@
+// runtime.cpp -- C ABI of the engine: contexts, model upload, argument checking, layout
+// handling and kernel dispatch (include/rbd.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <type_traits>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rbd.h"
+#include "kernels_common.hpp"
+#include "ops.hpp"
+
+namespace rbd {
+#define RBD_REGISTER(NAME) const TopoOps* ops_##NAME();
+#include "_gen/registry.inc"
+#undef RBD_REGISTER
+
+static const TopoOps* const* registry(int* n) {
+#define RBD_REGISTER(NAME) ops_##NAME(),
+  static const TopoOps* const all[] = {
+#include "_gen/registry.inc"
+  };
+#undef RBD_REGISTER
+  *n = (int)(sizeof(all) / sizeof(all[0]));
+  return all;
+}
+}  // namespace rbd
+
+using namespace rbd;
+
+namespace {
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+#define RBD_CHECK_ARG(cond, msg) \
+  do {                           \
+    if (!(cond)) throw Error(RBD_EINVAL, msg); \
+  } while (0)
+#define HIP_CHECK(expr)                                                                            \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) throw Error(RBD_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template<class F> int guarded(F&& f) {
+  try {
+    f();
+    return RBD_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return RBD_ENOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return RBD_EINVAL;
+  }
+}
+}  // namespace
+
+struct rbd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // device staging for RBD_HOST_PTRS calls, grown on demand
+  std::vector<void*> stage;
+  std::vector<size_t> stageBytes;
+  // device workspace for intermediate results (ABA-derivative factor, ddq), grown on demand
+  void* work = nullptr;
+  size_t workBytes = 0;
+  void* workspace(size_t bytes) {
+    if (workBytes < bytes) {
+      if (work) HIP_CHECK(hipFree(work));
+      work = nullptr;
+      HIP_CHECK(hipMalloc(&work, bytes));
+      workBytes = bytes;
+    }
+    return work;
+  }
+  void* stageBuf(int slot, size_t bytes) {
+    if ((int)stage.size() <= slot) {
+      stage.resize(slot + 1, nullptr);
+      stageBytes.resize(slot + 1, 0);
+    }
+    if (stageBytes[slot] < bytes) {
+      if (stage[slot]) HIP_CHECK(hipFree(stage[slot]));
+      stage[slot] = nullptr;
+      HIP_CHECK(hipMalloc(&stage[slot], bytes));
+      stageBytes[slot] = bytes;
+    }
+    return stage[slot];
+  }
+};
+
+struct rbd_model {
+  rbd_ctx* ctx = nullptr;
+  const TopoOps* ops = nullptr;
+  int njoints = 0, nq = 0, nv = 0, compact = 0;
+  std::vector<int> parents, jtype, iq, iv;
+  std::vector<double> gravity;
+  float* mc32 = nullptr;
+  double* mc64 = nullptr;
+  // packed strictly-upper structural non-zeros of M (tree pattern), column by column
+  std::vector<int32_t> patRows, patCols;
+};
+
+namespace {
+
+void setDevice(rbd_ctx* ctx) { HIP_CHECK(hipSetDevice(ctx->device)); }
+
+// Host flattening of the model constants into per-joint records (topo.hpp MC_* offsets).
+std::vector<double> buildConstants(const rbd_model_desc* d, int nv) {
+  const int nj = d->njoints;
+  std::vector<double> mc((size_t)nj * MC_REC + nv + 6, 0.0);
+  for (int j = 0; j < nj; ++j) {
+    double* r = &mc[(size_t)j * MC_REC];
+    const double* P = d->placement + 12 * j;
+    for (int k = 0; k < 12; ++k) r[MC_R + k] = P[k];
+    const double* Y = d->inertia + 10 * j;
+    const double m = Y[0], cx = Y[1], cy = Y[2], cz = Y[3];
+    r[MC_M] = m;
+    r[MC_C] = cx; r[MC_C + 1] = cy; r[MC_C + 2] = cz;
+    for (int k = 0; k < 6; ++k) r[MC_I + k] = Y[4 + k];
+    const double Ixx = Y[4], Ixy = Y[5], Iyy = Y[6], Ixz = Y[7], Iyz = Y[8], Izz = Y[9];
+    // packed symmetric 6x6 of the body inertia, spatial.hpp symIdx order
+    double S6[6][6] = {};
+    for (int k = 0; k < 3; ++k) S6[k][k] = m;
+    const double mc_[3] = {m * cx, m * cy, m * cz};
+    // -m[c]x in block (0,3)
+    S6[0][4] = mc_[2]; S6[0][5] = -mc_[1];
+    S6[1][3] = -mc_[2]; S6[1][5] = mc_[0];
+    S6[2][3] = mc_[1]; S6[2][4] = -mc_[0];
+    const double Io[3][3] = {{Ixx + m * (cy * cy + cz * cz), Ixy - m * cx * cy, Ixz - m * cx * cz},
+                             {Ixy - m * cx * cy, Iyy + m * (cx * cx + cz * cz), Iyz - m * cy * cz},
+                             {Ixz - m * cx * cz, Iyz - m * cy * cz, Izz + m * (cx * cx + cy * cy)}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) S6[3 + a][3 + b] = Io[a][b];
+    int idx = 0;
+    for (int a = 0; a < 6; ++a)
+      for (int b = a; b < 6; ++b) r[MC_Y6 + idx++] = S6[a][b];
+    r[MC_CI + 0] = m;
+    r[MC_CI + 1] = mc_[0]; r[MC_CI + 2] = mc_[1]; r[MC_CI + 3] = mc_[2];
+    r[MC_CI + 4] = Io[0][0]; r[MC_CI + 5] = Io[0][1]; r[MC_CI + 6] = Io[1][1];
+    r[MC_CI + 7] = Io[0][2]; r[MC_CI + 8] = Io[1][2]; r[MC_CI + 9] = Io[2][2];
+  }
+  double* arm = &mc[(size_t)nj * MC_REC];
+  for (int k = 0; k < nv; ++k) arm[k] = d->armature ? d->armature[k] : 0.0;
+  double* g = arm + nv;
+  const double g0[6] = {0, 0, -9.81, 0, 0, 0};
+  for (int k = 0; k < 6; ++k) g[k] = d->gravity ? d->gravity[k] : g0[k];
+  return mc;
+}
+
+// Validated strided view over a batched argument with `rows` rows.
+template<class S> struct ArgView {
+  int64_t rs, bs;
+  size_t extent;  // elements spanned
+};
+template<class S> ArgView<S> argView(int rows, int64_t B, int64_t ld, uint32_t flags, const char* what) {
+  ArgView<S> v{};
+  if (flags & RBD_LAYOUT_BATCH) {
+    if (ld < B) throw Error(RBD_EINVAL, std::string(what) + ": leading dimension smaller than the batch");
+    v.rs = ld;
+    v.bs = 1;
+    v.extent = (size_t)(ld * (rows - 1) + B);
+  } else {
+    if (ld < rows) throw Error(RBD_EINVAL, std::string(what) + ": leading dimension smaller than the row count");
+    v.rs = 1;
+    v.bs = ld;
+    v.extent = (size_t)(ld * (B - 1) + rows);
+  }
+  return v;
+}
+
+// Moves an argument to the device when the call uses host pointers.
+struct Staged {
+  rbd_ctx* ctx;
+  bool host;
+  int slot = 0;
+  std::vector<std::pair<void*, std::pair<const void*, size_t>>> back;  // device, host, bytes
+  const void* in(const void* p, size_t bytes) {
+    if (!host || !p) return p;
+    void* d = ctx->stageBuf(slot++, bytes);
+    HIP_CHECK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return d;
+  }
+  void* out(void* p, size_t bytes, bool preload) {
+    if (!host || !p) return p;
+    void* d = ctx->stageBuf(slot++, bytes);
+    if (preload) HIP_CHECK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    back.push_back({d, {p, bytes}});
+    return d;
+  }
+  void finish() {
+    if (!host) return;
+    for (auto& b : back)
+      HIP_CHECK(hipMemcpyAsync(const_cast<void*>(b.second.first), b.first, b.second.second, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  }
+};
+
+void finishCall(rbd_ctx* ctx, uint32_t flags, Staged& st) {
+  HIP_CHECK(hipGetLastError());
+  st.finish();
+  if (!(flags & RBD_ASYNC) || (flags & RBD_HOST_PTRS)) HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void checkCommon(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype) {
+  RBD_CHECK_ARG(ctx != nullptr, "null context");
+  RBD_CHECK_ARG(m != nullptr, "null model");
+  RBD_CHECK_ARG(m->ctx == ctx, "model belongs to another context");
+  RBD_CHECK_ARG(B >= 0, "negative batch size");
+  RBD_CHECK_ARG(dtype == RBD_F32 || dtype == RBD_F64, "dtype must be RBD_F32 or RBD_F64");
+}
+
+template<class S>
+int vec3Op(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* x0, int64_t ld0, int r0, const void* x1,
+           int64_t ld1, const void* x2, int64_t ld2, void* out, int64_t ldo, uint32_t flags, LaunchVec3<S> fn,
+           const char* opname) {
+  return guarded([&] {
+    if (!fn) throw Error(RBD_EUNSUPPORTED, std::string(opname) + ": no kernel for this dtype/topology");
+    RBD_CHECK_ARG(x0 && x1 && x2 && out, "null array argument");
+    const int nv = m->nv;
+    auto v0 = argView<S>(r0, B, ld0, flags, "input 0");
+    auto v1 = argView<S>(nv, B, ld1, flags, "input 1");
+    auto v2 = argView<S>(nv, B, ld2, flags, "input 2");
+    auto vo = argView<S>(nv, B, ldo, flags, "output");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const S* d0 = (const S*)st.in(x0, v0.extent * sizeof(S));
+    const S* d1 = (const S*)st.in(x1, v1.extent * sizeof(S));
+    const S* d2 = (const S*)st.in(x2, v2.extent * sizeof(S));
+    S* dout = (S*)st.out(out, vo.extent * sizeof(S), ldo != nv || (flags & RBD_LAYOUT_BATCH));
+    const S* mc = std::is_same<S, float>::value ? (const S*)m->mc32 : (const S*)m->mc64;
+    fn(ctx->stream, mc, B, CView<S>{d0, v0.rs, v0.bs}, CView<S>{d1, v1.rs, v1.bs}, CView<S>{d2, v2.rs, v2.bs},
+       View<S>{dout, vo.rs, vo.bs});
+    finishCall(ctx, flags, st);
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rbd_last_error(void) { return g_err.c_str(); }
+const char* rbd_version(void) { return "pinocchio_amd 0.1 (gfx950)"; }
+
+int rbd_ctx_create(int device, rbd_ctx** out) {
+  return guarded([&] {
+    RBD_CHECK_ARG(out != nullptr, "null output pointer");
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    RBD_CHECK_ARG(device >= 0 && device < n, "device index out of range");
+    auto* c = new rbd_ctx();
+    c->device = device;
+    try {
+      HIP_CHECK(hipSetDevice(device));
+      HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int rbd_ctx_destroy(rbd_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->stage)
+      if (p) (void)hipFree(p);
+    if (ctx->work) (void)hipFree(ctx->work);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+void* rbd_ctx_stream(rbd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int rbd_ctx_synchronize(rbd_ctx* ctx) {
+  return guarded([&] {
+    RBD_CHECK_ARG(ctx, "null context");
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* d, rbd_model** out) {
+  return guarded([&] {
+    RBD_CHECK_ARG(ctx && d && out, "null argument");
+    RBD_CHECK_ARG(d->njoints >= 2, "model needs at least one moving joint");
+    RBD_CHECK_ARG(d->parents && d->jtype && d->placement && d->inertia, "null model array");
+    const int nj = d->njoints;
+    auto m = std::make_unique<rbd_model>();
+    m->ctx = ctx;
+    m->njoints = nj;
+    m->parents.assign(d->parents, d->parents + nj);
+    m->jtype.assign(d->jtype, d->jtype + nj);
+    m->iq.assign(nj, 0);
+    m->iv.assign(nj, 0);
+    for (int j = 1; j < nj; ++j) {
+      RBD_CHECK_ARG(m->parents[j] >= 0 && m->parents[j] < j, "parents[i] must precede i");
+      const int t = m->jtype[j];
+      if (t != RBD_JOINT_FREEFLYER && t != RBD_JOINT_RX && t != RBD_JOINT_RY && t != RBD_JOINT_RZ)
+        throw Error(RBD_EUNSUPPORTED, "joint type not supported (FreeFlyer, RX, RY, RZ only)");
+      if (t == RBD_JOINT_FREEFLYER && m->parents[j] != 0)
+        throw Error(RBD_EUNSUPPORTED, "a free-flyer joint must be attached to the universe");
+      m->iq[j] = m->nq;
+      m->iv[j] = m->nv;
+      m->nq += t == RBD_JOINT_FREEFLYER ? 7 : 1;
+      m->nv += t == RBD_JOINT_FREEFLYER ? 6 : 1;
+    }
+    for (int j = 1; j < nj; ++j) {
+      const double* Y = d->inertia + 10 * j;
+      RBD_CHECK_ARG(std::isfinite(Y[0]) && Y[0] >= 0.0, "body mass must be finite and non-negative");
+    }
+    int n = 0;
+    const TopoOps* const* all = registry(&n);
+    for (int k = 0; k < n && !m->ops; ++k) {
+      const TopoOps* o = all[k];
+      if (o->njoints != nj) continue;
+      if (std::memcmp(o->parent, m->parents.data(), sizeof(int) * nj) != 0) continue;
+      bool same = true;
+      for (int j = 1; j < nj; ++j) same = same && o->type[j] == m->jtype[j];
+      if (same) m->ops = o;
+    }
+    if (!m->ops) throw Error(RBD_EUNSUPPORTED, "no compiled kernels for this model topology");
+    m->compact = m->ops->compact;
+    m->gravity.assign(6, 0.0);
+    const double g0[6] = {0, 0, -9.81, 0, 0, 0};
+    for (int k = 0; k < 6; ++k) m->gravity[k] = d->gravity ? d->gravity[k] : g0[k];
+    // structural pattern of the upper triangle: (r, c) with r an ancestor dof of c
+    {
+      std::vector<int> dofParent(m->nv, -1);
+      for (int j = 1; j < nj; ++j) {
+        const int p = m->parents[j];
+        const int nvj = m->jtype[j] == RBD_JOINT_FREEFLYER ? 6 : 1;
+        dofParent[m->iv[j]] = p > 0 ? m->iv[p] + (m->jtype[p] == RBD_JOINT_FREEFLYER ? 6 : 1) - 1 : -1;
+        for (int r = 1; r < nvj; ++r) dofParent[m->iv[j] + r] = m->iv[j] + r - 1;
+      }
+      for (int c = 0; c < m->nv; ++c) {
+        std::vector<int> anc;
+        for (int r = dofParent[c]; r >= 0; r = dofParent[r]) anc.push_back(r);
+        for (auto it = anc.rbegin(); it != anc.rend(); ++it) {
+          m->patRows.push_back(*it);
+          m->patCols.push_back(c);
+        }
+      }
+    }
+    const std::vector<double> mc = buildConstants(d, m->nv);
+    std::vector<float> mcf(mc.begin(), mc.end());
+    setDevice(ctx);
+    HIP_CHECK(hipMalloc(&m->mc64, mc.size() * sizeof(double)));
+    HIP_CHECK(hipMalloc(&m->mc32, mcf.size() * sizeof(float)));
+    HIP_CHECK(hipMemcpy(m->mc64, mc.data(), mc.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->mc32, mcf.data(), mcf.size() * sizeof(float), hipMemcpyHostToDevice));
+    *out = m.release();
+  });
+}
+
+int rbd_model_destroy(rbd_model* m) {
+  return guarded([&] {
+    if (!m) return;
+    hipSetDevice(m->ctx->device);
+    if (m->mc32) hipFree(m->mc32);
+    if (m->mc64) hipFree(m->mc64);
+    delete m;
+  });
+}
+
+int rbd_model_dims(const rbd_model* m, int32_t* nq, int32_t* nv, int32_t* nj, int32_t* compact) {
+  return guarded([&] {
+    RBD_CHECK_ARG(m, "null model");
+    if (nq) *nq = m->nq;
+    if (nv) *nv = m->nv;
+    if (nj) *nj = m->njoints;
+    if (compact) *compact = m->compact;
+  });
+}
+
+int rbd_rnea(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, const void* v,
+             int64_t ldv, const void* a, int64_t lda, void* tau, int64_t ldtau, uint32_t flags) {
+  int rc = guarded([&] { checkCommon(ctx, m, B, dtype); });
+  if (rc) return rc;
+  if (dtype == RBD_F32)
+    return vec3Op<float>(ctx, m, B, q, ldq, m->nq, v, ldv, a, lda, tau, ldtau, flags, m->ops->rnea_f32, "rnea");
+  return vec3Op<double>(ctx, m, B, q, ldq, m->nq, v, ldv, a, lda, tau, ldtau, flags, m->ops->rnea_f64, "rnea");
+}
+
+int rbd_aba(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, const void* v,
+            int64_t ldv, const void* tau, int64_t ldtau, void* ddq, int64_t ldddq, uint32_t flags) {
+  int rc = guarded([&] { checkCommon(ctx, m, B, dtype); });
+  if (rc) return rc;
+  if (dtype == RBD_F32)
+    return vec3Op<float>(ctx, m, B, q, ldq, m->nq, v, ldv, tau, ldtau, ddq, ldddq, flags, m->ops->aba_f32, "aba");
+  return vec3Op<double>(ctx, m, B, q, ldq, m->nq, v, ldv, tau, ldtau, ddq, ldddq, flags, m->ops->aba_f64, "aba");
+}
+
+int rbd_crba(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, void* M,
+             int64_t ldM, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(q && M, "null array argument");
+    if (!m->compact) throw Error(RBD_EUNSUPPORTED, "crba: the model is not DFS-compact (crba.hxx:573-595)");
+    const int nv = m->nv;
+    const bool f32 = dtype == RBD_F32;
+    const size_t es = f32 ? 4 : 8;
+    if ((f32 && !m->ops->crba_f32) || (!f32 && !m->ops->crba_f64))
+      throw Error(RBD_EUNSUPPORTED, "crba: no kernel for this dtype/topology");
+    auto vq = argView<float>(m->nq, B, ldq, flags, "q");
+    auto vM = argView<float>(nv * nv, B, ldM, flags, "M");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const void* dq = st.in(q, vq.extent * es);
+    void* dM = st.out(M, vM.extent * es, true);
+    const int upper = (flags & RBD_UPPER_ONLY) ? 1 : 0;
+    if (f32)
+      m->ops->crba_f32(ctx->stream, m->mc32, B, CView<float>{(const float*)dq, vq.rs, vq.bs},
+                       View<float>{(float*)dM, vM.rs, vM.bs}, upper);
+    else
+      m->ops->crba_f64(ctx->stream, m->mc64, B, CView<double>{(const double*)dq, vq.rs, vq.bs},
+                       View<double>{(double*)dM, vM.rs, vM.bs}, upper);
+    finishCall(ctx, flags, st);
+  });
+}
+
+int rbd_cholesky_decompose(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* M, int64_t ldM,
+                           double* U, int64_t ldU, double* D, int64_t ldD, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(M && U && D, "null array argument");
+    if (!m->compact) throw Error(RBD_EUNSUPPORTED, "cholesky: the model is not DFS-compact");
+    const int nv = m->nv, nnz = (int)m->patRows.size();
+    const bool f32 = dtype == RBD_F32;
+    const size_t es = f32 ? 4 : 8;
+    auto vM = argView<float>(nv * nv, B, ldM, flags, "M");
+    auto vU = argView<double>(nnz > 0 ? nnz : 1, B, ldU, flags, "U");
+    auto vD = argView<double>(nv, B, ldD, flags, "D");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const void* dM = st.in(M, vM.extent * es);
+    double* dU = (double*)st.out(U, vU.extent * 8, true);
+    double* dD = (double*)st.out(D, vD.extent * 8, true);
+    if (f32)
+      m->ops->chol_f32(ctx->stream, B, CView<float>{(const float*)dM, vM.rs, vM.bs}, View<double>{dU, vU.rs, vU.bs},
+                       View<double>{dD, vD.rs, vD.bs});
+    else
+      m->ops->chol_f64(ctx->stream, B, CView<double>{(const double*)dM, vM.rs, vM.bs},
+                       View<double>{dU, vU.rs, vU.bs}, View<double>{dD, vD.rs, vD.bs});
+    finishCall(ctx, flags, st);
+  });
+}
+
+int rbd_cholesky_solve(rbd_ctx* ctx, const rbd_model* m, int64_t B, const double* U, int64_t ldU, const double* D,
+                       int64_t ldD, int dtype, void* x, int64_t ldx, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(U && D && x, "null array argument");
+    if (!m->compact) throw Error(RBD_EUNSUPPORTED, "cholesky: the model is not DFS-compact");
+    const int nv = m->nv, nnz = (int)m->patRows.size();
+    const bool f32 = dtype == RBD_F32;
+    const size_t es = f32 ? 4 : 8;
+    auto vU = argView<double>(nnz > 0 ? nnz : 1, B, ldU, flags, "U");
+    auto vD = argView<double>(nv, B, ldD, flags, "D");
+    auto vx = argView<float>(nv, B, ldx, flags, "x");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const double* dU = (const double*)st.in(U, vU.extent * 8);
+    const double* dD = (const double*)st.in(D, vD.extent * 8);
+    void* dx = st.out(x, vx.extent * es, true);
+    if (f32)
+      m->ops->cholsolve_f32(ctx->stream, B, CView<double>{dU, vU.rs, vU.bs}, CView<double>{dD, vD.rs, vD.bs},
+                            View<float>{(float*)dx, vx.rs, vx.bs});
+    else
+      m->ops->cholsolve_f64(ctx->stream, B, CView<double>{dU, vU.rs, vU.bs}, CView<double>{dD, vD.rs, vD.bs},
+                            View<double>{(double*)dx, vx.rs, vx.bs});
+    finishCall(ctx, flags, st);
+  });
+}
+
+namespace {
+void checkDerivativeModel(const rbd_model* m) {
+  if (!m->compact) throw Error(RBD_EUNSUPPORTED, "derivatives: the model is not DFS-compact");
+  if (m->gravity[3] != 0.0 || m->gravity[4] != 0.0 || m->gravity[5] != 0.0)
+    throw Error(RBD_EINVAL, "The gravity must be a pure force vector, no angular part");
+  if (!m->ops->rnead_f32) throw Error(RBD_EUNSUPPORTED, "derivatives: no kernel for this topology");
+}
+}  // namespace
+
+int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                         const void* v, int64_t ldv, const void* a, int64_t lda, void* dq, void* dv, void* da,
+                         int64_t ldout, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(q && v && a, "null input argument");
+    if (dtype != RBD_F32) throw Error(RBD_EUNSUPPORTED, "rnea_derivatives: FP32 only");
+    checkDerivativeModel(m);
+    const int nv = m->nv;
+    auto vq = argView<float>(m->nq, B, ldq, flags, "q");
+    auto vv = argView<float>(nv, B, ldv, flags, "v");
+    auto va = argView<float>(nv, B, lda, flags, "a");
+    auto vo = argView<float>(nv * nv, B, ldout, flags, "outputs");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const float* dq_ = (const float*)st.in(q, vq.extent * 4);
+    const float* dv_ = (const float*)st.in(v, vv.extent * 4);
+    const float* da_ = (const float*)st.in(a, va.extent * 4);
+    float* o1 = (float*)st.out(dq, vo.extent * 4, true);
+    float* o2 = (float*)st.out(dv, vo.extent * 4, true);
+    float* o3 = (float*)st.out(da, vo.extent * 4, true);
+    const int mask = (dq ? 1 : 0) | (dv ? 2 : 0) | (da ? 4 : 0);
+    m->ops->rnead_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
+                      CView<float>{da_, va.rs, va.bs}, View<float>{o1, vo.rs, vo.bs}, View<float>{o2, vo.rs, vo.bs},
+                      View<float>{o3, vo.rs, vo.bs}, mask);
+    finishCall(ctx, flags, st);
+  });
+}
+
+int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                        const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq, void* ddq_dv,
+                        void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(q && v && tau && ddq_dq && ddq_dv && ddq_dtau, "null array argument");
+    if (dtype != RBD_F32) throw Error(RBD_EUNSUPPORTED, "aba_derivatives: FP32 only");
+    checkDerivativeModel(m);
+    const int nv = m->nv, nnz = m->ops->nnzU;
+    auto vq = argView<float>(m->nq, B, ldq, flags, "q");
+    auto vv = argView<float>(nv, B, ldv, flags, "v");
+    auto vt = argView<float>(nv, B, ldtau, flags, "tau");
+    auto vo = argView<float>(nv * nv, B, ldout, flags, "outputs");
+    ArgView<float> va{};
+    if (ddq) va = argView<float>(nv, B, ldddq, flags, "ddq");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const float* dq_ = (const float*)st.in(q, vq.extent * 4);
+    const float* dv_ = (const float*)st.in(v, vv.extent * 4);
+    const float* dt_ = (const float*)st.in(tau, vt.extent * 4);
+    float* Xq = (float*)st.out(ddq_dq, vo.extent * 4, true);
+    float* Xv = (float*)st.out(ddq_dv, vo.extent * 4, true);
+    float* Mi = (float*)st.out(ddq_dtau, vo.extent * 4, true);
+    // workspace: ddq (batch-fastest nv x B) unless the caller wants it, then U and D
+    const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
+    float* wsp = (float*)ctx->workspace((nU + nD + (size_t)nv * B) * 4);
+    float* wU = wsp;
+    float* wD = wU + nU;
+    float* wA = wD + nD;
+    View<float> vddq{wA, B, 1};
+    if (ddq) {
+      float* dd = (float*)st.out(ddq, va.extent * 4, true);
+      vddq = View<float>{dd, va.rs, va.bs};
+    }
+    // 1. ddq = aba(q, v, tau)
+    m->ops->aba_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
+                    CView<float>{dt_, vt.rs, vt.bs}, vddq);
+    // 2. dtau_dq, dtau_dv, M at a = ddq, into the output buffers
+    m->ops->rnead_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
+                      CView<float>{vddq.p, vddq.rs, vddq.bs}, View<float>{Xq, vo.rs, vo.bs},
+                      View<float>{Xv, vo.rs, vo.bs}, View<float>{Mi, vo.rs, vo.bs}, 7);
+    // 3. tree factor of M, then Minv and -Minv dtau in place
+    m->ops->chol32_f32(ctx->stream, B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1}, View<float>{wD, B, 1});
+    m->ops->abad_solve_f32(ctx->stream, B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1},
+                           View<float>{Xq, vo.rs, vo.bs}, View<float>{Xv, vo.rs, vo.bs},
+                           View<float>{Mi, vo.rs, vo.bs});
+    finishCall(ctx, flags, st);
+  });
+}
+
+int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t nmodels, const rbd_model* const* models, int64_t ninstances,
+                       const int32_t* model_id, const int64_t* offset_q, const int64_t* offset_v, const float* q,
+                       const float* v, const float* a, const float* tau_in, float* tau_out, float* ddq_out,
+                       uint32_t flags) {
+  (void)nmodels; (void)models; (void)ninstances; (void)model_id; (void)offset_q; (void)offset_v;
+  (void)q; (void)v; (void)a; (void)tau_in; (void)tau_out; (void)ddq_out; (void)flags;
+  g_err = "rbd_multi_rnea_aba: not available in this build";
+  return ctx ? RBD_EUNSUPPORTED : RBD_EINVAL;
+}
+
+int rbd_model_cholesky_pattern(const rbd_model* m, int32_t* nnz, int32_t* rows, int32_t* cols) {
+  return guarded([&] {
+    RBD_CHECK_ARG(m && nnz, "null argument");
+    *nnz = (int32_t)m->patRows.size();
+    if (rows) std::memcpy(rows, m->patRows.data(), m->patRows.size() * sizeof(int32_t));
+    if (cols) std::memcpy(cols, m->patCols.data(), m->patCols.size() * sizeof(int32_t));
+  });
+}
+
+}  // extern "C"
