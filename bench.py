@@ -64,6 +64,9 @@ class Timer:
         ev = self.acc.get(name, [])
         return float(np.mean([s.elapsed_time(e) for s, e in ev])) if ev else float("nan")
 
+    def count(self, name):
+        return len(self.acc.get(name, []))
+
     def reset(self):
         self.acc = {}
 
@@ -93,6 +96,18 @@ def main():
     dev = torch.device("cuda:0")
     ctx = pa.Context(0)
     timer = Timer(torch, ctx.stream)
+
+    if args.only == "copy":
+        # plain device copy of a known size: calibrates the FETCH_SIZE / WRITE_SIZE counters
+        n = 1 << 28  # 1 GiB of float32
+        src = torch.ones(n, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(args.warmup + args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"only": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                          "KiB_per_dispatch": 4 * n / 1024}))
+        return
 
     desc = M.humanoid36()
     model = pa.Model(ctx, desc)
@@ -140,7 +155,7 @@ def main():
         ctx.synchronize()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-    if only not in (None, "step"):
+    if only is not None:
         extra = {}
         if only in ("rnea", "rnead", "abad"):
             Bd = args.deriv_batch
@@ -157,7 +172,9 @@ def main():
             ctx.synchronize()
             extra["ms"] = (time.perf_counter() - t0) / args.steps * 1e3
         else:
-            extra["ms"] = {k: timer.mean_ms(k) for k in ("aba", "crba", "chol")}
+            extra["ms"] = {k: timer.mean_ms(k) for k in ("aba", "crba", "chol") if timer.count(k)}
+            if only == "step":
+                extra["ms_per_step"] = elapsed / args.steps * 1e3
         print(json.dumps({"only": only, **extra}))
         return
 
@@ -228,6 +245,23 @@ def main():
     t = time_call(lambda: em.rnea(Qm, Vm, Am, async_=True), reps=50)
     others["rnea_manipulator6_f64"] = {"batch": 4096, "configs_per_s": 4096 / t, "us": t * 1e6}
 
+    # mixed-robot batch: RNEA + ABA per instance, one graph-replayed launch
+    mix_spec = [("arm7", 40000), ("quadruped18", 40000), ("humanoid36", 15000), ("random60", 5000)]
+    groups = []
+    for j, (name, n) in enumerate(mix_spec):
+        dsc = M.SAMPLE_MODELS[name]()
+        mdl = pa.Model(ctx, dsc)
+        xs = M.sample_inputs(dsc, n, M.INPUT_SEED_BASE + 1 + j)
+        groups.append((mdl, *(torch.tensor(x.astype(np.float32), device=dev) for x in xs)))
+    mb = pa.MixedBatch(ctx, groups)
+    mb.run()
+    tune = mb.tuning()
+    t = time_call(lambda: mb.run(async_=True), reps=10)
+    n_mix = sum(n for _, n in mix_spec)
+    others["mixed_rnea_aba_f32"] = {"instances": n_mix, "groups": dict(mix_spec), "configs_per_s": n_mix / t,
+                                    "ms": t * 1e3, "chosen": tune.get("chosen"),
+                                    "candidates_ms": {c["variant"]: c["ms"] for c in tune.get("candidates", [])}}
+
     # ---------------- accuracy against the FP64 oracle ----------------
     errors = {}
     if not args.no_check:
@@ -254,28 +288,34 @@ def main():
                                                            Am[:, :1024].contiguous()).cpu().numpy(), rm)
 
     # ---------------- CPU baseline (oracle, FP64, OpenMP over configurations) ----------------
+    # The loop is timed inside the oracle library (C++ clock around an OpenMP static
+    # schedule over configurations, one Data per thread, as benchmark/timings-parallel.cpp
+    # drives rneaInParallel / abaInParallel), median of `reps` after one warm-up pass.
     cpu = None
     if not args.no_cpu:
         cores = len(os.sched_getaffinity(0))
         nb = min(B, 65536)
-        qc, vc, tc = q[:, :nb], v[:, :nb], tau[:, :nb]
-
-        def cpu_step():
-            oracle.aba(desc, qc, vc, tc, nthreads=cores)
-            Mc = oracle.crba(desc, qc, nthreads=cores)
-            oracle.cholesky(desc, Mc, nthreads=cores)
-
-        t_end = time.perf_counter() + 3.0  # warm-up (benchmark/timings-parallel.cpp:103-111)
-        while time.perf_counter() < t_end:
-            cpu_step()
-        reps = []
-        for _ in range(5):
-            t0 = time.perf_counter()
-            cpu_step()
-            reps.append(time.perf_counter() - t0)
-        cpu = {"value": nb / float(np.median(reps)), "unit": "configs/s", "cores": cores, "kind": "port",
-               "sample": f"{nb} humanoid36 configurations, ABA(WORLD)+CRBA+Cholesky in FP64, median of 5 "
-                         f"after 3 s warm-up, OpenMP static schedule over configurations"}
+        reps = 9
+        sec = oracle.time_batch(desc, "step", q[:, :nb], v[:, :nb], tau[:, :nb], nthreads=cores, reps=reps)
+        cpu = {"value": nb / sec, "unit": "configs/s", "cores": cores, "kind": "port",
+               "sample": f"{nb} humanoid36 configurations x {reps} repetitions (median), "
+                         f"ABA(WORLD)+CRBA+Cholesky in FP64, OpenMP static schedule over configurations"}
+        cpu_alg = {}
+        for alg, nbb, x in (("rnea", nb, a), ("aba", nb, tau), ("rnea_derivatives", Bd, a),
+                            ("aba_derivatives", Bd, tau)):
+            s_ = oracle.time_batch(desc, alg, q[:, :nbb], v[:, :nbb], x[:, :nbb], nthreads=cores, reps=5)
+            cpu_alg[alg] = {"batch": nbb, "configs_per_s": nbb / s_, "cores": cores}
+        gpu_key = {"rnea": "rnea_humanoid36_f32", "aba": "aba_humanoid36_f32",
+                   "rnea_derivatives": "rnea_derivatives_humanoid36_f32",
+                   "aba_derivatives": "aba_derivatives_humanoid36_f32"}
+        for alg, c in cpu_alg.items():
+            c["gpu_over_cpu"] = others[gpu_key[alg]]["configs_per_s"] / c["configs_per_s"]
+        others["cpu_fp64_oracle"] = cpu_alg
+        # ABA followed by computeABADerivatives per configuration, as one pipeline
+        g = 1.0 / (1.0 / others["aba_humanoid36_f32"]["configs_per_s"] +
+                   1.0 / others["aba_derivatives_humanoid36_f32"]["configs_per_s"])
+        c_ = 1.0 / (1.0 / cpu_alg["aba"]["configs_per_s"] + 1.0 / cpu_alg["aba_derivatives"]["configs_per_s"])
+        others["aba_plus_aba_derivatives_gpu_over_cpu"] = g / c_
 
     out = {
         "metric": METRIC,

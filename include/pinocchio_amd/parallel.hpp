@@ -1,0 +1,239 @@
+// pinocchio_amd/parallel.hpp -- C++ drop-in for pinocchio's batched dynamics calls.
+//
+// Mirrors, over the C ABI of rbd.h, the call shapes of
+//   rneaInParallel(num_threads, pool, q, v, a, tau)   algorithm/parallel/rnea.hpp:39-83
+//   abaInParallel (num_threads, pool, q, v, tau, a)   algorithm/parallel/aba.hpp:40-84
+// and adds the batched counterparts of crba, computeRNEADerivatives and
+// computeABADerivatives that the reference only offers per configuration
+// (crba.hpp:47-52, rnea-derivatives.hpp:110-260, aba-derivatives.hpp:42-318).
+//
+// Matrices are column-major views with one configuration per column, exactly the
+// memory of an Eigen::MatrixXd (pass m.data(), m.rows(), m.cols(), m.outerStride()).
+// With Eigen available, the Eigen overloads at the bottom take the matrices directly.
+// Argument errors throw std::invalid_argument for the reference's conditions
+// (rnea.hpp:53-68, aba.hpp:54-69) plus the tau shape that abaInParallel leaves unchecked.
+// Calls are synchronous; num_threads is validated against pool.size() and otherwise
+// ignored (one GPU does the whole batch).  No global OpenMP state is touched.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../rbd.h"
+
+namespace pinocchio_amd {
+
+inline void check(int rc) {
+  if (rc == RBD_OK) return;
+  const std::string msg = rbd_last_error();
+  if (rc == RBD_EINVAL) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+// A column-major matrix view: element (r, c) at data[r + c * ld].
+template<class S> struct ColView {
+  S* data;
+  int64_t rows, cols, ld;
+};
+using CMat = ColView<const double>;
+using MMat = ColView<double>;
+
+// Holds one model on one GPU: the counterpart of ModelPoolTpl (pool/model.hpp:19-165),
+// whose per-thread Model/Data copies are not needed on the GPU.
+class ModelPool {
+ public:
+  explicit ModelPool(const rbd_model_desc& desc, size_t pool_size = 1, int device = 0) : size_(pool_size) {
+    if (pool_size == 0) throw std::invalid_argument("The pool size should be greater than zero");
+    check(rbd_ctx_create(device, &ctx_));
+    const int rc = rbd_model_create(ctx_, &desc, &model_);
+    if (rc != RBD_OK) {
+      rbd_ctx_destroy(ctx_);
+      check(rc);
+    }
+    int32_t nq, nv, nj, compact;
+    check(rbd_model_dims(model_, &nq, &nv, &nj, &compact));
+    nq_ = nq;
+    nv_ = nv;
+  }
+  ~ModelPool() {
+    rbd_model_destroy(model_);
+    rbd_ctx_destroy(ctx_);
+  }
+  ModelPool(const ModelPool&) = delete;
+  ModelPool& operator=(const ModelPool&) = delete;
+
+  size_t size() const { return size_; }
+  int nq() const { return nq_; }
+  int nv() const { return nv_; }
+  rbd_ctx* ctx() const { return ctx_; }
+  rbd_model* model() const { return model_; }
+
+ private:
+  size_t size_;
+  rbd_ctx* ctx_ = nullptr;
+  rbd_model* model_ = nullptr;
+  int nq_ = 0, nv_ = 0;
+};
+
+namespace detail {
+inline void checkThreads(size_t num_threads, const ModelPool& pool) {
+  if (pool.size() == 0) throw std::invalid_argument("The pool should have at least one element");
+  if (num_threads > pool.size()) throw std::invalid_argument("The number of threads should be lower than the pool size");
+}
+template<class V> void checkRows(const V& m, int64_t rows, const char* what) {
+  if (m.rows != rows) throw std::invalid_argument(std::string(what) + " has wrong number of rows");
+}
+template<class V> void checkCols(const V& m, int64_t cols, const char* what) {
+  if (m.cols != cols) throw std::invalid_argument(std::string(what) + " has wrong number of columns");
+}
+// Runs one batched vector op in FP32 or FP64 on host matrices.
+template<class Fn>
+void vecOp(Fn fn, const ModelPool& pool, bool fp32, const CMat& x0, const CMat& x1, const CMat& x2, const MMat& out) {
+  const int64_t B = x0.cols;
+  if (!fp32) {
+    check(fn(pool.ctx(), pool.model(), B, RBD_F64, x0.data, x0.ld, x1.data, x1.ld, x2.data, x2.ld, out.data, out.ld,
+             RBD_HOST_PTRS));
+    return;
+  }
+  // FP32 compute: round the inputs, widen the result
+  auto pack = [B](const CMat& m) {
+    std::vector<float> f((size_t)(m.rows * B));
+    for (int64_t c = 0; c < B; ++c)
+      for (int64_t r = 0; r < m.rows; ++r) f[(size_t)(r + c * m.rows)] = (float)m.data[r + c * m.ld];
+    return f;
+  };
+  std::vector<float> f0 = pack(x0), f1 = pack(x1), f2 = pack(x2), fo((size_t)(out.rows * B));
+  check(fn(pool.ctx(), pool.model(), B, RBD_F32, f0.data(), x0.rows, f1.data(), x1.rows, f2.data(), x2.rows,
+           fo.data(), out.rows, RBD_HOST_PTRS));
+  for (int64_t c = 0; c < B; ++c)
+    for (int64_t r = 0; r < out.rows; ++r) out.data[r + c * out.ld] = fo[(size_t)(r + c * out.rows)];
+}
+}  // namespace detail
+
+// tau.col(i) = rnea(q.col(i), v.col(i), a.col(i)) for every column (FP64 by default).
+inline void rneaInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v, const CMat& a,
+                           const MMat& tau, bool fp32 = false) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(v, pool.nv(), "The velocity matrix");
+  detail::checkRows(a, pool.nv(), "The acceleration matrix");
+  detail::checkRows(tau, pool.nv(), "The output matrix");
+  detail::checkCols(v, q.cols, "The velocity matrix");
+  detail::checkCols(a, q.cols, "The acceleration matrix");
+  detail::checkCols(tau, q.cols, "The output matrix");
+  detail::vecOp(&rbd_rnea, pool, fp32, q, v, a, tau);
+}
+
+// a.col(i) = aba(q.col(i), v.col(i), tau.col(i)); the engine computes in FP32.
+inline void abaInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v, const CMat& tau,
+                          const MMat& a) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(v, pool.nv(), "The velocity matrix");
+  detail::checkRows(tau, pool.nv(), "The torque matrix");
+  detail::checkRows(a, pool.nv(), "The output matrix");
+  detail::checkCols(v, q.cols, "The velocity matrix");
+  detail::checkCols(tau, q.cols, "The torque matrix");
+  detail::checkCols(a, q.cols, "The output matrix");
+  detail::vecOp(&rbd_aba, pool, true, q, v, tau, a);
+}
+
+// M_i = crba(q.col(i)) for every column, written as nv x nv blocks side by side in M
+// (nv rows, nv * B columns), symmetric.
+inline void crbaInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const MMat& M) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(M, pool.nv(), "The output matrix");
+  detail::checkCols(M, q.cols * pool.nv(), "The output matrix");
+  if (M.ld != pool.nv()) throw std::invalid_argument("The output matrix must be contiguous");
+  check(rbd_crba(pool.ctx(), pool.model(), q.cols, RBD_F64, q.data, q.ld, M.data, (int64_t)pool.nv() * pool.nv(),
+                 RBD_HOST_PTRS));
+}
+
+namespace detail {
+inline std::vector<float> toF32(const CMat& m) {
+  std::vector<float> f((size_t)(m.rows * m.cols));
+  for (int64_t c = 0; c < m.cols; ++c)
+    for (int64_t r = 0; r < m.rows; ++r) f[(size_t)(r + c * m.rows)] = (float)m.data[r + c * m.ld];
+  return f;
+}
+inline void fromF32(const std::vector<float>& f, const MMat& m) {
+  for (int64_t c = 0; c < m.cols; ++c)
+    for (int64_t r = 0; r < m.rows; ++r) m.data[r + c * m.ld] = f[(size_t)(r + c * m.rows)];
+}
+}  // namespace detail
+
+// computeRNEADerivatives per column: three nv x (nv * B) outputs (blocks side by side).
+inline void computeRNEADerivativesInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
+                                             const CMat& a, const MMat& dtau_dq, const MMat& dtau_dv,
+                                             const MMat& dtau_da) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(v, pool.nv(), "The velocity matrix");
+  detail::checkRows(a, pool.nv(), "The acceleration matrix");
+  detail::checkCols(v, q.cols, "The velocity matrix");
+  detail::checkCols(a, q.cols, "The acceleration matrix");
+  for (const MMat* o : {&dtau_dq, &dtau_dv, &dtau_da}) {
+    detail::checkRows(*o, pool.nv(), "An output matrix");
+    detail::checkCols(*o, q.cols * pool.nv(), "An output matrix");
+  }
+  const int64_t nn = (int64_t)pool.nv() * pool.nv();
+  std::vector<float> fq = detail::toF32(q), fv = detail::toF32(v), fa = detail::toF32(a);
+  std::vector<float> o1((size_t)(nn * q.cols)), o2(o1.size()), o3(o1.size());
+  check(rbd_rnea_derivatives(pool.ctx(), pool.model(), q.cols, RBD_F32, fq.data(), q.rows, fv.data(), v.rows,
+                             fa.data(), a.rows, o1.data(), o2.data(), o3.data(), nn, RBD_HOST_PTRS));
+  detail::fromF32(o1, dtau_dq);
+  detail::fromF32(o2, dtau_dv);
+  detail::fromF32(o3, dtau_da);
+}
+
+// computeABADerivatives per column: ddq_dq, ddq_dv, ddq_dtau (= Minv) as nv x (nv * B).
+inline void computeABADerivativesInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
+                                            const CMat& tau, const MMat& ddq_dq, const MMat& ddq_dv,
+                                            const MMat& ddq_dtau) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(v, pool.nv(), "The velocity matrix");
+  detail::checkRows(tau, pool.nv(), "The torque matrix");
+  detail::checkCols(v, q.cols, "The velocity matrix");
+  detail::checkCols(tau, q.cols, "The torque matrix");
+  for (const MMat* o : {&ddq_dq, &ddq_dv, &ddq_dtau}) {
+    detail::checkRows(*o, pool.nv(), "An output matrix");
+    detail::checkCols(*o, q.cols * pool.nv(), "An output matrix");
+  }
+  const int64_t nn = (int64_t)pool.nv() * pool.nv();
+  std::vector<float> fq = detail::toF32(q), fv = detail::toF32(v), ft = detail::toF32(tau);
+  std::vector<float> o1((size_t)(nn * q.cols)), o2(o1.size()), o3(o1.size());
+  check(rbd_aba_derivatives(pool.ctx(), pool.model(), q.cols, RBD_F32, fq.data(), q.rows, fv.data(), v.rows,
+                            ft.data(), tau.rows, o1.data(), o2.data(), o3.data(), nn, nullptr, 0, RBD_HOST_PTRS));
+  detail::fromF32(o1, ddq_dq);
+  detail::fromF32(o2, ddq_dv);
+  detail::fromF32(o3, ddq_dtau);
+}
+
+}  // namespace pinocchio_amd
+
+#if __has_include(<Eigen/Core>)
+#include <Eigen/Core>
+namespace pinocchio_amd {
+template<class D> CMat view(const Eigen::MatrixBase<D>& m) {
+  return CMat{m.derived().data(), m.rows(), m.cols(), m.derived().outerStride()};
+}
+template<class D> MMat mview(const Eigen::MatrixBase<D>& m) {
+  auto& mm = const_cast<Eigen::MatrixBase<D>&>(m).derived();
+  return MMat{mm.data(), mm.rows(), mm.cols(), mm.outerStride()};
+}
+template<class Q, class V, class A, class T>
+void rneaInParallel(size_t n, ModelPool& pool, const Eigen::MatrixBase<Q>& q, const Eigen::MatrixBase<V>& v,
+                    const Eigen::MatrixBase<A>& a, const Eigen::MatrixBase<T>& tau) {
+  rneaInParallel(n, pool, view(q), view(v), view(a), mview(tau));
+}
+template<class Q, class V, class T, class A>
+void abaInParallel(size_t n, ModelPool& pool, const Eigen::MatrixBase<Q>& q, const Eigen::MatrixBase<V>& v,
+                   const Eigen::MatrixBase<T>& tau, const Eigen::MatrixBase<A>& a) {
+  abaInParallel(n, pool, view(q), view(v), view(tau), mview(a));
+}
+}  // namespace pinocchio_amd
+#endif

@@ -141,16 +141,31 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int
                         int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq,
                         void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags);
 
-/* Mixed-model batch: instances of up to 8 models in one launch, each instance running
- * RNEA (tau from q, v, a) and ABA (ddq from q, v, tau_in).  Instance k uses model
- * models[model_id[k]]; its vectors start at offset_q[k] / offset_v[k] elements in the
- * packed float32 device arrays q, v, a, tau_in, tau_out, ddq_out (each instance's
- * vector contiguous).  Device pointers only.  The per-model tile width is chosen by the
- * autotuner on first use. */
-int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t nmodels, const rbd_model* const* models, int64_t ninstances,
-                       const int32_t* model_id, const int64_t* offset_q, const int64_t* offset_v,
-                       const float* q, const float* v, const float* a, const float* tau_in, float* tau_out,
-                       float* ddq_out, uint32_t flags);
+/* Mixed-robot batch (no reference counterpart: the reference batches one model per
+ * call).  Each group is `count` instances of one model, with FP32 device arrays in the
+ * batch-fastest layout (element (k, b) at p[k * count + b]); every instance runs RNEA
+ * (tau_out from q, v, a) and ABA (ddq_out from q, v, tau).  All groups go to the GPU in
+ * one kernel launch over a cost-ordered tile table.  On the first call with a given set
+ * of groups the autotuner times the candidate tile widths (64-instance chunks per
+ * workgroup, per model) and a per-model launch variant, keeps the fastest and captures
+ * it in a HIP graph that later calls with the same arguments replay. */
+typedef struct rbd_batch_group {
+  const rbd_model* model;
+  int64_t count;
+  const float* q;
+  const float* v;
+  const float* a;
+  const float* tau;
+  float* tau_out;
+  float* ddq_out;
+} rbd_batch_group;
+
+#define RBD_NO_GRAPH 0x10u    /* rbd_multi_rnea_aba: launch directly, no graph capture/replay */
+#define RBD_NO_AUTOTUNE 0x20u /* rbd_multi_rnea_aba: use the default tile widths */
+
+int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t ngroups, const rbd_batch_group* groups, uint32_t flags);
+/* The autotuner's last decision and timings as a JSON string (NUL-terminated, truncated to len). */
+int rbd_multi_tuning(rbd_ctx* ctx, char* buf, int32_t len);
 
 #ifdef __cplusplus
 }

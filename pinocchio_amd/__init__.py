@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libpinocchio_amd.so")
 
 RBD_F32, RBD_F64 = 0, 1
 RBD_HOST_PTRS, RBD_ASYNC, RBD_LAYOUT_BATCH, RBD_UPPER_ONLY = 0x1, 0x2, 0x4, 0x8
+RBD_NO_GRAPH, RBD_NO_AUTOTUNE = 0x10, 0x20
 _ERRORS = {1: ValueError, 2: NotImplementedError, 3: RuntimeError, 4: MemoryError}
 
 _lib = None
@@ -160,13 +161,16 @@ class Engine:
             return torch.empty((B, rows), dtype=like.dtype, device=like.device)
         return torch.empty((rows, B), dtype=like.dtype, device=like.device)
 
-    def _new_mat(self, like, B, dtype=None):
+    def _new_mat(self, like, B, dtype=None, zero=False):
+        # every kernel writes every entry of its matrices (structural zeros included),
+        # so fresh outputs need no fill unless only one triangle will be written
         import torch
         nv = self.model.nv
         dt = dtype or like.dtype
+        alloc = torch.zeros if zero else torch.empty
         if self.layout == "col":
-            return torch.zeros((B, nv, nv), dtype=dt, device=like.device)
-        return torch.zeros((nv * nv, B), dtype=dt, device=like.device)
+            return alloc((B, nv, nv), dtype=dt, device=like.device)
+        return alloc((nv * nv, B), dtype=dt, device=like.device)
 
     def _mat_ld(self, t):
         nv = self.model.nv
@@ -207,7 +211,7 @@ class Engine:
         """Joint-space inertia matrix per configuration (symmetric unless upper_only)."""
         m = self.model
         B, ldq = self._vec(q, m.nq, "q")
-        out = self._new_mat(q, B) if out is None else out
+        out = self._new_mat(q, B, zero=upper_only) if out is None else out
         ldm = self._mat_arg(out, B, "M")
         flags = self._flags(async_) | (RBD_UPPER_ONLY if upper_only else 0)
         _check(lib().rbd_crba(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(out), _i64(ldm),
@@ -223,12 +227,12 @@ class Engine:
         ldm = self._mat_arg(M, B, "M")
         nnz = len(self.model.cholesky_pattern()[0])
         if self.layout == "col":
-            U = torch.zeros((B, max(nnz, 1)), dtype=torch.float64, device=M.device)
-            D = torch.zeros((B, m.nv), dtype=torch.float64, device=M.device)
+            U = torch.empty((B, max(nnz, 1)), dtype=torch.float64, device=M.device)
+            D = torch.empty((B, m.nv), dtype=torch.float64, device=M.device)
             ldu, ldd = max(nnz, 1), m.nv
         else:
-            U = torch.zeros((max(nnz, 1), B), dtype=torch.float64, device=M.device)
-            D = torch.zeros((m.nv, B), dtype=torch.float64, device=M.device)
+            U = torch.empty((max(nnz, 1), B), dtype=torch.float64, device=M.device)
+            D = torch.empty((m.nv, B), dtype=torch.float64, device=M.device)
             ldu = ldd = max(B, 1)
         _check(lib().rbd_cholesky_decompose(m.ctx.handle, m.handle, _i64(B), _dtype_code(M), _ptr(M), _i64(ldm),
                                             _ptr(U), _i64(ldu), _ptr(D), _i64(ldd),
@@ -281,6 +285,53 @@ class Engine:
             return a.transpose(0, 2, 1)  # [b, c, r] -> [b, r, c]
         B = a.shape[1]
         return a.reshape(nv, nv, B).transpose(2, 1, 0)  # [c, r, b] -> [b, r, c]
+
+
+# ------------------------------------------------------------------------------------
+# Mixed-robot batches (one launch over instances of several models)
+# ------------------------------------------------------------------------------------
+class _BatchGroup(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_void_p), ("count", ctypes.c_int64), ("q", ctypes.c_void_p),
+                ("v", ctypes.c_void_p), ("a", ctypes.c_void_p), ("tau", ctypes.c_void_p),
+                ("tau_out", ctypes.c_void_p), ("ddq_out", ctypes.c_void_p)]
+
+
+class MixedBatch:
+    """RNEA + ABA over groups of instances of different models in one kernel launch.
+
+    ``groups`` is a list of (Model, q, v, a, tau) with FP32 device tensors in the
+    batch-fastest layout ((nq, count) and (nv, count), contiguous).  Outputs (tau_out,
+    ddq_out) are allocated once; the first call autotunes and captures a HIP graph that
+    later calls replay (inputs may be rewritten in place between calls)."""
+
+    def __init__(self, ctx: Context, groups, graph=True, autotune=True):
+        import torch
+        self.ctx = ctx
+        self.groups = []
+        arr = (_BatchGroup * len(groups))()
+        for k, (m, q, v, a, tau) in enumerate(groups):
+            n = q.shape[1]
+            for t, rows, name in ((q, m.nq, "q"), (v, m.nv, "v"), (a, m.nv, "a"), (tau, m.nv, "tau")):
+                if t.dtype != torch.float32 or tuple(t.shape) != (rows, n) or not t.is_contiguous():
+                    raise ValueError(f"group {k}: {name} must be a contiguous float32 ({rows}, {n}) tensor")
+            tau_out = torch.empty((m.nv, n), dtype=torch.float32, device=q.device)
+            ddq_out = torch.empty((m.nv, n), dtype=torch.float32, device=q.device)
+            self.groups.append((m, q, v, a, tau, tau_out, ddq_out))
+            arr[k] = _BatchGroup(m.handle.value, n, q.data_ptr(), v.data_ptr(), a.data_ptr(), tau.data_ptr(),
+                                 tau_out.data_ptr(), ddq_out.data_ptr())
+        self._arr = arr
+        self._flags = (0 if graph else RBD_NO_GRAPH) | (0 if autotune else RBD_NO_AUTOTUNE)
+
+    def run(self, async_=False):
+        _check(lib().rbd_multi_rnea_aba(self.ctx.handle, ctypes.c_int32(len(self.groups)), self._arr,
+                                        ctypes.c_uint32(self._flags | (RBD_ASYNC if async_ else 0))))
+        return [(g[5], g[6]) for g in self.groups]
+
+    def tuning(self) -> dict:
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().rbd_multi_tuning(self.ctx.handle, buf, ctypes.c_int32(len(buf))))
+        return json.loads(buf.value.decode()) if buf.value else {}
 
 
 # ------------------------------------------------------------------------------------

@@ -40,14 +40,10 @@ void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v,
 }
 template<class TP, class S>
 void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv) {
-  hipLaunchKernelGGL((k_abad_solve<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, U, D, Xq, Xv, Minv);
+  hipLaunchKernelGGL((k_abad_solve<TP, S>), dim3(gridFor(B)), dim3(kTile * kSolveWaves), 0, st, B, U, D, Xq, Xv,
+                     Minv);
 }
-template<class TP> constexpr int nnzUpper() {
-  int n = 0;
-  for (int c = 0; c < TP::T.nv; ++c)
-    for (int x = TP::T.dofParent[c]; x >= 0; x = TP::T.dofParent[x]) ++n;
-  return n;
-}
+
 
 template<class TP> TopoOps makeOpsBase() {
   TopoOps o{};
@@ -79,7 +75,7 @@ template<class TP> void add_chol(TopoOps& o) {
   o.cholsolve_f32 = &launchCholSolve<TP, float>;
   o.cholsolve_f64 = &launchCholSolve<TP, double>;
   o.chol32_f32 = &launchChol32<TP>;
-  o.nnzU = nnzUpper<TP>();
+  o.nnzU = nnzOf<TP>();
 }
 template<class TP> void add_rnead(TopoOps& o) { o.rnead_f32 = &launchRneaD<TP, float>; }
 template<class TP> void add_abads(TopoOps& o) { o.abad_solve_f32 = &launchAbaDSolve<TP, float>; }

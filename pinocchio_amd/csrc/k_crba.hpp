@@ -100,13 +100,18 @@ __global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_
       for (int k = 0; k < nvi; ++k) {
         const int c = iv + k;
         const Frc<S> F = cmul(Y, worldAxis<ty>(oM[i], k));
-        // own block (all dofs of this joint)
+        // own block (all dofs of this joint): one product per pair, written to both
+        // triangles (the reference writes the whole row block, so a free-flyer's 6x6
+        // block is full even in upper-only mode)
 #pragma unroll
-        for (int k2 = 0; k2 < nvi; ++k2) {
+        for (int k2 = 0; k2 <= k; ++k2) {
           const int r = iv + k2;
           S m = dot(worldAxis<ty>(oM[i], k2), F);
           if (r == c) m += mc[mcArmature<NJ>() + c];
-          if (live && (!upperOnly || r <= c || ty == JT_FF)) stMat(gM, b, NV, r, c, m);
+          if (live) {
+            stMat(gM, b, NV, r, c, m);
+            if (r != c) stMat(gM, b, NV, c, r, m);
+          }
         }
         // ancestors
         static_for<1, NJ>([&](auto A) {
@@ -131,11 +136,13 @@ __global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_
       }
     }
   });
-  // structural zeros
-  if (live && !upperOnly) {
+  // structural zeros (upper triangle only in upper-only mode)
+  if (live) {
     static_for<0, NV>([&](auto C) {
       static_for<0, NV>([&](auto R) {
-        if constexpr (!CrbaPattern<TP>::nz(R, C)) stMat(gM, b, NV, R, C, S(0));
+        if constexpr (!CrbaPattern<TP>::nz(R, C)) {
+          if (!upperOnly || R <= C) stMat(gM, b, NV, R, C, S(0));
+        }
       });
     });
   }
@@ -146,6 +153,14 @@ __global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_
 // structural entries (any dtype), U written packed (strictly-upper pattern, column by
 // column, rows from the root down) and D as nv doubles.
 // ---------------------------------------------------------------------------------------
+// number of strictly-upper structural non-zeros of M (the packed factor's length)
+template<class TP> constexpr int nnzOf() {
+  int n = 0;
+  for (int c = 0; c < TP::T.nv; ++c)
+    for (int x = TP::T.dofParent[c]; x >= 0; x = TP::T.dofParent[x]) ++n;
+  return n;
+}
+
 template<class TP> struct CholPattern {
   static constexpr auto& T = TP::T;
   static constexpr int NV = T.nv;

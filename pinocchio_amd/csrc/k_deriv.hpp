@@ -251,79 +251,117 @@ __global__ __launch_bounds__(64) void k_rnead(const S* __restrict__ mc_in, int64
 }
 
 // ---------------------------------------------------------------------------------------
-// ABA-derivative solve: with the packed tree factor of M (from k_chol<.., S, S>), reads
-// dtau_dq (from gXq) and dtau_dv (from gXv); writes -Minv dtau_dq, -Minv dtau_dv in place
-// and Minv into gMinv.  The whole factor is held in registers across the 3 nv solves.
+// ABA-derivative solve.  Input: the packed FP32 tree factor of M (k_chol<.., float,
+// float>, batch-fastest workspace), dtau_dq in gXq and dtau_dv in gXv.  Output, in
+// place: -Minv dtau_dq, -Minv dtau_dv, and Minv in gMinv.
+//
+// A workgroup owns 64 consecutive configurations: their factors (nnz + nv scalars each)
+// are staged in LDS as [entry][configuration], and its waves share out the 3 nv
+// right-hand sides, two at a time; lane b of a wave solves configuration b0 + b, so
+// every column load and store is one coalesced 256-byte row in the batch-fastest
+// layout and every factor read is a conflict-free LDS row.
 // ---------------------------------------------------------------------------------------
+constexpr int kSolveWaves = 8;
+
 template<class TP, class S>
-__global__ __launch_bounds__(64) void k_abad_solve(int64_t B, CView<S> gU, CView<S> gD, View<S> gXq, View<S> gXv,
-                                                   View<S> gMinv) {
+__global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CView<S> gU, CView<S> gD, View<S> gXq,
+                                                                 View<S> gXv, View<S> gMinv) {
   constexpr auto& T = TP::T;
   constexpr int NV = T.nv;
+  constexpr int NNZ = nnzOf<TP>();
   using P = CholPattern<TP>;
-  const int L = laneId();
-  const int64_t b = int64_t(blockIdx.x) * kTile + L;
-  if (b >= B) return;
-  auto ld = [&](View<S> g, int r, int c) { return g.p[(int64_t)(r + c * NV) * g.rs + b * g.bs]; };
-
-  S U[NV][NV];
-  S Dinv[NV];
-  {
-    const S* __restrict__ pU = gU.p + b * gU.bs;
-    const S* __restrict__ pD = gD.p + b * gD.bs;
-    static_for<0, NV>([&](auto CC) {
-      constexpr int c = CC;
-      Dinv[c] = S(1) / pD[c * gD.rs];
-      static_for<0, NV>([&](auto RR) {
-        constexpr int r = RR;
-        if constexpr (P::anc(r, c)) {
-          constexpr int pk = P::packed(r, c);
-          U[r][c] = pU[pk * gU.rs];
-        }
-      });
-    });
+  __shared__ S sU[(NNZ + NV) * kTile];  // U entries then D^-1, [entry][lane]
+  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
+  const int L = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  for (int e = threadIdx.x; e < (NNZ + NV) * kTile; e += 64 * kSolveWaves) {
+    const int k = e >> 6, b = e & 63;
+    S x = S(1);
+    if (b < nb) x = k < NNZ ? gU.p[(int64_t)k * gU.rs + (b0 + b) * gU.bs] : S(1) / gD.p[(int64_t)(k - NNZ) * gD.rs + (b0 + b) * gD.bs];
+    sU[e] = x;
   }
+  __syncthreads();
+  const bool live = L < nb;
+  const int64_t b = b0 + L;
 
-  // x <- M^-1 x  (cholesky::solve: Uiv, D^-1, Utiv, cholesky.hxx:466-520)
-  auto solve = [&](S (&x)[NV]) {
+  // two right-hand sides at a time: x <- M^-1 x (Uiv, D^-1, Utiv; cholesky.hxx:466-520).
+  // A compiler memory clobber at every elimination step keeps the factor reads where they
+  // are: without it they are hoisted out of the job loop and all 337 values stay live.
+  const S* su = sU + L;
+  auto solve2 = [&](S (&x)[NV], S (&y)[NV]) {
     static_for<0, NV>([&](auto KK) {
       constexpr int k = NV - 1 - KK;
       stepFence();
+      asm volatile("" ::: "memory");
       static_for<k + 1, NV>([&](auto EE) {
         constexpr int e2 = EE;
-        if constexpr (P::anc(k, e2)) x[k] -= U[k][e2] * x[e2];
+        if constexpr (P::anc(k, e2)) {
+          constexpr int pk = P::packed(k, e2);
+          const S u = su[pk * kTile];
+          x[k] -= u * x[e2];
+          y[k] -= u * y[e2];
+        }
       });
+      pin(x[k], y[k]);
     });
+    asm volatile("" ::: "memory");
 #pragma unroll
-    for (int k = 0; k < NV; ++k) x[k] *= Dinv[k];
+    for (int k = 0; k < NV; ++k) {
+      const S di = su[(NNZ + k) * kTile];
+      x[k] *= di;
+      y[k] *= di;
+    }
     static_for<0, NV>([&](auto KK) {
       constexpr int k = KK;
       stepFence();
+      asm volatile("" ::: "memory");
       static_for<k + 1, NV>([&](auto EE) {
         constexpr int e2 = EE;
-        if constexpr (P::anc(k, e2)) x[e2] -= U[k][e2] * x[k];
+        if constexpr (P::anc(k, e2)) {
+          constexpr int pk = P::packed(k, e2);
+          const S u = su[pk * kTile];
+          x[e2] -= u * x[k];
+          y[e2] -= u * y[k];
+          pin(x[e2], y[e2]);
+        }
       });
     });
   };
-
+  // job j < NV: column j of dtau_dq; < 2 NV: of dtau_dv; else unit column j - 2 NV
+  // the three outputs share one layout (same rs, bs): select the base pointer only
+  const int64_t rs = gXq.rs, lane = b * gXq.bs;
+  auto colPtr = [&](int j) -> S* {
+    S* base = j < NV ? gXq.p : (j < 2 * NV ? gXv.p : gMinv.p);
+    const int c = j < NV ? j : (j < 2 * NV ? j - NV : j - 2 * NV);
+    return base + (int64_t)(c * NV) * rs + lane;
+  };
+  auto load = [&](int j, S (&x)[NV]) {
+    if (j < 2 * NV && live) {
+      const S* p = colPtr(j);
+#pragma unroll
+      for (int r = 0; r < NV; ++r) x[r] = p[r * rs];
+    } else {
+      const int u = j - 2 * NV;
+#pragma unroll
+      for (int r = 0; r < NV; ++r) x[r] = (r == u) ? S(1) : S(0);
+    }
+  };
+  auto store = [&](int j, const S (&x)[NV]) {
+    if (!live || j >= 3 * NV) return;
+    S* p = colPtr(j);
+    const S sgn = j < 2 * NV ? S(-1) : S(1);
+#pragma unroll
+    for (int r = 0; r < NV; ++r) p[r * rs] = sgn * x[r];
+  };
 #pragma unroll 1
-  for (int c = 0; c < NV; ++c) {
-    S x[NV];
-#pragma unroll
-    for (int r = 0; r < NV; ++r) x[r] = ld(gXq, r, c);
-    solve(x);
-#pragma unroll
-    for (int r = 0; r < NV; ++r) gXq.p[(int64_t)(r + c * NV) * gXq.rs + b * gXq.bs] = -x[r];
-#pragma unroll
-    for (int r = 0; r < NV; ++r) x[r] = ld(gXv, r, c);
-    solve(x);
-#pragma unroll
-    for (int r = 0; r < NV; ++r) gXv.p[(int64_t)(r + c * NV) * gXv.rs + b * gXv.bs] = -x[r];
-#pragma unroll
-    for (int r = 0; r < NV; ++r) x[r] = (r == c) ? S(1) : S(0);
-    solve(x);
-#pragma unroll
-    for (int r = 0; r < NV; ++r) gMinv.p[(int64_t)(r + c * NV) * gMinv.rs + b * gMinv.bs] = x[r];
+  for (int j = 2 * w; j < 3 * NV; j += 2 * kSolveWaves) {
+    S x[NV], y[NV];
+    load(j, x);
+    load(j + 1 < 3 * NV ? j + 1 : j, y);
+    solve2(x, y);
+    store(j, x);
+    if (j + 1 < 3 * NV) store(j + 1, y);
   }
 }
 

@@ -34,16 +34,19 @@ template<class S> __device__ __forceinline__ Mot<S> mcGravityNeg(const S* __rest
 // ---------------------------------------------------------------------------------------
 // RNEA: tau = M(q) a + C(q, v) v + g(q) + armature * a
 // ---------------------------------------------------------------------------------------
+// LDS rows a tile of this topology needs (q, v and one nv-vector in/out)
+template<class TP> constexpr int dynLdsRows() { return TP::T.nq + 2 * TP::T.nv; }
+
+// One 64-configuration tile starting at b0; `lds` holds dynLdsRows<TP>() * kPitch scalars.
 template<class TP, class S>
-__global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
-                                             CView<S> ga, View<S> gtau) {
+__device__ __forceinline__ void rneaTile(const S* __restrict__ mc_in, int64_t B, int64_t b0, CView<S> gq,
+                                         CView<S> gv, CView<S> ga, View<S> gtau, S* __restrict__ lds) {
   using C = DynCfg<TP, S>;
   constexpr auto& T = TP::T;
   constexpr int NJ = C::NJ, NQ = C::NQ, NV = C::NV;
-  __shared__ S sq[NQ * kPitch];
-  __shared__ S sv[NV * kPitch];
-  __shared__ S sa[NV * kPitch];  // a in, tau out
-  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  S* sq = lds;
+  S* sv = lds + NQ * kPitch;
+  S* sa = lds + (NQ + NV) * kPitch;  // a in, tau out
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
   stageIn<NQ>(sq, gq, b0, nb);
   stageIn<NV>(sv, gv, b0, nb);
@@ -92,6 +95,10 @@ __global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc_in, int64_
       aj[i] = ai;
       const Body<S> Y = mcBody(mc, i);
       fj[i] = inertiaMul(Y, ai) + crossMF(vi, inertiaMul(Y, vi));
+      pinAll<S>(vj[i]);
+      pinAll<S>(aj[i]);
+      pinAll<S>(fj[i]);
+      if constexpr (ty != JT_FF) pin(sn[i], cs[i]);
     } else {
       if constexpr (ty == JT_FF) {
         const S f6[6] = {fj[i].l.x, fj[i].l.y, fj[i].l.z, fj[i].a.x, fj[i].a.y, fj[i].a.z};
@@ -114,6 +121,7 @@ __global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc_in, int64_
         if constexpr (p > 0) {
           const Xform<S> X = revoluteX(mc, i, ax, sn[i], cs[i]);
           fj[p] = fj[p] + act(X, fj[i]);
+          pinAll<S>(fj[p]);
         }
       }
     }
@@ -122,19 +130,25 @@ __global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc_in, int64_
   stageOut<NV>(sa, gtau, b0, nb);
 }
 
+template<class TP, class S>
+__global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc, int64_t B, CView<S> gq, CView<S> gv,
+                                             CView<S> ga, View<S> gtau) {
+  __shared__ S lds[dynLdsRows<TP>() * kPitch];
+  rneaTile<TP, S>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, ga, gtau, lds);
+}
+
 // ---------------------------------------------------------------------------------------
 // ABA: ddq = FD(q, v, tau)
 // ---------------------------------------------------------------------------------------
 template<class TP, class S>
-__global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
-                                            CView<S> gt, View<S> gddq) {
+__device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, int64_t b0, CView<S> gq,
+                                        CView<S> gv, CView<S> gt, View<S> gddq, S* __restrict__ lds) {
   using C = DynCfg<TP, S>;
   constexpr auto& T = TP::T;
   constexpr int NJ = C::NJ, NQ = C::NQ, NV = C::NV;
-  __shared__ S sq[NQ * kPitch];
-  __shared__ S sv[NV * kPitch];
-  __shared__ S st[NV * kPitch];  // tau in, ddq out
-  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  S* sq = lds;
+  S* sv = lds + NQ * kPitch;
+  S* st = lds + (NQ + NV) * kPitch;  // tau in, ddq out
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
   stageIn<NQ>(sq, gq, b0, nb);
   stageIn<NV>(sv, gv, b0, nb);
@@ -182,6 +196,9 @@ __global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc_in, int64_t
       Mot<S> ci = crossMM(vi, jv);
       if constexpr (p > 0) chat[i] = actInv(X, chat[p]) + ci;
       else chat[i] = ci;
+      pinAll<S>(vj[i]);
+      pinAll<S>(chat[i]);
+      if constexpr (ty != JT_FF) pin(sn[i], cs[i]);
     } else {
       // ---- CLOSE(i): articulated inertia and bias force of the subtree ----
       const Body<S> Y = mcBody(mc, i);
@@ -212,6 +229,7 @@ __global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc_in, int64_t
         for (int r = 0; r < 6; ++r) st[(iv + r) * kPitch + L] = S(x[r]);
         ahat[i] = {{ap.l.x + S(x[0]), ap.l.y + S(x[1]), ap.l.z + S(x[2])},
                    {ap.a.x + S(x[3]), ap.a.y + S(x[4]), ap.a.z + S(x[5])}};
+        pinAll<S>(ahat[i]);
       } else {
         constexpr int ax = ty - JT_RX;
         const Frc<S> U = symCol(Ia, 3 + ax);
@@ -220,6 +238,8 @@ __global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc_in, int64_t
         const S u = st[iv * kPitch + L] - comp(pA, 3 + ax);
         dd[i] = scale(Dinv, U);
         ss[i] = Dinv * u - dot(chat[i], dd[i]);
+        pinAll<S>(dd[i]);
+        pin(ss[i]);
         if constexpr (p > 0) {
           symRank1Sub(Ia, U, Dinv);
           const Mot<S> ci = crossMM(vj[i], axisMot(ax, ldV(iv)));
@@ -234,6 +254,8 @@ __global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc_in, int64_t
             symAdd(Iacc[p], Iup);
             pacc[p] = pacc[p] + pup;
           }
+          pinAll<S>(Iacc[p]);
+          pinAll<S>(pacc[p]);
         }
       }
     }
@@ -255,11 +277,76 @@ __global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc_in, int64_t
       else t = actInv(X, a0);
       const S qdd = ss[i] - dot(t, dd[i]);
       st[iv * kPitch + L] = qdd;
-      if constexpr (TP::T.nchild[i] > 0) ahat[i] = t + axisMot(ax, qdd);
+      if constexpr (TP::T.nchild[i] > 0) {
+        ahat[i] = t + axisMot(ax, qdd);
+        pinAll<S>(ahat[i]);
+      }
     }
   });
   __syncthreads();
   stageOut<NV>(st, gddq, b0, nb);
+}
+
+template<class TP, class S>
+__global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc, int64_t B, CView<S> gq, CView<S> gv,
+                                            CView<S> gt, View<S> gddq) {
+  __shared__ S lds[dynLdsRows<TP>() * kPitch];
+  abaTile<TP, S>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, gt, gddq, lds);
+}
+
+// ---------------------------------------------------------------------------------------
+// Mixed-robot batch: one launch over instances of several models.  Each workgroup takes
+// one entry of a tile table (group, first 64-instance chunk, number of chunks) and runs
+// RNEA then ABA on those chunks with the group's compile-time topology; the topology
+// branch is uniform per workgroup, so waves never diverge.  Tiles are ordered by
+// decreasing cost, so the hardware dispatcher balances the uneven mix.
+// ---------------------------------------------------------------------------------------
+struct MultiGroup {
+  const float* mc;
+  int64_t count;
+  const float* q;
+  const float* v;
+  const float* a;
+  const float* tau;
+  float* tau_out;
+  float* ddq_out;
+  int32_t topo;
+  int32_t pad;
+};
+struct MultiTile {
+  int32_t group, chunk0, nchunks, pad;
+};
+
+template<class TP0, class... TPs> constexpr int maxDynRows() {
+  if constexpr (sizeof...(TPs) == 0) return dynLdsRows<TP0>();
+  else return dynLdsRows<TP0>() > maxDynRows<TPs...>() ? dynLdsRows<TP0>() : maxDynRows<TPs...>();
+}
+template<class F, class TP0, class... TPs> __device__ __forceinline__ void dispatchTopo(int id, F& f) {
+  if (id == 0) f(TP0{});
+  else if constexpr (sizeof...(TPs) > 0) dispatchTopo<F, TPs...>(id - 1, f);
+}
+
+template<class... TPs>
+__global__ __launch_bounds__(64) void k_multi(const MultiGroup* __restrict__ groups,
+                                              const MultiTile* __restrict__ tiles) {
+  __shared__ float lds[maxDynRows<TPs...>() * kPitch];
+  const MultiTile t = tiles[blockIdx.x];
+  const MultiGroup g = groups[t.group];
+  auto run = [&](auto tp) {
+    using TP = decltype(tp);
+    const int64_t n = g.count;
+#pragma unroll 1
+    for (int c = 0; c < t.nchunks; ++c) {
+      const int64_t b0 = int64_t(t.chunk0 + c) * kTile;
+      rneaTile<TP, float>(g.mc, n, b0, CView<float>{g.q, n, 1}, CView<float>{g.v, n, 1}, CView<float>{g.a, n, 1},
+                          View<float>{g.tau_out, n, 1}, lds);
+      __syncthreads();
+      abaTile<TP, float>(g.mc, n, b0, CView<float>{g.q, n, 1}, CView<float>{g.v, n, 1}, CView<float>{g.tau, n, 1},
+                         View<float>{g.ddq_out, n, 1}, lds);
+      __syncthreads();
+    }
+  };
+  dispatchTopo<decltype(run), TPs...>(g.topo, run);
 }
 
 }  // namespace rbd

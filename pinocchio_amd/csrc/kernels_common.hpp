@@ -43,11 +43,33 @@ __device__ __forceinline__ void stepFence() {
 // through it cannot be merged with, or hoisted above, those of earlier steps, so a
 // joint's constants occupy scalar registers only during that joint's step.
 template<class S> __device__ __forceinline__ const S* stepConsts(const S* mc) {
-#ifndef RBD_NO_LAUNDER
+  // the pointer is wave-uniform; make that explicit so it always lives in SGPRs
+  const uint64_t u = reinterpret_cast<uint64_t>(mc);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  mc = reinterpret_cast<const S*>(((uint64_t)hi << 32) | lo);
   asm volatile("" : "+s"(mc));
-#endif
   return mc;
 }
+
+// Pins freshly computed values at this point of the instruction stream: the empty asm
+// consumes and redefines them, so their arithmetic cannot float past later scheduling
+// fences (a fence orders only side-effecting instructions, not pure arithmetic).
+#ifndef RBD_NO_PIN
+template<class S> __device__ __forceinline__ void pin(S& a, S& b) { asm volatile("" : "+v"(a), "+v"(b)); }
+template<class S> __device__ __forceinline__ void pin(S& a) { asm volatile("" : "+v"(a)); }
+// every scalar member of a struct of scalars (V3, Mot, Frc, Sym6, Xform, CInertia, ...)
+template<class S, class T> __device__ __forceinline__ void pinAll(T& o) {
+  static_assert(sizeof(T) % sizeof(S) == 0, "struct of scalars expected");
+  S* f = reinterpret_cast<S*>(&o);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / sizeof(S)); ++k) asm volatile("" : "+v"(f[k]));
+}
+#else
+template<class S> __device__ __forceinline__ void pin(S&, S&) {}
+template<class S> __device__ __forceinline__ void pin(S&) {}
+template<class S, class T> __device__ __forceinline__ void pinAll(T&) {}
+#endif
 
 // Per-lane pointer made opaque at the point of use, so loads through it are neither
 // merged with nor reused from (CSE) earlier steps: values re-read later stay out of the

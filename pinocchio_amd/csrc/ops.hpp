@@ -48,4 +48,14 @@ struct TopoOps {
   int nnzU;  // strictly-upper structural non-zeros of M
 };
 
+struct MultiGroup;
+struct MultiTile;
+// A compiled mix of topologies for the one-launch mixed-robot kernel (k_multi).
+struct MixOps {
+  const char* name;
+  int ntopo;
+  const TopoOps* topo[8];
+  void (*launch)(hipStream_t, unsigned grid, const MultiGroup* groups, const MultiTile* tiles);
+};
+
 }  // namespace rbd

@@ -2,6 +2,8 @@
 // handling and kernel dispatch (include/rbd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <memory>
 #include <type_traits>
@@ -12,13 +14,16 @@
 #include <vector>
 
 #include "../../include/rbd.h"
-#include "kernels_common.hpp"
+#include "k_dyn.hpp"
 #include "ops.hpp"
 
 namespace rbd {
 #define RBD_REGISTER(NAME) const TopoOps* ops_##NAME();
 #include "_gen/registry.inc"
 #undef RBD_REGISTER
+#define RBD_REGISTER_MIX(NAME) const MixOps* mix_##NAME();
+#include "_gen/mixes.inc"
+#undef RBD_REGISTER_MIX
 
 static const TopoOps* const* registry(int* n) {
 #define RBD_REGISTER(NAME) ops_##NAME(),
@@ -102,6 +107,7 @@ struct rbd_ctx {
 
 struct rbd_model {
   rbd_ctx* ctx = nullptr;
+  uint64_t serial = 0;  // unique per created model: mixed-batch plans key on it
   const TopoOps* ops = nullptr;
   int njoints = 0, nq = 0, nv = 0, compact = 0;
   std::vector<int> parents, jtype, iq, iv;
@@ -115,6 +121,11 @@ struct rbd_model {
 namespace {
 
 void setDevice(rbd_ctx* ctx) { HIP_CHECK(hipSetDevice(ctx->device)); }
+
+std::atomic<uint64_t> g_modelSerial{0};
+
+// releases the mixed-batch plans of a context (model == nullptr) or those using a model
+void dropMultiPlans(rbd_ctx* ctx, const rbd_model* model);
 
 // Host flattening of the model constants into per-joint records (topo.hpp MC_* offsets).
 std::vector<double> buildConstants(const rbd_model_desc* d, int nv) {
@@ -227,7 +238,7 @@ int vec3Op(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* x0, int64_t 
            const char* opname) {
   return guarded([&] {
     if (!fn) throw Error(RBD_EUNSUPPORTED, std::string(opname) + ": no kernel for this dtype/topology");
-    RBD_CHECK_ARG(x0 && x1 && x2 && out, "null array argument");
+    RBD_CHECK_ARG(B == 0 || (x0 && x1 && x2 && out), "null array argument");
     const int nv = m->nv;
     auto v0 = argView<S>(r0, B, ld0, flags, "input 0");
     auto v1 = argView<S>(nv, B, ld1, flags, "input 1");
@@ -278,6 +289,7 @@ int rbd_ctx_destroy(rbd_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    dropMultiPlans(ctx, nullptr);
     for (void* p : ctx->stage)
       if (p) (void)hipFree(p);
     if (ctx->work) (void)hipFree(ctx->work);
@@ -303,6 +315,7 @@ int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* d, rbd_model** out) {
     const int nj = d->njoints;
     auto m = std::make_unique<rbd_model>();
     m->ctx = ctx;
+    m->serial = ++g_modelSerial;
     m->njoints = nj;
     m->parents.assign(d->parents, d->parents + nj);
     m->jtype.assign(d->jtype, d->jtype + nj);
@@ -372,6 +385,8 @@ int rbd_model_destroy(rbd_model* m) {
   return guarded([&] {
     if (!m) return;
     hipSetDevice(m->ctx->device);
+    hipStreamSynchronize(m->ctx->stream);  // kernels in flight may still read the constants
+    dropMultiPlans(m->ctx, m);
     if (m->mc32) hipFree(m->mc32);
     if (m->mc64) hipFree(m->mc64);
     delete m;
@@ -410,7 +425,7 @@ int rbd_crba(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void*
              int64_t ldM, uint32_t flags) {
   return guarded([&] {
     checkCommon(ctx, m, B, dtype);
-    RBD_CHECK_ARG(q && M, "null array argument");
+    RBD_CHECK_ARG(B == 0 || (q && M), "null array argument");
     if (!m->compact) throw Error(RBD_EUNSUPPORTED, "crba: the model is not DFS-compact (crba.hxx:573-595)");
     const int nv = m->nv;
     const bool f32 = dtype == RBD_F32;
@@ -439,7 +454,7 @@ int rbd_cholesky_decompose(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtyp
                            double* U, int64_t ldU, double* D, int64_t ldD, uint32_t flags) {
   return guarded([&] {
     checkCommon(ctx, m, B, dtype);
-    RBD_CHECK_ARG(M && U && D, "null array argument");
+    RBD_CHECK_ARG(B == 0 || (M && U && D), "null array argument");
     if (!m->compact) throw Error(RBD_EUNSUPPORTED, "cholesky: the model is not DFS-compact");
     const int nv = m->nv, nnz = (int)m->patRows.size();
     const bool f32 = dtype == RBD_F32;
@@ -467,7 +482,7 @@ int rbd_cholesky_solve(rbd_ctx* ctx, const rbd_model* m, int64_t B, const double
                        int64_t ldD, int dtype, void* x, int64_t ldx, uint32_t flags) {
   return guarded([&] {
     checkCommon(ctx, m, B, dtype);
-    RBD_CHECK_ARG(U && D && x, "null array argument");
+    RBD_CHECK_ARG(B == 0 || (U && D && x), "null array argument");
     if (!m->compact) throw Error(RBD_EUNSUPPORTED, "cholesky: the model is not DFS-compact");
     const int nv = m->nv, nnz = (int)m->patRows.size();
     const bool f32 = dtype == RBD_F32;
@@ -505,7 +520,7 @@ int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
                          int64_t ldout, uint32_t flags) {
   return guarded([&] {
     checkCommon(ctx, m, B, dtype);
-    RBD_CHECK_ARG(q && v && a, "null input argument");
+    RBD_CHECK_ARG(B == 0 || (q && v && a), "null input argument");
     if (dtype != RBD_F32) throw Error(RBD_EUNSUPPORTED, "rnea_derivatives: FP32 only");
     checkDerivativeModel(m);
     const int nv = m->nv;
@@ -535,7 +550,7 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, 
                         void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags) {
   return guarded([&] {
     checkCommon(ctx, m, B, dtype);
-    RBD_CHECK_ARG(q && v && tau && ddq_dq && ddq_dv && ddq_dtau, "null array argument");
+    RBD_CHECK_ARG(B == 0 || (q && v && tau && ddq_dq && ddq_dv && ddq_dtau), "null array argument");
     if (dtype != RBD_F32) throw Error(RBD_EUNSUPPORTED, "aba_derivatives: FP32 only");
     checkDerivativeModel(m);
     const int nv = m->nv, nnz = m->ops->nnzU;
@@ -581,14 +596,311 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, 
   });
 }
 
-int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t nmodels, const rbd_model* const* models, int64_t ninstances,
-                       const int32_t* model_id, const int64_t* offset_q, const int64_t* offset_v, const float* q,
-                       const float* v, const float* a, const float* tau_in, float* tau_out, float* ddq_out,
-                       uint32_t flags) {
-  (void)nmodels; (void)models; (void)ninstances; (void)model_id; (void)offset_q; (void)offset_v;
-  (void)q; (void)v; (void)a; (void)tau_in; (void)tau_out; (void)ddq_out; (void)flags;
-  g_err = "rbd_multi_rnea_aba: not available in this build";
-  return ctx ? RBD_EUNSUPPORTED : RBD_EINVAL;
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Mixed-robot batches: plan (tile table + strategy), autotuner, graph replay
+// ---------------------------------------------------------------------------------------
+namespace {
+
+struct MultiPlan {
+  std::vector<unsigned char> key;  // the caller's group array byte for byte, then model serials
+  std::vector<const rbd_model*> models;
+  const MixOps* mix = nullptr;     // null: per-model launches only
+  bool fused = true;
+  std::vector<int> width;          // chunks of 64 instances per workgroup, per group
+  MultiGroup* dGroups = nullptr;
+  MultiTile* dTiles = nullptr;
+  int ntiles = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string report;
+};
+
+void releasePlan(MultiPlan& p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  if (p.dGroups) (void)hipFree(p.dGroups);
+  if (p.dTiles) (void)hipFree(p.dTiles);
+  p.exec = nullptr;
+  p.graph = nullptr;
+  p.dGroups = nullptr;
+  p.dTiles = nullptr;
+}
+
+struct MultiCache {
+  std::vector<std::unique_ptr<MultiPlan>> plans;
+  std::string lastReport;
+  ~MultiCache() {
+    for (auto& p : plans) releasePlan(*p);
+  }
+};
+
+std::mutex g_multiMutex;
+std::vector<std::pair<rbd_ctx*, std::unique_ptr<MultiCache>>> g_multi;
+
+MultiCache& cacheFor(rbd_ctx* ctx) {
+  for (auto& e : g_multi)
+    if (e.first == ctx) return *e.second;
+  g_multi.emplace_back(ctx, std::make_unique<MultiCache>());
+  return *g_multi.back().second;
+}
+
+void dropMultiPlans(rbd_ctx* ctx, const rbd_model* model) {
+  std::lock_guard<std::mutex> lock(g_multiMutex);
+  for (size_t e = 0; e < g_multi.size(); ++e) {
+    if (g_multi[e].first != ctx) continue;
+    if (!model) {
+      g_multi.erase(g_multi.begin() + e);
+      return;
+    }
+    auto& plans = g_multi[e].second->plans;
+    for (size_t k = 0; k < plans.size();) {
+      const auto& ms = plans[k]->models;
+      if (std::find(ms.begin(), ms.end(), model) != ms.end()) {
+        releasePlan(*plans[k]);
+        plans.erase(plans.begin() + k);
+      } else {
+        ++k;
+      }
+    }
+    return;
+  }
+}
+
+const MixOps* findMix(const std::vector<const TopoOps*>& need) {
+#define RBD_REGISTER_MIX(NAME) rbd::mix_##NAME(),
+  static const MixOps* const mixes[] = {
+#include "_gen/mixes.inc"
+  };
+#undef RBD_REGISTER_MIX
+  for (const MixOps* mx : mixes) {
+    bool all = true;
+    for (const TopoOps* o : need) {
+      bool found = false;
+      for (int k = 0; k < mx->ntopo; ++k) found = found || mx->topo[k] == o;
+      all = all && found;
+    }
+    if (all) return mx;
+  }
+  return nullptr;
+}
+
+// per-instance cost proxy of RNEA + ABA: linear in the number of joints
+double costOf(const rbd_model* m) { return (double)m->njoints; }
+
+void buildTiles(MultiPlan& p, const rbd_batch_group* g, int ng, std::vector<MultiTile>& tiles) {
+  struct T {
+    double cost;
+    MultiTile t;
+  };
+  std::vector<T> all;
+  for (int k = 0; k < ng; ++k) {
+    const int64_t chunks = (g[k].count + kTile - 1) / kTile;
+    const int w = std::max(1, p.width[k]);
+    for (int64_t c = 0; c < chunks; c += w) {
+      const int n = (int)std::min<int64_t>(w, chunks - c);
+      all.push_back({costOf(g[k].model) * n, MultiTile{k, (int32_t)c, n, 0}});
+    }
+  }
+  std::stable_sort(all.begin(), all.end(), [](const T& a, const T& b) { return a.cost > b.cost; });
+  tiles.clear();
+  for (auto& e : all) tiles.push_back(e.t);
+}
+
+void enqueue(rbd_ctx* ctx, MultiPlan& p, const rbd_batch_group* g, int ng) {
+  if (p.fused) {
+    if (p.ntiles > 0) p.mix->launch(ctx->stream, (unsigned)p.ntiles, p.dGroups, p.dTiles);
+    return;
+  }
+  for (int k = 0; k < ng; ++k) {
+    const rbd_model* m = g[k].model;
+    const int64_t n = g[k].count;
+    if (n == 0) continue;
+    m->ops->rnea_f32(ctx->stream, m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
+                     CView<float>{g[k].a, n, 1}, View<float>{g[k].tau_out, n, 1});
+    m->ops->aba_f32(ctx->stream, m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
+                    CView<float>{g[k].tau, n, 1}, View<float>{g[k].ddq_out, n, 1});
+  }
+}
+
+void uploadTiles(MultiPlan& p, const rbd_batch_group* g, int ng) {
+  std::vector<MultiTile> tiles;
+  buildTiles(p, g, ng, tiles);
+  if (p.dTiles) HIP_CHECK(hipFree(p.dTiles));
+  p.dTiles = nullptr;
+  p.ntiles = (int)tiles.size();
+  if (!tiles.empty()) {
+    HIP_CHECK(hipMalloc(&p.dTiles, tiles.size() * sizeof(MultiTile)));
+    HIP_CHECK(hipMemcpy(p.dTiles, tiles.data(), tiles.size() * sizeof(MultiTile), hipMemcpyHostToDevice));
+  }
+}
+
+float timeMs(rbd_ctx* ctx, MultiPlan& p, const rbd_batch_group* g, int ng, int reps) {
+  hipEvent_t a, b;
+  HIP_CHECK(hipEventCreate(&a));
+  HIP_CHECK(hipEventCreate(&b));
+  enqueue(ctx, p, g, ng);  // warm
+  float best = 1e30f;
+  for (int r = 0; r < reps; ++r) {
+    HIP_CHECK(hipEventRecord(a, ctx->stream));
+    enqueue(ctx, p, g, ng);
+    HIP_CHECK(hipEventRecord(b, ctx->stream));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return best;
+}
+
+// Coordinate search over per-group tile widths {1, 2, 4, 8} for the fused launch (most
+// expensive group first), then the per-model launch variant; the fastest wins.
+void autotune(rbd_ctx* ctx, MultiPlan& p, const rbd_batch_group* g, int ng, bool tune) {
+  std::string rep = "{\"candidates\": [";
+  bool first = true;
+  auto note = [&](const std::string& desc, float ms) {
+    rep += std::string(first ? "" : ", ") + "{\"variant\": \"" + desc + "\", \"ms\": " + std::to_string(ms) + "}";
+    first = false;
+  };
+  auto widthStr = [&](const std::vector<int>& w) {
+    std::string s;
+    for (int k = 0; k < ng; ++k) s += std::string(k ? "," : "") + g[k].model->ops->name + "=" + std::to_string(w[k]);
+    return s;
+  };
+  p.width.assign(ng, 1);
+  float best = 1e30f;
+  bool bestFused = p.mix != nullptr;
+  std::vector<int> bestW = p.width;
+  if (p.mix) {
+    p.fused = true;
+    uploadTiles(p, g, ng);
+    if (tune) {
+      best = timeMs(ctx, p, g, ng, 3);
+      note("fused " + widthStr(p.width), best);
+      std::vector<int> order(ng);
+      for (int k = 0; k < ng; ++k) order[k] = k;
+      std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return costOf(g[x].model) * g[x].count > costOf(g[y].model) * g[y].count;
+      });
+      for (int k : order) {
+        for (int w : {2, 4, 8}) {
+          std::vector<int> trial = bestW;
+          trial[k] = w;
+          p.width = trial;
+          uploadTiles(p, g, ng);
+          const float ms = timeMs(ctx, p, g, ng, 3);
+          note("fused " + widthStr(trial), ms);
+          if (ms < best) {
+            best = ms;
+            bestW = trial;
+          }
+        }
+      }
+    }
+  }
+  if (tune || !p.mix) {
+    p.fused = false;
+    const float ms = tune ? timeMs(ctx, p, g, ng, 3) : 0.f;
+    if (tune) note("per-model launches", ms);
+    if (!p.mix || ms < best) {
+      best = ms;
+      bestFused = false;
+    }
+  }
+  p.fused = bestFused;
+  p.width = bestW;
+  if (p.fused) uploadTiles(p, g, ng);
+  rep += "], \"chosen\": \"" + std::string(p.fused ? "fused " + widthStr(p.width) : "per-model launches") +
+         "\", \"mix\": \"" + (p.mix ? p.mix->name : "none") + "\", \"tiles\": " + std::to_string(p.ntiles) + "}";
+  p.report = rep;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t ngroups, const rbd_batch_group* groups, uint32_t flags) {
+  return guarded([&] {
+    RBD_CHECK_ARG(ctx != nullptr, "null context");
+    RBD_CHECK_ARG(ngroups >= 1 && ngroups <= 8 && groups, "between 1 and 8 groups are supported");
+    std::vector<const TopoOps*> need;
+    for (int k = 0; k < ngroups; ++k) {
+      const rbd_batch_group& g = groups[k];
+      RBD_CHECK_ARG(g.model && g.model->ctx == ctx, "group model missing or from another context");
+      RBD_CHECK_ARG(g.count >= 0, "negative group size");
+      if (g.count > 0)
+        RBD_CHECK_ARG(g.q && g.v && g.a && g.tau && g.tau_out && g.ddq_out, "null group array");
+      if (!g.model->ops->rnea_f32 || !g.model->ops->aba_f32)
+        throw Error(RBD_EUNSUPPORTED, "no RNEA/ABA kernels for a group's topology");
+      // the autotuner runs the batch several times: outputs must not overwrite inputs
+      for (const void* out : {(const void*)g.tau_out, (const void*)g.ddq_out})
+        for (const void* in : {(const void*)g.q, (const void*)g.v, (const void*)g.a, (const void*)g.tau})
+          RBD_CHECK_ARG(g.count == 0 || out != in, "group outputs must not alias its inputs");
+      need.push_back(g.model->ops);
+    }
+    setDevice(ctx);
+    std::lock_guard<std::mutex> lock(g_multiMutex);
+    MultiCache& cache = cacheFor(ctx);
+    const unsigned char* kb = reinterpret_cast<const unsigned char*>(groups);
+    std::vector<unsigned char> key(kb, kb + sizeof(rbd_batch_group) * ngroups);
+    for (int k = 0; k < ngroups; ++k) {
+      const unsigned char* sb = reinterpret_cast<const unsigned char*>(&groups[k].model->serial);
+      key.insert(key.end(), sb, sb + sizeof(uint64_t));
+    }
+    MultiPlan* plan = nullptr;
+    for (auto& p : cache.plans)
+      if (p->key == key) plan = p.get();
+    if (!plan) {
+      if (cache.plans.size() >= 8) {
+        releasePlan(*cache.plans.front());
+        cache.plans.erase(cache.plans.begin());
+      }
+      auto p = std::make_unique<MultiPlan>();
+      p->key = key;
+      for (int k = 0; k < ngroups; ++k) p->models.push_back(groups[k].model);
+      p->mix = findMix(need);
+      if (p->mix) {
+        std::vector<MultiGroup> hg(ngroups);
+        for (int k = 0; k < ngroups; ++k) {
+          const rbd_batch_group& g = groups[k];
+          int topo = 0;
+          for (int t = 0; t < p->mix->ntopo; ++t)
+            if (p->mix->topo[t] == g.model->ops) topo = t;
+          hg[k] = MultiGroup{g.model->mc32, g.count, g.q, g.v, g.a, g.tau, g.tau_out, g.ddq_out, topo, 0};
+        }
+        HIP_CHECK(hipMalloc(&p->dGroups, sizeof(MultiGroup) * ngroups));
+        HIP_CHECK(hipMemcpy(p->dGroups, hg.data(), sizeof(MultiGroup) * ngroups, hipMemcpyHostToDevice));
+      }
+      autotune(ctx, *p, groups, ngroups, !(flags & RBD_NO_AUTOTUNE));
+      if (!(flags & RBD_NO_GRAPH)) {
+        HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue(ctx, *p, groups, ngroups);
+        HIP_CHECK(hipStreamEndCapture(ctx->stream, &p->graph));
+        HIP_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+      }
+      cache.plans.push_back(std::move(p));
+      plan = cache.plans.back().get();
+    }
+    cache.lastReport = plan->report;
+    if (plan->exec && !(flags & RBD_NO_GRAPH)) HIP_CHECK(hipGraphLaunch(plan->exec, ctx->stream));
+    else enqueue(ctx, *plan, groups, ngroups);
+    HIP_CHECK(hipGetLastError());
+    if (!(flags & RBD_ASYNC)) HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int rbd_multi_tuning(rbd_ctx* ctx, char* buf, int32_t len) {
+  return guarded([&] {
+    RBD_CHECK_ARG(ctx && buf && len > 0, "bad argument");
+    std::lock_guard<std::mutex> lock(g_multiMutex);
+    const std::string& s = cacheFor(ctx).lastReport;
+    const size_t n = std::min<size_t>(s.size(), (size_t)len - 1);
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  });
 }
 
 int rbd_model_cholesky_pattern(const rbd_model* m, int32_t* nnz, int32_t* rows, int32_t* cols) {
