@@ -203,8 +203,18 @@ def main():
     else:
         achieved = w["flop"] * B / (k_ms[dom] * 1e-3) / 1e12
         unit = "TFLOP/s"
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
+    # summary (FETCH_SIZE and WRITE_SIZE, each scaled by a 1 GiB device copy)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "step_humanoid36_b65536_traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        t_ = tj["bytes_per_launch"].get("k_" + dom)
+        if t_ and tj["batch"] == B:
+            traffic = t_["read"] + t_["write"]
     roofline = {"bound": w["bound"], "achieved": achieved, "peak": w["peak"], "unit": unit,
-                "frac": achieved / w["peak"], "traffic": None, "kernel": dom,
+                "frac": achieved / w["peak"], "traffic": traffic, "kernel": dom,
+                "algorithmic_bytes_per_launch": w["bytes"] * B,
                 "per_kernel_ms": k_ms,
                 "per_kernel": {k: {"ms": k_ms[k], "GB_s": work[k]["bytes"] * B / (k_ms[k] * 1e-3) / 1e9,
                                    "TFLOP_s": work[k]["flop"] * B / (k_ms[k] * 1e-3) / 1e12,
