@@ -16,6 +16,8 @@
  *   rbd_rnea_derivatives  computeRNEADerivatives    algorithm/rnea-derivatives.hxx:472-541, 756-766
  *   rbd_aba_derivatives   computeABADerivatives     algorithm/aba-derivatives.hxx:380-453, 900-923
  *   rbd_multi_rnea_aba    (no counterpart: one launch over instances of several models)
+ *   rbd_mlp_*             (no counterpart: learned forward-dynamics surrogate, checked by
+ *                          the RNEA residual with exact ABA as fallback)
  *
  * Conventions (identical to the reference): 6-vectors are [linear; angular]; free-flyer
  * configuration is [x, y, z, qx, qy, qz, qw]; joint 0 is the universe; parents[i] < i.
@@ -166,6 +168,61 @@ typedef struct rbd_batch_group {
 int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t ngroups, const rbd_batch_group* groups, uint32_t flags);
 /* The autotuner's last decision and timings as a JSON string (NUL-terminated, truncated to len). */
 int rbd_multi_tuning(rbd_ctx* ctx, char* buf, int32_t len);
+
+/* Learned forward-dynamics surrogate (no reference counterpart).  A multilayer perceptron
+ * in -> H -> ... -> H -> nv with SiLU activations, BF16 weights and activations, FP32
+ * accumulation (matrix cores), FP32 master weights trained with Adam on an MSE loss in
+ * normalised units.  Inputs per configuration: sin and cos of every revolute coordinate,
+ * the free-flyer quaternion and its rotation matrix (if the root is a free flyer), v and
+ * tau -- 73 features for the 18-DoF quadruped -- each normalised as (x - mean) / std.
+ * All batched arrays are FP32 device arrays in the batch-fastest layout (element (k, b) at
+ * p[k * ld + b], ld >= batch).  Parameters are exchanged in the packed order: per layer,
+ * W (out x in, row major) then b (out). */
+typedef struct rbd_mlp rbd_mlp;
+int rbd_mlp_create(rbd_ctx* ctx, const rbd_model* model, int32_t hidden, int32_t nhidden, uint64_t seed,
+                   rbd_mlp** out);
+int rbd_mlp_destroy(rbd_mlp* mlp);
+int rbd_mlp_dims(const rbd_mlp* mlp, int32_t* nin, int32_t* nout, int32_t* hidden, int32_t* nhidden,
+                 int64_t* nparams);
+/* host arrays of nin (x) and nout (y) entries; scales must be positive */
+int rbd_mlp_set_normalization(rbd_mlp* mlp, const float* xmean, const float* xstd, const float* ymean,
+                              const float* ystd);
+/* host array of nparams floats (packed order); set also resets the optimiser state */
+int rbd_mlp_get_params(rbd_mlp* mlp, float* params, int64_t nparams);
+int rbd_mlp_set_params(rbd_mlp* mlp, const float* params, int64_t nparams);
+/* raw (unnormalised) features, device array [batch][nin] (row per configuration) */
+int rbd_mlp_features(rbd_ctx* ctx, rbd_mlp* mlp, int64_t batch, const float* q, int64_t ldq, const float* v,
+                     int64_t ldv, const float* tau, int64_t ldtau, float* features, uint32_t flags);
+/* ddq = surrogate(q, v, tau), no check */
+int rbd_mlp_predict(rbd_ctx* ctx, rbd_mlp* mlp, int64_t batch, const float* q, int64_t ldq, const float* v,
+                    int64_t ldv, const float* tau, int64_t ldtau, float* ddq, int64_t ldddq, uint32_t flags);
+/* one Adam step (lr; beta 0.9, 0.999; eps 1e-8) on the minibatch with targets ddq; the
+ * minibatch MSE (normalised units) is added to *loss (a device float) when loss != NULL */
+int rbd_mlp_train_step(rbd_ctx* ctx, rbd_mlp* mlp, int64_t batch, const float* q, int64_t ldq, const float* v,
+                       int64_t ldv, const float* tau, int64_t ldtau, const float* ddq, int64_t ldddq, float lr,
+                       float* loss, uint32_t flags);
+/* loss (host float) and its gradient w.r.t. every parameter (host array, packed order),
+ * no update */
+int rbd_mlp_gradients(rbd_ctx* ctx, rbd_mlp* mlp, int64_t batch, const float* q, int64_t ldq, const float* v,
+                      int64_t ldv, const float* tau, int64_t ldtau, const float* ddq, int64_t ldddq, float* grad,
+                      int64_t nparams, float* loss);
+/* Forward dynamics through the surrogate, checked per configuration by the RNEA residual
+ * ||rnea(q, v, ddq) - tau||_2 <= tol ||tau||_2; configurations above it are recomputed with
+ * ABA.  *nfallback (host) receives their count; accepted (device, batch bytes: 1 = surrogate
+ * kept) and residual (device, batch floats: the surrogate's relative residual) may be
+ * NULL.  Synchronous. */
+int rbd_mlp_forward_dynamics(rbd_ctx* ctx, rbd_mlp* mlp, int64_t batch, const float* q, int64_t ldq, const float* v,
+                             int64_t ldv, const float* tau, int64_t ldtau, float tol, float* ddq, int64_t ldddq,
+                             int64_t* nfallback, unsigned char* accepted, float* residual);
+
+/* Seeded FP32 batch inputs generated on the device (batch-fastest layout), draw for draw
+ * the host sampler of the package (pinocchio_amd/models.py sample_inputs): per joint in
+ * order, free-flyer translation U(-base_range, base_range)^3 and a uniform unit quaternion
+ * [x, y, z, w], revolute angles U(-pi, pi); then v ~ N(0, 1), a ~ N(0, 1), tau ~
+ * N(0, tau_sigma^2).  a may be NULL. */
+int rbd_sample_inputs(rbd_ctx* ctx, const rbd_model* model, int64_t batch, uint64_t seed, double tau_sigma,
+                      double base_range, float* q, int64_t ldq, float* v, int64_t ldv, float* a, int64_t lda,
+                      float* tau, int64_t ldtau, uint32_t flags);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,337 @@
+// k_mlp.hpp -- kernels of the forward-dynamics surrogate: a multilayer perceptron
+// (features -> H -> H -> H -> nv, SiLU) trained and evaluated with BF16 matrix-core
+// products and FP32 accumulation.
+//
+// Activations are sample-major (row b holds one configuration's vector, stride ld);
+// weights are [out, in] row-major like a linear layer (y = W x + b).  Every product is
+// C(m, n) = sum_k A(m, k) B(n, k) over LDS tiles fed to v_mfma_f32_16x16x32_bf16; the
+// operands may be k-contiguous (forward: activations x weights) or m/n-contiguous
+// (backward: weight gradients reduce over the batch, input gradients over the outputs),
+// in which case the tile is transposed on its way into LDS.  Epilogues fuse the bias,
+// the activation, its derivative, and split-K partial sums.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rbd {
+namespace mlp {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBK = 32;       // k extent of one LDS stage (= one MFMA's k)
+constexpr int kLdk = kBK + 8; // LDS row pitch in bf16 (80 B: rows stay 16-B aligned)
+constexpr int kGemmThreads = 256;
+
+enum Epi : int {
+  EPI_SILU = 0,       // Cb = bf16(silu(acc + bias)), optionally Zout = acc + bias (fp32)
+  EPI_LINEAR = 1,     // Cf = acc + bias (fp32)
+  EPI_SILU_GRAD = 2,  // Cb = bf16(acc * silu'(Z))
+  EPI_PARTIAL = 3     // Cf[z] = acc (fp32 split-K partial, z = blockIdx.z)
+};
+
+struct GemmArgs {
+  const bf16* A;
+  int64_t lda;  // A(m, k) = AK ? A[m * lda + k] : A[k * lda + m]
+  const bf16* B;
+  int64_t ldb;  // B(n, k) = BK ? B[n * ldb + k] : B[k * ldb + n]
+  int M, N, K;  // multiples of the tile extents and of kBK
+  int kChunk;   // k range of one blockIdx.z (split-K); multiple of kBK
+  const float* bias;
+  const float* Z;  // EPI_SILU_GRAD: pre-activation, [M, ldz]
+  float* Zout;     // EPI_SILU: pre-activation out (may be null)
+  int64_t ldz;
+  bf16* Cb;
+  float* Cf;
+  int64_t ldc;
+  int64_t zstride;  // EPI_PARTIAL: distance between the partials of consecutive splits
+};
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// Stage ROWS x kBK operand values (row r, k) into s[r * kLdk + k].
+template<int ROWS, bool KCONTIG>
+__device__ __forceinline__ void loadTile(bf16* __restrict__ s, const bf16* __restrict__ g, int64_t ld, int r0, int k0,
+                                         int tid) {
+  if constexpr (KCONTIG) {
+    // 16-byte chunks along k: 4 per row
+    for (int c = tid; c < ROWS * (kBK / 8); c += kGemmThreads) {
+      const int r = c / (kBK / 8), kc = (c % (kBK / 8)) * 8;
+      *reinterpret_cast<uint4*>(s + r * kLdk + kc) =
+          *reinterpret_cast<const uint4*>(g + (int64_t)(r0 + r) * ld + k0 + kc);
+    }
+  } else {
+    // 16-byte chunks along r, transposed into LDS
+    for (int c = tid; c < kBK * (ROWS / 8); c += kGemmThreads) {
+      const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
+      const uint4 w = *reinterpret_cast<const uint4*>(g + (int64_t)(k0 + k) * ld + r0 + rc);
+      const bf16* e = reinterpret_cast<const bf16*>(&w);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[(rc + j) * kLdk + k] = e[j];
+    }
+  }
+}
+
+// C tile of BM x BN per workgroup; four waves in a 2 x 2 grid, each wave BM/2 x BN/2 as
+// (BM/32) x (BN/32) MFMA blocks of 16 x 16.
+template<int BM, int BN, bool AK, bool BK, int EPI>
+__global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  static_assert(TM >= 1 && TN >= 1, "tile too small");
+  __shared__ __attribute__((aligned(16))) bf16 sA[BM * kLdk];
+  __shared__ __attribute__((aligned(16))) bf16 sB[BN * kLdk];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kBeg = blockIdx.z * g.kChunk;
+  const int kEnd = min(g.K, kBeg + g.kChunk);
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  for (int k0 = kBeg; k0 < kEnd; k0 += kBK) {
+    loadTile<BM, AK>(sA, g.A, g.lda, m0, k0, tid);
+    loadTile<BN, BK>(sB, g.B, g.ldb, n0, k0, tid);
+    __syncthreads();
+    bf16x8 af[TM], bfr[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm * WM + i * 16 + lr) * kLdk + lk);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) bfr[j] = *reinterpret_cast<const bf16x8*>(sB + (wn * WN + j * 16 + lr) * kLdk + lk);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+
+  // lane holds C(row = 4 * (lane / 16) + e, col = lane % 16) of each 16 x 16 block
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = n0 + wn * WN + j * 16 + lr;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = m0 + wm * WM + i * 16 + (lane >> 4) * 4 + e;
+        const float a = acc[i][j][e];
+        if constexpr (EPI == EPI_SILU) {
+          const float z = a + g.bias[col];
+          if (g.Zout) g.Zout[(int64_t)row * g.ldz + col] = z;
+          g.Cb[(int64_t)row * g.ldc + col] = (bf16)(z * sigmoidf(z));
+        } else if constexpr (EPI == EPI_LINEAR) {
+          g.Cf[(int64_t)row * g.ldc + col] = a + g.bias[col];
+        } else if constexpr (EPI == EPI_SILU_GRAD) {
+          const float z = g.Z[(int64_t)row * g.ldz + col];
+          const float s = sigmoidf(z);
+          g.Cb[(int64_t)row * g.ldc + col] = (bf16)(a * s * (1.0f + z * (1.0f - s)));
+        } else {
+          g.Cf[(int64_t)blockIdx.z * g.zstride + (int64_t)row * g.ldc + col] = a;
+        }
+      }
+    }
+  }
+}
+
+// ---- features ------------------------------------------------------------------------
+// Input vector of one configuration (feature order):
+//   sin q_r, cos q_r for every revolute joint r       2 * nrev
+//   free-flyer quaternion (qx, qy, qz, qw)            4   (if the root is a free flyer)
+//   its rotation matrix, row major                    9   (if the root is a free flyer)
+//   v, tau                                            2 * nv
+// normalised as (x - mean) / std and zero-padded to the product's k extent.
+constexpr int kMaxFeat = 256;
+struct FeatMap {
+  int nq, nv, nrev, hasFF, nfeat, ldx;  // ldx: padded row length
+  int revq[96];                          // q index of each revolute joint
+};
+
+__device__ __forceinline__ void quatRot9(float x, float y, float z, float w, float* r) {
+  // unit-quaternion rotation matrix (Eigen's Quaternion::toRotationMatrix formula)
+  const float tx = 2 * x, ty = 2 * y, tz = 2 * z;
+  const float twx = tx * w, twy = ty * w, twz = tz * w;
+  const float txx = tx * x, txy = ty * x, txz = tz * x;
+  const float tyy = ty * y, tyz = tz * y, tzz = tz * z;
+  r[0] = 1 - (tyy + tzz); r[1] = txy - twz;       r[2] = txz + twy;
+  r[3] = txy + twz;       r[4] = 1 - (txx + tzz); r[5] = tyz - twx;
+  r[6] = txz - twy;       r[7] = tyz + twx;       r[8] = 1 - (txx + tyy);
+}
+
+// raw (unnormalised) feature f of configuration b; q, v, tau batch-fastest (k * ld + b)
+__device__ __forceinline__ float featureAt(const FeatMap& fm, int f, const float* q, const float* v, const float* tau,
+                                           int64_t ldq, int64_t ldv, int64_t ldt, int64_t b) {
+  if (f < fm.nrev) return sinf(q[fm.revq[f] * ldq + b]);
+  f -= fm.nrev;
+  if (f < fm.nrev) return cosf(q[fm.revq[f] * ldq + b]);
+  f -= fm.nrev;
+  if (fm.hasFF) {
+    if (f < 4) return q[(3 + f) * ldq + b];
+    f -= 4;
+    if (f < 9) {
+      float r[9];
+      quatRot9(q[3 * ldq + b], q[4 * ldq + b], q[5 * ldq + b], q[6 * ldq + b], r);
+      return r[f];
+    }
+    f -= 9;
+  }
+  if (f < fm.nv) return v[f * ldv + b];
+  f -= fm.nv;
+  return tau[f * ldt + b];
+}
+
+// X[b, f] = bf16((x_f - mean_f) * invstd_f), zero for f >= nfeat (one thread per entry)
+__global__ void k_features(FeatMap fm, int64_t B, const float* q, int64_t ldq, const float* v, int64_t ldv,
+                           const float* tau, int64_t ldt, const float* mean, const float* invstd, bf16* X,
+                           float* Xraw) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = e / fm.ldx;
+  const int f = (int)(e - b * fm.ldx);
+  if (b >= B) return;
+  float x = 0.f;
+  if (f < fm.nfeat) {
+    const float r = featureAt(fm, f, q, v, tau, ldq, ldv, ldt, b);
+    if (Xraw) Xraw[b * fm.nfeat + f] = r;
+    x = (r - mean[f]) * invstd[f];
+  }
+  X[b * fm.ldx + f] = (bf16)x;
+}
+
+// zero rows [B, Bpad) of a bf16 matrix (padding configurations of the last tile)
+__global__ void k_zero_rows(bf16* X, int64_t ld, int64_t B, int64_t Bpad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (Bpad - B) * ld) X[B * ld + e] = (bf16)0.f;
+}
+
+// normalised target Y[b, n] = (ddq[n, b] - mean_n) * invstd_n  (fp32, zero padding)
+__global__ void k_targets(int nv, int ldy, int64_t B, int64_t Bpad, const float* ddq, int64_t ldd, const float* mean,
+                          const float* invstd, float* Y) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = e / ldy;
+  const int n = (int)(e - b * ldy);
+  if (b >= Bpad) return;
+  Y[e] = (n < nv && b < B) ? (ddq[n * ldd + b] - mean[n]) * invstd[n] : 0.f;
+}
+
+// dOut = 2 (out - Y) / (B nv) over real rows and outputs (bf16), loss partial sums
+__global__ void k_loss_grad(int nv, int ldy, int64_t B, int64_t Bpad, const float* out, const float* Y, bf16* dOut,
+                            float* lossAcc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = e / ldy;
+  const int n = (int)(e - b * ldy);
+  float d = 0.f;
+  if (b < Bpad) {
+    if (n < nv && b < B) d = out[e] - Y[e];
+    dOut[e] = (bf16)(d * (2.0f / ((float)B * (float)nv)));
+  }
+  // wave reduction of d^2, one atomic per wave (monitoring only)
+  float s = d * d;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0 && lossAcc) atomicAdd(lossAcc, s / ((float)B * (float)nv));
+}
+
+// column sums of a bf16 matrix over row splits: P[s * pstride + n] = sum_{b in split s} D[b, n]
+__global__ void k_colsum(const bf16* D, int64_t ld, int N, int64_t rowsPerSplit, float* P, int64_t pstride) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int64_t r0 = (int64_t)blockIdx.y * rowsPerSplit;
+  float s = 0.f;
+  for (int64_t r = r0; r < r0 + rowsPerSplit; ++r) s += (float)D[r * ld + n];
+  P[(int64_t)blockIdx.y * pstride + n] = s;
+}
+
+// Adam on fp32 master parameters with gradient = sum of split partials; refreshes the
+// bf16 copy used by the products.
+__global__ void k_adam(int64_t n, float* P, float* m1, float* m2, const float* G, int splits, int64_t gstride, float lr,
+                       float b1, float b2, float eps, float c1, float c2, bf16* Pb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float g = 0.f;
+  for (int s = 0; s < splits; ++s) g += G[s * gstride + i];
+  const float a = b1 * m1[i] + (1.f - b1) * g;
+  const float c = b2 * m2[i] + (1.f - b2) * g * g;
+  m1[i] = a;
+  m2[i] = c;
+  const float p = P[i] - lr * (a * c1) / (sqrtf(c * c2) + eps);
+  P[i] = p;
+  if (Pb) Pb[i] = (bf16)p;
+}
+
+// gradient = sum of split partials (exported for checks)
+__global__ void k_sum_partials(int64_t n, const float* G, int splits, int64_t gstride, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float g = 0.f;
+  for (int s = 0; s < splits; ++s) g += G[s * gstride + i];
+  out[i] = g;
+}
+
+// fp32 -> bf16 copy
+__global__ void k_to_bf16(int64_t n, const float* x, bf16* y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (bf16)x[i];
+}
+
+// ddq[n, b] = out[b, n] * std_n + mean_n  (batch-fastest output)
+__global__ void k_outputs(int nv, int ldo, int64_t B, const float* out, const float* mean, const float* std_,
+                          float* ddq, int64_t ldd) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = e / B, b = e - n * B;
+  if (n >= nv) return;
+  ddq[n * ldd + b] = out[b * ldo + n] * std_[n] + mean[n];
+}
+
+// Residual gate: ||tau_chk - tau|| > tol ||tau|| (2-norms) sends configuration b to the
+// exact path: its index and inputs are appended to the compact lists.
+__global__ void k_gate(int nq, int nv, int64_t B, const float* tauChk, int64_t ldc, const float* tau, int64_t ldt,
+                       const float* q, int64_t ldq, const float* v, int64_t ldv, float tol, unsigned int* count,
+                       int64_t* idx, float* cq, float* cv, float* ct, int64_t ldcmp, unsigned char* accepted,
+                       float* relOut) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float num = 0.f, den = 0.f;
+  for (int k = 0; k < nv; ++k) {
+    const float t = tau[k * ldt + b];
+    const float d = tauChk[k * ldc + b] - t;
+    num += d * d;
+    den += t * t;
+  }
+  const float rel = sqrtf(num) / fmaxf(sqrtf(den), 1e-30f);
+  if (relOut) relOut[b] = rel;
+  const bool ok = rel <= tol;
+  if (accepted) accepted[b] = ok ? 1 : 0;
+  if (!ok) {
+    const unsigned int s = atomicAdd(count, 1u);
+    idx[s] = b;
+    for (int k = 0; k < nq; ++k) cq[k * ldcmp + s] = q[k * ldq + b];
+    for (int k = 0; k < nv; ++k) {
+      cv[k * ldcmp + s] = v[k * ldv + b];
+      ct[k * ldcmp + s] = tau[k * ldt + b];
+    }
+  }
+}
+
+__global__ void k_scatter(int nv, int64_t n, const int64_t* idx, const float* src, int64_t lds, float* dst,
+                          int64_t ldd) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = e / n, j = e - k * n;
+  if (k >= nv) return;
+  dst[k * ldd + idx[j]] = src[k * lds + j];
+}
+
+// fill kernel (used instead of a memset so that call sequences stay capturable)
+__global__ void k_fill_u32(unsigned int* p, int64_t n, unsigned int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace mlp
+}  // namespace rbd

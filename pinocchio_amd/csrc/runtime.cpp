@@ -16,6 +16,7 @@
 #include "../../include/rbd.h"
 #include "k_dyn.hpp"
 #include "ops.hpp"
+#include "runtime.hpp"
 
 namespace rbd {
 #define RBD_REGISTER(NAME) const TopoOps* ops_##NAME();
@@ -38,85 +39,9 @@ static const TopoOps* const* registry(int* n) {
 
 using namespace rbd;
 
-namespace {
+namespace rbd_rt {
 thread_local std::string g_err;
-
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-#define RBD_CHECK_ARG(cond, msg) \
-  do {                           \
-    if (!(cond)) throw Error(RBD_EINVAL, msg); \
-  } while (0)
-#define HIP_CHECK(expr)                                                                            \
-  do {                                                                                             \
-    hipError_t e_ = (expr);                                                                        \
-    if (e_ != hipSuccess) throw Error(RBD_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-template<class F> int guarded(F&& f) {
-  try {
-    f();
-    return RBD_OK;
-  } catch (const Error& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "out of host memory";
-    return RBD_ENOMEM;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return RBD_EINVAL;
-  }
-}
-}  // namespace
-
-struct rbd_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  // device staging for RBD_HOST_PTRS calls, grown on demand
-  std::vector<void*> stage;
-  std::vector<size_t> stageBytes;
-  // device workspace for intermediate results (ABA-derivative factor, ddq), grown on demand
-  void* work = nullptr;
-  size_t workBytes = 0;
-  void* workspace(size_t bytes) {
-    if (workBytes < bytes) {
-      if (work) HIP_CHECK(hipFree(work));
-      work = nullptr;
-      HIP_CHECK(hipMalloc(&work, bytes));
-      workBytes = bytes;
-    }
-    return work;
-  }
-  void* stageBuf(int slot, size_t bytes) {
-    if ((int)stage.size() <= slot) {
-      stage.resize(slot + 1, nullptr);
-      stageBytes.resize(slot + 1, 0);
-    }
-    if (stageBytes[slot] < bytes) {
-      if (stage[slot]) HIP_CHECK(hipFree(stage[slot]));
-      stage[slot] = nullptr;
-      HIP_CHECK(hipMalloc(&stage[slot], bytes));
-      stageBytes[slot] = bytes;
-    }
-    return stage[slot];
-  }
-};
-
-struct rbd_model {
-  rbd_ctx* ctx = nullptr;
-  uint64_t serial = 0;  // unique per created model: mixed-batch plans key on it
-  const TopoOps* ops = nullptr;
-  int njoints = 0, nq = 0, nv = 0, compact = 0;
-  std::vector<int> parents, jtype, iq, iv;
-  std::vector<double> gravity;
-  float* mc32 = nullptr;
-  double* mc64 = nullptr;
-  // packed strictly-upper structural non-zeros of M (tree pattern), column by column
-  std::vector<int32_t> patRows, patCols;
-};
+}  // namespace rbd_rt
 
 namespace {
 
@@ -287,13 +212,13 @@ int rbd_ctx_create(int device, rbd_ctx** out) {
 int rbd_ctx_destroy(rbd_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     dropMultiPlans(ctx, nullptr);
     for (void* p : ctx->stage)
       if (p) (void)hipFree(p);
     if (ctx->work) (void)hipFree(ctx->work);
-    hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });
 }
@@ -384,11 +309,11 @@ int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* d, rbd_model** out) {
 int rbd_model_destroy(rbd_model* m) {
   return guarded([&] {
     if (!m) return;
-    hipSetDevice(m->ctx->device);
-    hipStreamSynchronize(m->ctx->stream);  // kernels in flight may still read the constants
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);  // kernels in flight may still read the constants
     dropMultiPlans(m->ctx, m);
-    if (m->mc32) hipFree(m->mc32);
-    if (m->mc64) hipFree(m->mc64);
+    if (m->mc32) (void)hipFree(m->mc32);
+    if (m->mc64) (void)hipFree(m->mc64);
     delete m;
   });
 }

@@ -1,0 +1,179 @@
+"""GPU parity of the forward-dynamics surrogate (rbd_mlp_*) against its FP64 oracle.
+
+Bars.  Features: FP32 vs FP64 within 2e-6 absolute (values are O(1) to O(60)) relative to
+each feature's magnitude.  Forward pass: against the oracle with bfloat16 rounding
+emulated where the kernels round, max |difference| within 2e-2 of the output's max
+magnitude (FP32 accumulation order and the occasional bfloat16 tie flipped by it), and
+within 1e-1 of the pure FP64 network (bfloat16 operands).  Gradients: per layer,
+||g_gpu - g_ref|| / ||g_ref|| below 3e-2 against the emulating oracle.  Device sampler:
+equal to the host sampler rounded to FP32 within 1 FP32 ulp.  The residual gate: tol = 0
+returns ABA's result bit for bit, an infinite tol returns the network's, and in between
+every kept configuration satisfies the residual bound recomputed by the FP64 oracle.
+A short training run must cut the loss and the network's RNEA residual by large factors
+over the untrained network (what training gains; the full 4M-sample run is in bench.py).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import oracle.mlp_oracle as mo
+import pinocchio_amd as pa
+from pinocchio_amd import models as M
+from pinocchio_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def quad(gpu_ctx):
+    desc = M.quadruped18()
+    return desc, pa.Model(gpu_ctx, desc)
+
+
+def host_inputs(desc, B, k, tau_sigma=20.0):
+    q, v, a, tau = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 400 + k, tau_sigma=tau_sigma)
+    return tuple(np.asarray(x, np.float32) for x in (q, v, tau))
+
+
+def dev(x):
+    return torch.tensor(np.ascontiguousarray(x), dtype=torch.float32, device="cuda")
+
+
+def random_norm(sur, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.normal(size=sur.nin) * 0.1, rng.uniform(0.5, 2.0, sur.nin), rng.normal(size=sur.nout),
+            rng.uniform(0.5, 3.0, sur.nout))
+
+
+def test_device_sampler_matches_host(quad):
+    desc, model = quad
+    B = 1001
+    q, v, a, tau = S.sample_inputs_device(model, B, M.INPUT_SEED_BASE + 9, tau_sigma=20.0, with_a=True)
+    hq, hv, ha, ht = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 9, tau_sigma=20.0)
+    for g, h in ((q, hq), (v, hv), (a, ha), (tau, ht)):
+        g = g.cpu().numpy()
+        h32 = h.astype(np.float32)
+        ulp = np.spacing(np.abs(h32))
+        assert np.all(np.abs(g - h32) <= ulp), float(np.abs(g - h32).max())
+
+
+def test_features_match_oracle(quad):
+    desc, model = quad
+    sur = S.Surrogate(model)
+    assert sur.nin == 73 and sur.nout == 18
+    q, v, tau = host_inputs(desc, 333, 0)
+    X = sur.features(dev(q), dev(v), dev(tau)).cpu().numpy()
+    ref = mo.features(desc, *(x.astype(np.float64) for x in (q, v, tau)))
+    scale = np.maximum(np.abs(ref).max(0), 1.0)
+    assert (np.abs(X - ref) / scale).max() < 2e-6
+
+
+@pytest.mark.parametrize("B", [300, 256, 1])
+def test_forward_matches_oracle(quad, B):
+    desc, model = quad
+    sur = S.Surrogate(model, seed=11)
+    norm = random_norm(sur, 1)
+    sur.set_normalization(*norm)
+    p = sur.params().astype(np.float64)
+    layers = mo.unpack(p, sur.layer_shapes())
+    q, v, tau = host_inputs(desc, B, 1)
+    out = sur.predict(dev(q), dev(v), dev(tau)).cpu().numpy().T  # (B, nv)
+    X = mo.features(desc, *(x.astype(np.float64) for x in (q, v, tau)))
+    for bf16, bar in ((True, 2e-2), (False, 1e-1)):
+        Xn = mo.normalise_inputs(X, norm[0].astype(np.float32), norm[1].astype(np.float32), bf16=bf16)
+        o, _, _ = mo.forward(layers, Xn, bf16=bf16)
+        ref = o * norm[3].astype(np.float32) + norm[2].astype(np.float32)
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        assert err < bar, (bf16, err)
+
+
+@pytest.mark.parametrize("B", [256, 200])
+def test_gradients_match_oracle(quad, B):
+    desc, model = quad
+    sur = S.Surrogate(model, seed=12)
+    norm = random_norm(sur, 2)
+    sur.set_normalization(*norm)
+    layers = mo.unpack(sur.params().astype(np.float64), sur.layer_shapes())
+    q, v, tau = host_inputs(desc, B, 2)
+    ddq = oracle.aba(desc, *(x.astype(np.float64) for x in (q, v, tau))).astype(np.float32)
+    loss, g = sur.gradients(dev(q), dev(v), dev(tau), dev(ddq))
+    X = mo.features(desc, *(x.astype(np.float64) for x in (q, v, tau)))
+    Xn = mo.normalise_inputs(X, norm[0].astype(np.float32), norm[1].astype(np.float32), bf16=True)
+    inv = (1.0 / norm[3].astype(np.float32)).astype(np.float64)
+    Yn = (ddq.T.astype(np.float64) - norm[2].astype(np.float32)) * inv
+    lref, grads = mo.loss_and_grad(layers, Xn, Yn, bf16=True)
+    assert abs(loss - lref) / lref < 1e-2, (loss, lref)
+    gl = mo.unpack(g.astype(np.float64), sur.layer_shapes())
+    for k, ((dW, db), (rW, rb)) in enumerate(zip(gl, grads)):
+        for name, x, r in (("W", dW, rW), ("b", db, rb)):
+            e = np.linalg.norm(x - r) / np.linalg.norm(r)
+            assert e < 3e-2, (k, name, e)
+
+
+def test_params_roundtrip_and_errors(quad):
+    desc, model = quad
+    sur = S.Surrogate(model, hidden=128, nhidden=2, seed=3)
+    p = sur.params()
+    assert p.size == sur.nparams == 73 * 128 + 128 + 128 * 128 + 128 + 128 * 18 + 18
+    p2 = np.random.default_rng(0).normal(size=p.size).astype(np.float32)
+    sur.set_params(p2)
+    assert np.array_equal(sur.params(), p2)
+    with pytest.raises(ValueError):
+        S.Surrogate(model, hidden=100)
+    with pytest.raises(ValueError):
+        sur.set_normalization(np.zeros(73), np.zeros(73), np.zeros(18), np.ones(18))
+
+
+def test_forward_dynamics_gate(quad, gpu_ctx):
+    desc, model = quad
+    sur = S.Surrogate(model, seed=4)
+    q, v, tau = host_inputs(desc, 1000, 3)
+    Q, V, T = dev(q), dev(v), dev(tau)
+    eng = pa.Engine(model, "batch")
+    aba = eng.aba(Q, V, T)
+    pred = sur.predict(Q, V, T)
+    # tol = 0: everything falls back to ABA, bit for bit
+    out, nfb = sur.forward_dynamics(Q, V, T, tol=0.0)
+    assert nfb == 1000 and torch.equal(out, aba)
+    # infinite tolerance: the network's output unchanged
+    out, nfb = sur.forward_dynamics(Q, V, T, tol=float("inf"))
+    assert nfb == 0 and torch.equal(out, pred)
+    # in between: kept configurations satisfy the bound (FP64 oracle RNEA), others are ABA's
+    out, nfb, acc, res = sur.forward_dynamics(Q, V, T, tol=float(np.median(res_of(sur, desc, Q, V, T))),
+                                              want_mask=True)
+    acc = acc.cpu().numpy().astype(bool)
+    assert 0 < nfb < 1000 and nfb == (~acc).sum()
+    o = out.cpu().numpy()
+    assert np.array_equal(o[:, ~acc], aba.cpu().numpy()[:, ~acc])
+    assert np.array_equal(o[:, acc], pred.cpu().numpy()[:, acc])
+    tr = oracle.rnea(desc, q.astype(np.float64)[:, acc], v.astype(np.float64)[:, acc], o[:, acc].astype(np.float64))
+    t = tau.astype(np.float64)[:, acc]
+    rel = np.linalg.norm(tr - t, axis=0) / np.linalg.norm(t, axis=0)
+    r32 = res.cpu().numpy()[acc]
+    assert np.allclose(rel, r32, rtol=1e-3, atol=1e-6)
+
+
+def res_of(sur, desc, Q, V, T):
+    _, _, _, res = sur.forward_dynamics(Q, V, T, tol=float("inf"), want_mask=True)
+    return res.cpu().numpy()
+
+
+def test_training_gain(quad):
+    """A short training run (65536 GPU-ABA samples, 4 epochs) against the untrained network:
+    held-out loss and median RNEA residual must both drop severalfold."""
+    desc, model = quad
+    data = S.make_dataset(model, 65536, M.INPUT_SEED_BASE + 500)
+    sur = S.Surrogate(model, seed=5)
+    sur.fit_normalization(*data)
+    q, v, tau = host_inputs(desc, 4096, 4)
+    Q, V, T = dev(q), dev(v), dev(tau)
+    D = pa.Engine(model, "batch").aba(Q, V, T)
+    l0, _ = sur.gradients(Q, V, T, D)
+    r0 = np.median(res_of(sur, desc, Q, V, T))
+    losses, secs = S.train(sur, data, epochs=4, batch=1024, lr=2e-3)
+    l1, _ = sur.gradients(Q, V, T, D)
+    r1 = np.median(res_of(sur, desc, Q, V, T))
+    print(f"held-out loss {l0:.4f} -> {l1:.4f}, median residual {r0:.4f} -> {r1:.4f}, epochs {losses}, {secs:.2f} s")
+    assert l1 < 0.3 * l0, (l0, l1)
+    assert r1 < 0.5 * r0, (r0, r1)
