@@ -71,6 +71,53 @@ class Timer:
         self.acc = {}
 
 
+def surrogate_bench(torch, pa, M, ctx, args):
+    """Learned forward dynamics on the 18-DoF quadruped: train the 73-512-512-512-18 SiLU
+    network on GPU-ABA samples (q as the headline inputs, v ~ N(0, 1), tau ~ N(0, 20^2)),
+    then run surrogate + RNEA-residual gate (1e-3) + ABA fallback on 262144 held-out
+    configurations next to plain ABA on the same inputs."""
+    from pinocchio_amd import surrogate as S
+    desc = M.quadruped18()
+    model = pa.Model(ctx, desc)
+    t0 = time.perf_counter()
+    data = S.make_dataset(model, args.sur_samples, M.INPUT_SEED_BASE + 1000)
+    t_data = time.perf_counter() - t0
+    sur = S.Surrogate(model)
+    sur.fit_normalization(*data)
+    losses, t_train = S.train(sur, data, epochs=args.sur_epochs, batch=4096, lr=5e-4)
+    del data
+    n = 262144
+    Q, V, T = S.sample_inputs_device(model, n, M.INPUT_SEED_BASE + 2000, tau_sigma=20.0)
+    eng = pa.Engine(model, "batch")
+    out = torch.empty((desc.nv, n), dtype=torch.float32, device=Q.device)
+
+    def tm(f, reps=10):
+        f()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            f()
+        ctx.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    t_aba = tm(lambda: eng.aba(Q, V, T, out=out, async_=True))
+    t_pred = tm(lambda: sur.predict(Q, V, T, out=out, async_=True))
+    t_fd = tm(lambda: sur.forward_dynamics(Q, V, T, tol=1e-3, out=out))
+    _, nfb, acc, res = sur.forward_dynamics(Q, V, T, tol=1e-3, out=out, want_mask=True)
+    r = res.cpu().numpy()
+    fwd_flop = 2 * sum(o * i for o, i in sur.layer_shapes())
+    return {
+        "train_samples": args.sur_samples, "epochs": args.sur_epochs, "batch": 4096, "train_s": t_train,
+        "data_s": t_data, "train_loss_per_epoch": [float(x) for x in losses],
+        "infer_batch": n, "tol": 1e-3, "fallbacks": int(nfb), "accepted_frac": float(1 - nfb / n),
+        "residual_p50": float(np.median(r)), "residual_p90": float(np.quantile(r, 0.9)),
+        "accepted_frac_at": {str(t): float((r <= t).mean()) for t in (1e-3, 1e-2, 1e-1)},
+        "aba_configs_per_s": n / t_aba, "surrogate_with_fallback_configs_per_s": n / t_fd,
+        "surrogate_only_configs_per_s": n / t_pred, "speedup_vs_aba": t_aba / t_fd,
+        "mlp_forward_TFLOP_s": fwd_flop * n / t_pred / 1e12, "mlp_frac_of_bf16_peak": fwd_flop * n / t_pred / 2.5e15,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +127,9 @@ def main():
     ap.add_argument("--deriv-batch", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-samples", type=int, default=1 << 22)
+    ap.add_argument("--sur-epochs", type=int, default=8)
     ap.add_argument("--only", default=None, help="profile one part: step|aba|crba|chol|rnea|rnead|abad")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -107,6 +157,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"only": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
                           "KiB_per_dispatch": 4 * n / 1024}))
+        return
+
+    if args.only == "surrogate":
+        print(json.dumps({"only": "surrogate", **surrogate_bench(torch, pa, M, ctx, args)}))
         return
 
     desc = M.humanoid36()
@@ -271,6 +325,8 @@ def main():
     others["mixed_rnea_aba_f32"] = {"instances": n_mix, "groups": dict(mix_spec), "configs_per_s": n_mix / t,
                                     "ms": t * 1e3, "chosen": tune.get("chosen"),
                                     "candidates_ms": {c["variant"]: c["ms"] for c in tune.get("candidates", [])}}
+    if not args.no_surrogate:
+        others["surrogate_quadruped18"] = surrogate_bench(torch, pa, M, ctx, args)
 
     # ---------------- accuracy against the FP64 oracle ----------------
     errors = {}

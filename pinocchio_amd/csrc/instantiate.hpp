@@ -1,6 +1,8 @@
 // instantiate.hpp -- per-topology kernel instantiation and launcher tables.
 #pragma once
 
+#include <cstdlib>
+
 #include "ops.hpp"
 #include "k_dyn.hpp"
 #include "k_crba.hpp"
@@ -15,8 +17,40 @@ template<class TP, class S>
 void launchRnea(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> out) {
   hipLaunchKernelGGL((k_rnea<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, out);
 }
+inline int envInt(const char* name) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : 0;
+}
+// Below about four tiles per CU the sweeps are latency-bound with one wave per tile.
+inline int wavesForBatch(int64_t B) {
+  const int64_t tiles = (B + kTile - 1) / kTile;
+  return tiles >= 1024 ? 1 : tiles >= 512 ? 2 : 4;
+}
+// a subtree split pays when the busiest wave (trunk included) has well under the whole tree
+template<class TP, int G> constexpr bool abaSplitPays() {
+  constexpr WaveSplit<TP, G> WS{};
+  int trunk = 0, busiest = 0;
+  for (int j = 1; j < TP::NJ; ++j) trunk += WS.owner[j] < 0 ? 1 : 0;
+  for (int k = 0; k < G; ++k) busiest = WS.load[k] > busiest ? WS.load[k] : busiest;
+  return 5 * (trunk + busiest) <= 3 * (TP::NJ - 1) && abaWavesLdsBytes<TP, G>() <= 160 * 1024;
+}
+// RBD_ABA_WAVES=1|2|4 fixes the waves per tile (default: chosen from the batch size)
 template<class TP, class S>
 void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
+  static const int forced = envInt("RBD_ABA_WAVES");
+  const int waves = forced > 0 ? forced : wavesForBatch(B);
+  if constexpr (abaSplitPays<TP, 4>()) {
+    if (waves >= 4) {
+      hipLaunchKernelGGL((k_aba_waves<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, tau, out);
+      return;
+    }
+  }
+  if constexpr (abaSplitPays<TP, 2>()) {
+    if (waves >= 2) {
+      hipLaunchKernelGGL((k_aba_waves<TP, S, 2>), dim3(gridFor(B)), dim3(kTile * 2), 0, st, mc, B, q, v, tau, out);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_aba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, tau, out);
 }
 template<class TP, class S>
@@ -33,10 +67,15 @@ template<class TP, class S>
 void launchCholSolve(hipStream_t st, int64_t B, CView<double> U, CView<double> D, View<S> x) {
   hipLaunchKernelGGL((k_chol64_solve<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, U, D, x);
 }
+// more waves per tile (each emitting a share of the outputs) shorten the critical path
 template<class TP, class S>
 void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
-                 View<S> M, int mask) {
-  hipLaunchKernelGGL((k_rnead<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, dq, dv, M, mask);
+                 View<S> M, int mask, int waves) {
+  if (waves <= 0) waves = wavesForBatch(B);
+  if (waves >= 2)
+    hipLaunchKernelGGL((k_rnead<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, a, dq, dv, M, mask);
+  else
+    hipLaunchKernelGGL((k_rnead<TP, S, 1>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, dq, dv, M, mask);
 }
 template<class TP, class S>
 void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv) {

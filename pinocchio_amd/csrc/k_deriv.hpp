@@ -85,23 +85,64 @@ template<class S> struct ColData {
 // ---------------------------------------------------------------------------------------
 // computeRNEADerivatives: dtau_dq, dtau_dv, M (any may be skipped) and optionally tau
 // ---------------------------------------------------------------------------------------
-template<class TP, class S>
-__global__ __launch_bounds__(64) void k_rnead(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
-                                              CView<S> ga, View<S> gdq, View<S> gdv, View<S> gM, int writeMask) {
+// Output-emission groups for G waves sharing one tile of configurations: every wave runs
+// the forward pass and the subtree accumulations, and joint i's output entries are
+// computed and stored by wave grp[i] only.  Joints are dealt to the least-loaded wave in
+// decreasing order of their emission cost nv_i (nv_i + ancestor dofs).
+template<class TP, int G> struct EmitGroups {
+  int grp[TP::NJ];
+  constexpr EmitGroups() : grp{} {
+    constexpr auto& T = TP::T;
+    long load[8] = {};
+    bool done[TP::NJ] = {};
+    for (int it = 1; it < TP::NJ; ++it) {
+      int best = 1;
+      long bc = -1;
+      for (int j = 1; j < TP::NJ; ++j) {
+        if (done[j]) continue;
+        int anc = 0;
+        for (int a = T.parent[j]; a > 0; a = T.parent[a]) anc += T.nvj[a];
+        const long c = (long)T.nvj[j] * (anc + T.nvj[j]);
+        if (c > bc) {
+          bc = c;
+          best = j;
+        }
+      }
+      done[best] = true;
+      int g = 0;
+      for (int k = 1; k < G; ++k)
+        if (load[k] < load[g]) g = k;
+      grp[best] = g;
+      load[g] += bc;
+    }
+  }
+};
+
+template<class TP, class S, int G = 1>
+__global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
+                                                  CView<S> ga, View<S> gdq, View<S> gdv, View<S> gM, int writeMask) {
+  static_assert(G >= 1 && G <= 8, "1 to 8 waves per tile");
   constexpr auto& T = TP::T;
   constexpr int NJ = TP::NJ, NQ = T.nq, NV = T.nv;
+  constexpr EmitGroups<TP, G> EG{};
   __shared__ S sq[NQ * kPitch];
   __shared__ S sv[NV * kPitch];
   __shared__ S sa[NV * kPitch];
   const int64_t b0 = int64_t(blockIdx.x) * kTile;
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  stageIn<NQ>(sq, gq, b0, nb);
-  stageIn<NV>(sv, gv, b0, nb);
-  stageIn<NV>(sa, ga, b0, nb);
+  const int w = G > 1 ? (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
+  if (w == 0) {
+    stageIn<NQ>(sq, gq, b0, nb);
+    stageIn<NV>(sv, gv, b0, nb);
+    stageIn<NV>(sa, ga, b0, nb);
+  }
   __syncthreads();
   const int L = laneId();
   const int64_t b = b0 + L;
   const bool live = L < nb;
+#ifdef RBD_RNEAD_FORCE_MASK
+  writeMask = RBD_RNEAD_FORCE_MASK;
+#endif
   const bool wq = live && (writeMask & 1), wv = live && (writeMask & 2), wM = live && (writeMask & 4);
   const S* gr = mc_in + mcGravity<NJ, NV>();
   const Mot<S> g = {{gr[0], gr[1], gr[2]}, {gr[3], gr[4], gr[5]}};
@@ -180,6 +221,7 @@ __global__ __launch_bounds__(64) void k_rnead(const S* __restrict__ mc_in, int64
         dyAdd(DYc, DYacc[i]);
         of = of + Facc[i];
       }
+      if (G == 1 || w == EG.grp[i]) {
 #pragma unroll
       for (int k = 0; k < nvi; ++k) {
         const int c = iv + k;
@@ -224,6 +266,7 @@ __global__ __launch_bounds__(64) void k_rnead(const S* __restrict__ mc_in, int64
           }
         });
       }
+      }
       if constexpr (p > 0) {
         if constexpr (TP::T.childIdx[i] == 0) {
           Yacc[p] = Y;
@@ -240,7 +283,7 @@ __global__ __launch_bounds__(64) void k_rnead(const S* __restrict__ mc_in, int64
   if (live) {
     static_for<0, NV>([&](auto C) {
       static_for<0, NV>([&](auto R) {
-        if constexpr (!CrbaPattern<TP>::nz(R, C)) {
+        if (constexpr int c = C; !CrbaPattern<TP>::nz(R, C) && (G == 1 || w == c % G)) {
           if (wq) stMat(gdq, b, NV, R, C, S(0));
           if (wv) stMat(gdv, b, NV, R, C, S(0));
           if (wM) stMat(gM, b, NV, R, C, S(0));

@@ -139,20 +139,35 @@ __global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc, int64_t B
 
 // ---------------------------------------------------------------------------------------
 // ABA: ddq = FD(q, v, tau)
+//
+// G = 1: one wave runs the whole tree for its 64 configurations.  G > 1 (small batches,
+// where one wave per tile leaves most SIMDs idle): G waves share the tile and split the
+// tree by subtrees (WaveSplit); this function is then the body of wave W.  Trunk joints
+// are computed by every wave; a subtree hands its articulated inertia and bias force to
+// its trunk parent through an LDS slot (xs), read after a barrier.  ddq goes to a
+// separate LDS block (so) because trunk joints are evaluated by several waves.
 // ---------------------------------------------------------------------------------------
-template<class TP, class S>
+constexpr int kAbaSlot = 27;  // floats per slot: 21 (articulated inertia) + 6 (bias force)
+template<class TP> constexpr int abaLdsRows(int G) { return TP::T.nq + (G > 1 ? 3 : 2) * TP::T.nv; }
+
+template<class TP, class S, int G = 1, int W = 0>
 __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, int64_t b0, CView<S> gq,
-                                        CView<S> gv, CView<S> gt, View<S> gddq, S* __restrict__ lds) {
+                                        CView<S> gv, CView<S> gt, View<S> gddq, S* __restrict__ lds,
+                                        S* __restrict__ xs = nullptr) {
   using C = DynCfg<TP, S>;
   constexpr auto& T = TP::T;
   constexpr int NJ = C::NJ, NQ = C::NQ, NV = C::NV;
+  constexpr WaveSplit<TP, G> WS{};
   S* sq = lds;
   S* sv = lds + NQ * kPitch;
-  S* st = lds + (NQ + NV) * kPitch;  // tau in, ddq out
+  S* st = lds + (NQ + NV) * kPitch;                    // tau in
+  S* so = G > 1 ? lds + (NQ + 2 * NV) * kPitch : st;  // ddq out
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  stageIn<NQ>(sq, gq, b0, nb);
-  stageIn<NV>(sv, gv, b0, nb);
-  stageIn<NV>(st, gt, b0, nb);
+  if (W == 0) {
+    stageIn<NQ>(sq, gq, b0, nb);
+    stageIn<NV>(sv, gv, b0, nb);
+    stageIn<NV>(st, gt, b0, nb);
+  }
   __syncthreads();
   const int L = laneId();
   const Mot<S> a0 = mcGravityNeg(mc_in, mcGravity<NJ, NV>());
@@ -167,13 +182,13 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
   // ---- passes 1 + 2, interleaved in DFS order ----
   static_for<0, T.nev>([&](auto E) {
     constexpr int e = E;
+    constexpr int i = TP::T.evJoint[e];
+    if constexpr (WS.runs(i, W)) {
     stepFence();
     const S* __restrict__ mc = stepConsts(mc_in);
-    constexpr int i = TP::T.evJoint[e];
     constexpr int p = TP::T.parent[i];
     constexpr int ty = TP::T.type[i];
     constexpr int iq = TP::T.iq[i], iv = TP::T.iv[i];
-    constexpr bool hasChildren = TP::T.nchild[i] > 0;
     if constexpr (TP::T.evKind[e] == EV_PRE) {
       Xform<S> X;
       Mot<S> jv;
@@ -204,9 +219,25 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
       const Body<S> Y = mcBody(mc, i);
       Sym6<S> Ia = mcSym6(mc, i);
       Frc<S> pA = crossMF(vj[i], inertiaMul(Y, vj[i]));
-      if constexpr (hasChildren) {
+      if constexpr (WS.nReg[i] > 0) {
         symAdd(Ia, Iacc[i]);
         pA = pA + pacc[i];
+      }
+      if constexpr (G > 1 && WS.ldsIn[i]) {
+        // contributions of subtrees owned by other waves (fixed order: identical in every wave)
+        __syncthreads();
+        static_for<1, NJ>([&](auto Cc) {
+          constexpr int c = Cc;
+          if constexpr (TP::T.parent[c] == i && WS.slot[c] >= 0) {
+            const S* x = xs + WS.slot[c] * kAbaSlot * kTile + L;
+            Sym6<S> Ic;
+#pragma unroll
+            for (int k = 0; k < 21; ++k) Ic.a[k] = x[k * kTile];
+            symAdd(Ia, Ic);
+            pA = pA + Frc<S>{{x[21 * kTile], x[22 * kTile], x[23 * kTile]},
+                             {x[24 * kTile], x[25 * kTile], x[26 * kTile]}};
+          }
+        });
       }
       if constexpr (ty == JT_FF) {
         // root free-flyer: D = Ia + armature (6x6, FP64 LLT), qdd = D^-1 (u - Ia a')
@@ -225,8 +256,10 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
           rhs[r] = double(st[(iv + r) * kPitch + L]) - double(comp(pA, r)) - double(comp(Iap, r));
         }
         spdSolve6(A, rhs, x);
+        if constexpr (WS.writer(i) == W) {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) st[(iv + r) * kPitch + L] = S(x[r]);
+          for (int r = 0; r < 6; ++r) so[(iv + r) * kPitch + L] = S(x[r]);
+        }
         ahat[i] = {{ap.l.x + S(x[0]), ap.l.y + S(x[1]), ap.l.z + S(x[2])},
                    {ap.a.x + S(x[3]), ap.a.y + S(x[4]), ap.a.z + S(x[5])}};
         pinAll<S>(ahat[i]);
@@ -247,23 +280,37 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
           const Xform<S> X = revoluteX(mc, i, ax, sn[i], cs[i]);
           const Sym6<S> Iup = transportUp(X, Ia);
           const Frc<S> pup = act(X, pa);
-          if constexpr (TP::T.childIdx[i] == 0) {
-            Iacc[p] = Iup;
-            pacc[p] = pup;
+          if constexpr (G > 1 && WS.slot[i] >= 0) {
+            S* x = xs + WS.slot[i] * kAbaSlot * kTile + L;
+#pragma unroll
+            for (int k = 0; k < 21; ++k) x[k * kTile] = Iup.a[k];
+            x[21 * kTile] = pup.l.x;
+            x[22 * kTile] = pup.l.y;
+            x[23 * kTile] = pup.l.z;
+            x[24 * kTile] = pup.a.x;
+            x[25 * kTile] = pup.a.y;
+            x[26 * kTile] = pup.a.z;
           } else {
-            symAdd(Iacc[p], Iup);
-            pacc[p] = pacc[p] + pup;
+            if constexpr (WS.regIdx[i] == 0) {
+              Iacc[p] = Iup;
+              pacc[p] = pup;
+            } else {
+              symAdd(Iacc[p], Iup);
+              pacc[p] = pacc[p] + pup;
+            }
+            pinAll<S>(Iacc[p]);
+            pinAll<S>(pacc[p]);
           }
-          pinAll<S>(Iacc[p]);
-          pinAll<S>(pacc[p]);
         }
       }
+    }
     }
   });
 
   // ---- pass 3: accelerations, DFS pre-order ----
   static_for<0, NJ - 1>([&](auto K) {
     constexpr int i = TP::T.order[K];
+    if constexpr (WS.runs(i, W)) {
     stepFence();
     const S* __restrict__ mc = stepConsts(mc_in);
     constexpr int p = TP::T.parent[i];
@@ -276,15 +323,16 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
       if constexpr (p > 0) t = actInv(X, ahat[p]);
       else t = actInv(X, a0);
       const S qdd = ss[i] - dot(t, dd[i]);
-      st[iv * kPitch + L] = qdd;
+      if constexpr (WS.writer(i) == W) so[iv * kPitch + L] = qdd;
       if constexpr (TP::T.nchild[i] > 0) {
         ahat[i] = t + axisMot(ax, qdd);
         pinAll<S>(ahat[i]);
       }
     }
+    }
   });
   __syncthreads();
-  stageOut<NV>(st, gddq, b0, nb);
+  if (W == 0) stageOut<NV>(so, gddq, b0, nb);
 }
 
 template<class TP, class S>
@@ -292,6 +340,24 @@ __global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc, int64_t B,
                                             CView<S> gt, View<S> gddq) {
   __shared__ S lds[dynLdsRows<TP>() * kPitch];
   abaTile<TP, S>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, gt, gddq, lds);
+}
+
+// G waves per 64-configuration tile, split by subtrees (small batches)
+template<class TP, int G> constexpr int abaWavesLdsBytes() {
+  constexpr WaveSplit<TP, G> WS{};
+  return (abaLdsRows<TP>(G) * kPitch + (WS.nslots > 0 ? WS.nslots : 1) * kAbaSlot * kTile) * 4;
+}
+template<class TP, class S, int G>
+__global__ __launch_bounds__(64 * G) void k_aba_waves(const S* __restrict__ mc, int64_t B, CView<S> gq, CView<S> gv,
+                                                      CView<S> gt, View<S> gddq) {
+  constexpr WaveSplit<TP, G> WS{};
+  __shared__ S lds[abaLdsRows<TP>(G) * kPitch];
+  __shared__ S xs[(WS.nslots > 0 ? WS.nslots : 1) * kAbaSlot * kTile];
+  const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  static_for<0, G>([&](auto Wc) {
+    constexpr int W = Wc;
+    if (w == W) abaTile<TP, S, G, W>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, gt, gddq, lds, xs);
+  });
 }
 
 // ---------------------------------------------------------------------------------------

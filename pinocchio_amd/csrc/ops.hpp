@@ -17,9 +17,10 @@ template<class S>
 using LaunchChol = void (*)(hipStream_t, int64_t B, CView<S> M, View<double> U, View<double> D);
 template<class S>
 using LaunchCholSolve = void (*)(hipStream_t, int64_t B, CView<double> U, CView<double> D, View<S> x);
+// waves: 1 or 4 (any value >= 2) waves share a tile of 64 configurations (0: chosen from the batch size)
 template<class S>
 using LaunchRneaD = void (*)(hipStream_t, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq,
-                             View<S> dv, View<S> M, int mask);
+                             View<S> dv, View<S> M, int mask, int waves);
 // FP32 factor for the ABA-derivative pipeline
 using LaunchChol32 = void (*)(hipStream_t, int64_t B, CView<float> M, View<float> U, View<float> D);
 template<class S>

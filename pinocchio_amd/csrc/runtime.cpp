@@ -7,6 +7,7 @@
 #include <cmath>
 #include <memory>
 #include <type_traits>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -432,6 +433,15 @@ int rbd_cholesky_solve(rbd_ctx* ctx, const rbd_model* m, int64_t B, const double
 }
 
 namespace {
+// RBD_RNEAD_WAVES=1|2|4 fixes the waves per tile of the RNEA-derivative sweep (default:
+// chosen from the batch size)
+int rneadWavesOverride() {
+  static const int w = [] {
+    const char* e = std::getenv("RBD_RNEAD_WAVES");
+    return e ? std::atoi(e) : 0;
+  }();
+  return w;
+}
 void checkDerivativeModel(const rbd_model* m) {
   if (!m->compact) throw Error(RBD_EUNSUPPORTED, "derivatives: the model is not DFS-compact");
   if (m->gravity[3] != 0.0 || m->gravity[4] != 0.0 || m->gravity[5] != 0.0)
@@ -465,7 +475,7 @@ int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
     const int mask = (dq ? 1 : 0) | (dv ? 2 : 0) | (da ? 4 : 0);
     m->ops->rnead_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
                       CView<float>{da_, va.rs, va.bs}, View<float>{o1, vo.rs, vo.bs}, View<float>{o2, vo.rs, vo.bs},
-                      View<float>{o3, vo.rs, vo.bs}, mask);
+                      View<float>{o3, vo.rs, vo.bs}, mask, rneadWavesOverride());
     finishCall(ctx, flags, st);
   });
 }
@@ -511,7 +521,7 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, 
     // 2. dtau_dq, dtau_dv, M at a = ddq, into the output buffers
     m->ops->rnead_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
                       CView<float>{vddq.p, vddq.rs, vddq.bs}, View<float>{Xq, vo.rs, vo.bs},
-                      View<float>{Xv, vo.rs, vo.bs}, View<float>{Mi, vo.rs, vo.bs}, 7);
+                      View<float>{Xv, vo.rs, vo.bs}, View<float>{Mi, vo.rs, vo.bs}, 7, rneadWavesOverride());
     // 3. tree factor of M, then Minv and -Minv dtau in place
     m->ops->chol32_f32(ctx->stream, B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1}, View<float>{wD, B, 1});
     m->ops->abad_solve_f32(ctx->stream, B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1},

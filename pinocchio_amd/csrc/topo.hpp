@@ -139,6 +139,85 @@ template<int B, int E, class F> __host__ __device__ __forceinline__ void static_
   }
 }
 
+// Subtree split of one sweep over G waves that share a tile of configurations.
+// Subtrees larger than total / G are opened (their root joins the "trunk", which every
+// wave computes redundantly) until the remaining subtrees fit; those are dealt to the
+// waves largest first.  A wave owns whole subtrees, so the forward and backward passes of
+// a subtree never leave its wave; what a subtree hands to its trunk parent goes through
+// an LDS slot (one per subtree root below the trunk), read after a workgroup barrier.
+//   owner[j]   -1 for trunk joints, else the wave owning j
+//   slot[j]    LDS slot of subtree root j whose parent is a trunk joint, else -1
+//   ldsIn[j]   trunk joint j receives at least one contribution through a slot
+//   nReg[j]    children of j whose contributions stay in registers (same owner)
+//   regIdx[j]  rank of j among its parent's register children
+template<class TP, int G> struct WaveSplit {
+  static constexpr int NJ = TP::NJ;
+  int owner[NJ] = {}, slot[NJ] = {}, nReg[NJ] = {}, regIdx[NJ] = {};
+  bool ldsIn[NJ] = {};
+  int nslots = 0;
+  int load[8] = {};
+  constexpr WaveSplit() {
+    constexpr auto& T = TP::T;
+    bool trunk[NJ] = {}, front[NJ] = {};
+    for (int j = 1; j < NJ; ++j) front[j] = T.parent[j] == 0;
+    if (G > 1) {
+      const int target = (NJ - 1 + G - 1) / G;
+      for (int it = 0; it < NJ; ++it) {
+        int big = -1;
+        for (int j = 1; j < NJ; ++j)
+          if (front[j] && (big < 0 || T.subsize[j] > T.subsize[big])) big = j;
+        if (big < 0 || T.subsize[big] <= target || T.nchild[big] == 0) break;
+        front[big] = false;
+        trunk[big] = true;
+        for (int c = 1; c < NJ; ++c)
+          if (T.parent[c] == big) front[c] = true;
+      }
+    }
+    // deal frontier subtrees to waves, largest first
+    int rootOwner[NJ] = {};
+    bool done[NJ] = {};
+    for (int it = 0; it < NJ; ++it) {
+      int big = -1;
+      for (int j = 1; j < NJ; ++j)
+        if (front[j] && !done[j] && (big < 0 || T.subsize[j] > T.subsize[big])) big = j;
+      if (big < 0) break;
+      done[big] = true;
+      int w = 0;
+      for (int k = 1; k < G; ++k)
+        if (load[k] < load[w]) w = k;
+      rootOwner[big] = w;
+      load[w] += T.subsize[big];
+    }
+    for (int j = 1; j < NJ; ++j) {
+      if (trunk[j]) {
+        owner[j] = -1;
+        continue;
+      }
+      int r = j;
+      while (!front[r]) r = T.parent[r];
+      owner[j] = rootOwner[r];
+    }
+    owner[0] = -1;
+    for (int j = 1; j < NJ; ++j) {
+      const int p = T.parent[j];
+      slot[j] = -1;
+      if (p > 0 && owner[j] != owner[p]) {
+        slot[j] = nslots++;
+        ldsIn[p] = true;
+      }
+    }
+    // register children in DFS order (the order their CLOSE events come in)
+    for (int k = 0; k < NJ - 1; ++k) {
+      const int j = T.order[k];
+      const int p = T.parent[j];
+      if (p > 0 && slot[j] < 0) regIdx[j] = nReg[p]++;
+    }
+  }
+  constexpr bool runs(int j, int w) const { return owner[j] < 0 || owner[j] == w; }
+  // the wave that stores joint j's outputs (wave 0 for trunk joints)
+  constexpr int writer(int j) const { return owner[j] < 0 ? 0 : owner[j]; }
+};
+
 // Per-joint record of model constants (device buffer, read with uniform loads).
 // Offsets in scalars; one record per joint, joint 0 included.
 enum : int {

@@ -174,3 +174,36 @@ def test_derivative_argument_errors(gpu_ctx):
     rm = pa.Model(gpu_ctx, r)
     with pytest.raises(NotImplementedError, match="compact"):
         pa.Engine(rm, "col").crba(torch.zeros((2, 61), device="cuda"))
+
+
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "random60"])
+def test_wave_split_variants_agree(gpu_ctx, name):
+    """The sweeps pick 1, 2 or 4 waves per 64-configuration tile from the batch size
+    (subtree split for ABA, output split for the RNEA derivatives).  The same
+    configurations evaluated inside batches of 70000 (1 wave), 40000 (2) and 1000 (4) must
+    agree with each other and with the oracle."""
+    desc, model = gpu_model(gpu_ctx, name)
+    eng = pa.Engine(model, "batch")
+    n = 1000
+    outs = {}
+    for B in (70000, 40000, n):
+        q, v, a, tau = f32_round(*M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 31))
+        Q, V, T = (to_dev(x, "batch") for x in (q, v, tau))
+        outs[B] = eng.aba(Q, V, T)[:, :n].double().cpu().numpy()
+        if B == n:
+            ref = oracle.aba(desc, q, v, tau)
+    for B, o in outs.items():
+        e = np.abs(o - ref).max(0) / np.maximum(np.abs(ref).max(0), 1e-30)
+        assert e.max() < 1e-4, (B, e.max())
+    d = np.abs(outs[70000] - outs[n]).max(0) / np.maximum(np.abs(outs[n]).max(0), 1e-30)
+    assert d.max() < 2e-5
+    if model.compact and name != "random60":
+        res = {}
+        for B in (65536, n):
+            q, v, a, tau = f32_round(*M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 32))
+            Q, V, A = (to_dev(x, "batch") for x in (q, v, a))
+            res[B] = [x[:, :n].cpu() for x in eng.rnea_derivatives(Q, V, A)]
+        for x1, x4 in zip(res[65536], res[n]):
+            # the same arithmetic per entry, only the emitting wave differs
+            a1, a4 = x1.double().numpy(), x4.double().numpy()
+            assert (np.abs(a1 - a4).max(0) / np.maximum(np.abs(a4).max(0), 1e-30)).max() < 1e-6
