@@ -67,7 +67,8 @@ template<class TP, class S>
 void launchCholSolve(hipStream_t st, int64_t B, CView<double> U, CView<double> D, View<S> x) {
   hipLaunchKernelGGL((k_chol64_solve<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, U, D, x);
 }
-// more waves per tile (each emitting a share of the outputs) shorten the critical path
+// RNEA derivatives: with few tiles, 4 waves per tile, each emitting the output entries of
+// a share of the joints (EmitGroups), shorten the critical path
 template<class TP, class S>
 void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
                  View<S> M, int mask, int waves) {

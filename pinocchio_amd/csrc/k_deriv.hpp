@@ -395,7 +395,7 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
     S* p = colPtr(j);
     const S sgn = j < 2 * NV ? S(-1) : S(1);
 #pragma unroll
-    for (int r = 0; r < NV; ++r) p[r * rs] = sgn * x[r];
+    for (int r = 0; r < NV; ++r) stOut(&p[r * rs], sgn * x[r]);
   };
 #pragma unroll 1
   for (int j = 2 * w; j < 3 * NV; j += 2 * kSolveWaves) {
