@@ -130,7 +130,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sur-samples", type=int, default=1 << 22)
     ap.add_argument("--sur-epochs", type=int, default=8)
-    ap.add_argument("--only", default=None, help="profile one part: step|aba|crba|chol|rnea|rnead|abad")
+    ap.add_argument("--only", default=None,
+                    help="one part: step|aba|crba|chol|rnea|rnead|abad|mlp|surrogate|copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; the batch shards with no "
@@ -157,6 +158,21 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"only": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
                           "KiB_per_dispatch": 4 * n / 1024}))
+        return
+
+    if args.only == "mlp":
+        # surrogate inference kernels alone (untrained weights: the kernel work is the same)
+        from pinocchio_amd import surrogate as S
+        dq = M.quadruped18()
+        mq = pa.Model(ctx, dq)
+        sur = S.Surrogate(mq)
+        n = 262144
+        Qm, Vm, Tm = S.sample_inputs_device(mq, n, M.INPUT_SEED_BASE + 2000, tau_sigma=20.0)
+        out = torch.empty((dq.nv, n), dtype=torch.float32, device=dev)
+        for _ in range(args.warmup + args.steps):
+            sur.predict(Qm, Vm, Tm, out=out, async_=True)
+        ctx.synchronize()
+        print(json.dumps({"only": "mlp", "batch": n}))
         return
 
     if args.only == "surrogate":

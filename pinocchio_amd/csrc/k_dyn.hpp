@@ -318,7 +318,11 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
     constexpr int iv = TP::T.iv[i];
     if constexpr (ty != JT_FF) {
       constexpr int ax = ty - JT_RX;
-      const Xform<S> X = revoluteX(mc, i, ax, sn[i], cs[i]);
+      // the joint's sine and cosine are recomputed rather than kept from pass 1: two
+      // registers per joint across the whole backward pass cost more than the polynomial
+      S sn3, cs3;
+      sincosT(sq[TP::T.iq[i] * kPitch + L], &sn3, &cs3);
+      const Xform<S> X = revoluteX(mc, i, ax, sn3, cs3);
       Mot<S> t;
       if constexpr (p > 0) t = actInv(X, ahat[p]);
       else t = actInv(X, a0);

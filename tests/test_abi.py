@@ -66,3 +66,15 @@ def test_inputs_distribution():
     assert abs(v.std() - 1.0) < 0.02 and abs(tau.std() - 20.0) < 0.4
     q2, *_ = M.sample_inputs(m, 20000, 99)
     np.testing.assert_array_equal(q, q2)
+
+
+def test_cpp_dropin_header_compiles_without_hip_or_eigen():
+    """The C++ drop-in needs only the C ABI: a host compiler with no HIP or Eigen headers
+    on the include path compiles a caller (tests/cpp/dropin_check.cpp)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    assert gxx, "g++ missing"
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

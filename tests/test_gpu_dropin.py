@@ -1,0 +1,64 @@
+"""The C++ drop-in (include/pinocchio_amd/parallel.hpp) driven like a pinocchio caller:
+tests/cpp/dropin_check builds a ModelPool from a flattened model, runs rneaInParallel,
+abaInParallel, crbaInParallel, computeRNEADerivativesInParallel and
+computeABADerivativesInParallel on column-major host matrices, and checks the reference's
+argument errors.  Outputs against the FP64 oracle: FP64 RNEA and CRBA 1e-12, FP32 ABA
+1e-4, FP32 RNEA derivatives 1e-5, FP32 ABA derivatives 1e-4 (per configuration, inf-norm
+relative; oracle on the FP32-rounded inputs for the FP32 paths).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from pinocchio_amd import models as M
+from helpers import f32_round, inputs, rel_err_cols, rel_err_mats
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "dropin_check")
+
+
+def blocks(x, nv, B):
+    """nv x (nv * B) column-major blocks -> (B, nv, nv)."""
+    return x.reshape(B, nv, nv).transpose(0, 2, 1)
+
+
+@pytest.mark.parametrize("name", ["humanoid36", "manipulator6"])
+def test_cpp_dropin(tmp_path, name):
+    assert os.path.exists(BIN), "tests/cpp/dropin_check is not built (make -C pinocchio_amd/csrc)"
+    desc = M.SAMPLE_MODELS[name]()
+    B = 300
+    q, v, a, tau = inputs(desc, B, k=60)
+    d = str(tmp_path)
+    with open(os.path.join(d, "dims.txt"), "w") as f:
+        f.write(f"{desc.njoints} {desc.nq} {desc.nv} {B}\n")
+    desc.parents.astype(np.int32).tofile(os.path.join(d, "parents.i32"))
+    desc.jtype.astype(np.int32).tofile(os.path.join(d, "jtype.i32"))
+    desc.placement.astype(np.float64).tofile(os.path.join(d, "placement.f64"))
+    desc.inertia.astype(np.float64).tofile(os.path.join(d, "inertia.f64"))
+    for n, x in (("q", q), ("v", v), ("a", a), ("tau", tau)):
+        np.asfortranarray(x).T.astype(np.float64).tofile(os.path.join(d, f"{n}.f64"))  # column-major
+    r = subprocess.run([BIN, d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "errors 1 1 1 1" in r.stdout
+    nv = desc.nv
+
+    def rd(n, shape_cols):
+        return np.fromfile(os.path.join(d, f"out_{n}.f64")).reshape(shape_cols, nv).T
+
+    tau_g = rd("tau", B)
+    assert rel_err_cols(tau_g, oracle.rnea(desc, q, v, a)).max() < 1e-12
+    qf, vf, af, tf = f32_round(q, v, a, tau)
+    assert rel_err_cols(rd("ddq", B), oracle.aba(desc, qf, vf, tf)).max() < 1e-4
+    Mg = blocks(np.fromfile(os.path.join(d, "out_M.f64")), nv, B)
+    assert rel_err_mats(Mg, oracle.crba(desc, q)).max() < 1e-12
+    rdq, rdv, _, _ = oracle.rnea_derivatives(desc, qf, vf, af)
+    for n, ref in (("dtau_dq", rdq), ("dtau_dv", rdv)):
+        assert rel_err_mats(blocks(np.fromfile(os.path.join(d, f"out_{n}.f64")), nv, B), ref).max() < 1e-5
+    rq, rv, rmi, _ = oracle.aba_derivatives(desc, qf, vf, tf)
+    for n, ref in (("ddq_dq", rq), ("ddq_dv", rv), ("minv", rmi)):
+        assert rel_err_mats(blocks(np.fromfile(os.path.join(d, f"out_{n}.f64")), nv, B), ref).max() < 1e-4
