@@ -165,6 +165,26 @@ inline void fromF32(const std::vector<float>& f, const MMat& m) {
 }
 }  // namespace detail
 
+// computeMinverse per column: M(q)^-1 as nv x nv blocks side by side (FP64 by default).
+inline void computeMinverseInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const MMat& Minv,
+                                      bool fp32 = false) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(Minv, pool.nv(), "The output matrix");
+  detail::checkCols(Minv, q.cols * pool.nv(), "The output matrix");
+  if (Minv.ld != pool.nv()) throw std::invalid_argument("The output matrix must be contiguous");
+  const int64_t nn = (int64_t)pool.nv() * pool.nv();
+  if (!fp32) {
+    check(rbd_compute_minverse(pool.ctx(), pool.model(), q.cols, RBD_F64, q.data, q.ld, Minv.data, nn,
+                               RBD_HOST_PTRS));
+    return;
+  }
+  std::vector<float> fq = detail::toF32(q), fo((size_t)(nn * q.cols));
+  check(rbd_compute_minverse(pool.ctx(), pool.model(), q.cols, RBD_F32, fq.data(), q.rows, fo.data(), nn,
+                             RBD_HOST_PTRS));
+  detail::fromF32(fo, Minv);
+}
+
 // computeRNEADerivatives per column: three nv x (nv * B) outputs (blocks side by side).
 inline void computeRNEADerivativesInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
                                              const CMat& a, const MMat& dtau_dq, const MMat& dtau_dv,

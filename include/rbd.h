@@ -15,6 +15,7 @@
  *   rbd_cholesky_solve    cholesky::solve           algorithm/cholesky.hxx:466-520
  *   rbd_rnea_derivatives  computeRNEADerivatives    algorithm/rnea-derivatives.hxx:472-541, 756-766
  *   rbd_aba_derivatives   computeABADerivatives     algorithm/aba-derivatives.hxx:380-453, 900-923
+ *   rbd_compute_minverse  computeMinverse           algorithm/aba.hxx:608-902
  *   rbd_multi_rnea_aba    (no counterpart: one launch over instances of several models)
  *   rbd_mlp_*             (no counterpart: learned forward-dynamics surrogate, checked by
  *                          the RNEA residual with exact ABA as fallback)
@@ -142,6 +143,12 @@ int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, in
 int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
                         int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq,
                         void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags);
+
+/* computeMinverse (aba.hxx:608-902; cholesky::computeMinv, cholesky.hxx:522-597): M(q)^-1
+ * per configuration, written whole (symmetric; the reference fills the upper triangle).
+ * FP32: CRBA, FP32 tree factor, unit right-hand sides; FP64: the same in double. */
+int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                         void* Minv, int64_t ldMinv, uint32_t flags);
 
 /* Mixed-robot batch (no reference counterpart: the reference batches one model per
  * call).  Each group is `count` instances of one model, with FP32 device arrays in the

@@ -277,6 +277,16 @@ class Engine:
                                          _ptr(ddq), _i64(ldd), ctypes.c_uint32(self._flags(async_))))
         return outs[0], outs[1], outs[2], ddq
 
+    def compute_minverse(self, q, out=None, async_=False):
+        """M(q)^-1 per configuration (computeMinverse), FP32 or FP64 like q."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        out = self._new_mat(q, B) if out is None else out
+        ldm = self._mat_arg(out, B, "Minv")
+        _check(lib().rbd_compute_minverse(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                          _ptr(out), _i64(ldm), ctypes.c_uint32(self._flags(async_))))
+        return out
+
     @staticmethod
     def mats_to_numpy(t, layout, nv):
         """Device matrix output -> (B, nv, nv) numpy array with [b] the configuration's matrix."""

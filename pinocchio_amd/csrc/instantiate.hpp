@@ -53,9 +53,15 @@ void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, C
   }
   hipLaunchKernelGGL((k_aba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, tau, out);
 }
+// CRBA: two waves per tile (output emission split) keep two waves resident per SIMD at
+// large batches; RBD_CRBA_WAVES=1|2 fixes the choice
 template<class TP, class S>
 void launchCrba(hipStream_t st, const S* mc, int64_t B, CView<S> q, View<S> M, int upper) {
-  hipLaunchKernelGGL((k_crba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, M, upper);
+  static const int forced = envInt("RBD_CRBA_WAVES");
+  if (forced == 1 || TP::NJ <= 3)
+    hipLaunchKernelGGL((k_crba<TP, S, 1>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, M, upper);
+  else
+    hipLaunchKernelGGL((k_crba<TP, S, 2>), dim3(gridFor(B)), dim3(kTile * 2), 0, st, mc, B, q, M, upper);
 }
 template<class TP, class S> void launchChol(hipStream_t st, int64_t B, CView<S> M, View<double> U, View<double> D) {
   hipLaunchKernelGGL((k_chol<TP, S, double>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
@@ -79,9 +85,10 @@ void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v,
     hipLaunchKernelGGL((k_rnead<TP, S, 1>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, dq, dv, M, mask);
 }
 template<class TP, class S>
-void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv) {
+void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv,
+                     int jobBegin) {
   hipLaunchKernelGGL((k_abad_solve<TP, S>), dim3(gridFor(B)), dim3(kTile * kSolveWaves), 0, st, B, U, D, Xq, Xv,
-                     Minv);
+                     Minv, jobBegin);
 }
 
 

@@ -58,15 +58,20 @@ template<class S> __device__ __forceinline__ void stMat(View<S> g, int64_t b, in
 // ---------------------------------------------------------------------------------------
 // CRBA: M(q), symmetric (or upper row blocks only), armature on the diagonal
 // ---------------------------------------------------------------------------------------
-template<class TP, class S>
-__global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_t B, CView<S> gq, View<S> gM,
-                                             int upperOnly) {
+// G > 1: G waves share the tile; all run the forward pass and the composite-inertia
+// accumulation (cheap), and joint i's entries are computed and stored by wave
+// EmitGroups<TP, G>::grp[i] only, so the store-heavy emission is spread over more waves.
+template<class TP, class S, int G = 1>
+__global__ __launch_bounds__(64 * G) void k_crba(const S* __restrict__ mc_in, int64_t B, CView<S> gq, View<S> gM,
+                                                 int upperOnly) {
   constexpr auto& T = TP::T;
   constexpr int NJ = TP::NJ, NQ = T.nq, NV = T.nv;
+  constexpr EmitGroups<TP, G> EG{};
   __shared__ S sq[NQ * kPitch];
   const int64_t b0 = int64_t(blockIdx.x) * kTile;
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  stageIn<NQ>(sq, gq, b0, nb);
+  const int w = G > 1 ? (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
+  if (w == 0) stageIn<NQ>(sq, gq, b0, nb);
   __syncthreads();
   const int L = laneId();
   const int64_t b = b0 + L;
@@ -100,6 +105,7 @@ __global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_
     } else {
       CInertia<S> Y = worldInertia(mc, i, oM[i]);
       if constexpr (TP::T.nchild[i] > 0) cadd(Y, Yacc[i]);
+      if (G == 1 || w == EG.grp[i]) {
 #pragma unroll
       for (int k = 0; k < nvi; ++k) {
         const int c = iv + k;
@@ -134,6 +140,7 @@ __global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_
           }
         });
       }
+      }
       if constexpr (p > 0) {
         if constexpr (TP::T.childIdx[i] == 0) Yacc[p] = Y;
         else cadd(Yacc[p], Y);
@@ -145,7 +152,7 @@ __global__ __launch_bounds__(64) void k_crba(const S* __restrict__ mc_in, int64_
     static_for<0, NV>([&](auto C) {
       static_for<0, NV>([&](auto R) {
         if constexpr (!CrbaPattern<TP>::nz(R, C)) {
-          if (!upperOnly || R <= C) stMat(gM, b, NV, R, C, S(0));
+          if ((G == 1 || w == (int)C % G) && (!upperOnly || R <= C)) stMat(gM, b, NV, R, C, S(0));
         }
       });
     });

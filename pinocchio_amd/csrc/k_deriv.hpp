@@ -85,39 +85,6 @@ template<class S> struct ColData {
 // ---------------------------------------------------------------------------------------
 // computeRNEADerivatives: dtau_dq, dtau_dv, M (any may be skipped) and optionally tau
 // ---------------------------------------------------------------------------------------
-// Output-emission groups for G waves sharing one tile of configurations: every wave runs
-// the forward pass and the subtree accumulations, and joint i's output entries are
-// computed and stored by wave grp[i] only.  Joints are dealt to the least-loaded wave in
-// decreasing order of their emission cost nv_i (nv_i + ancestor dofs).
-template<class TP, int G> struct EmitGroups {
-  int grp[TP::NJ];
-  constexpr EmitGroups() : grp{} {
-    constexpr auto& T = TP::T;
-    long load[8] = {};
-    bool done[TP::NJ] = {};
-    for (int it = 1; it < TP::NJ; ++it) {
-      int best = 1;
-      long bc = -1;
-      for (int j = 1; j < TP::NJ; ++j) {
-        if (done[j]) continue;
-        int anc = 0;
-        for (int a = T.parent[j]; a > 0; a = T.parent[a]) anc += T.nvj[a];
-        const long c = (long)T.nvj[j] * (anc + T.nvj[j]);
-        if (c > bc) {
-          bc = c;
-          best = j;
-        }
-      }
-      done[best] = true;
-      int g = 0;
-      for (int k = 1; k < G; ++k)
-        if (load[k] < load[g]) g = k;
-      grp[best] = g;
-      load[g] += bc;
-    }
-  }
-};
-
 template<class TP, class S, int G = 1>
 __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
                                                   CView<S> ga, View<S> gdq, View<S> gdv, View<S> gM, int writeMask) {
@@ -296,7 +263,8 @@ __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, i
 // ---------------------------------------------------------------------------------------
 // ABA-derivative solve.  Input: the packed FP32 tree factor of M (k_chol<.., float,
 // float>, batch-fastest workspace), dtau_dq in gXq and dtau_dv in gXv.  Output, in
-// place: -Minv dtau_dq, -Minv dtau_dv, and Minv in gMinv.
+// place: -Minv dtau_dq, -Minv dtau_dv, and Minv in gMinv.  Jobs (right-hand sides) before
+// jobBegin are skipped: jobBegin = 2 nv computes Minv alone (computeMinverse).
 //
 // A workgroup owns 64 consecutive configurations: their factors (nnz + nv scalars each)
 // are staged in LDS as [entry][configuration], and its waves share out the 3 nv
@@ -308,7 +276,7 @@ constexpr int kSolveWaves = 8;
 
 template<class TP, class S>
 __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CView<S> gU, CView<S> gD, View<S> gXq,
-                                                                 View<S> gXv, View<S> gMinv) {
+                                                                 View<S> gXv, View<S> gMinv, int jobBegin) {
   constexpr auto& T = TP::T;
   constexpr int NV = T.nv;
   constexpr int NNZ = nnzOf<TP>();
@@ -398,7 +366,7 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
     for (int r = 0; r < NV; ++r) stOut(&p[r * rs], sgn * x[r]);
   };
 #pragma unroll 1
-  for (int j = 2 * w; j < 3 * NV; j += 2 * kSolveWaves) {
+  for (int j = jobBegin + 2 * w; j < 3 * NV; j += 2 * kSolveWaves) {
     S x[NV], y[NV];
     load(j, x);
     load(j + 1 < 3 * NV ? j + 1 : j, y);

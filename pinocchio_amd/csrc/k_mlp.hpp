@@ -21,8 +21,9 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kBK = 32;       // k extent of one LDS stage (= one MFMA's k)
-constexpr int kLdk = kBK + 8; // LDS row pitch in bf16 (80 B: rows stay 16-B aligned)
+constexpr int kBK = 32;        // k granularity of shapes (one MFMA's k)
+constexpr int kStage = 64;     // k extent of one LDS stage (two MFMA k-steps)
+constexpr int kLdk = kStage + 8;  // LDS row pitch in bf16 (144 B: rows stay 16-B aligned)
 constexpr int kGemmThreads = 256;
 
 enum Epi : int {
@@ -37,8 +38,8 @@ struct GemmArgs {
   int64_t lda;  // A(m, k) = AK ? A[m * lda + k] : A[k * lda + m]
   const bf16* B;
   int64_t ldb;  // B(n, k) = BK ? B[n * ldb + k] : B[k * ldb + n]
-  int M, N, K;  // multiples of the tile extents and of kBK
-  int kChunk;   // k range of one blockIdx.z (split-K); multiple of kBK
+  int M, N, K;  // multiples of the tile extents; K a multiple of kStage
+  int kChunk;   // k range of one blockIdx.z (split-K); multiple of kStage
   const float* bias;
   const float* Z;  // EPI_SILU_GRAD: pre-activation, [M, ldz]
   float* Zout;     // EPI_SILU: pre-activation out (may be null)
@@ -51,31 +52,49 @@ struct GemmArgs {
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-// Stage ROWS x kBK operand values (row r, k) into s[r * kLdk + k].
+// One operand tile (ROWS x kStage values, element (r, k)) moves HBM -> registers -> LDS
+// as 16-byte chunks: along k when the operand is k-contiguous, along r otherwise (then
+// transposed on its way into LDS), so every global load is a coalesced 16-byte access.
+template<int ROWS> struct TileRegs {
+  static constexpr int kChunks = ROWS * kStage / 8 / kGemmThreads;
+  static_assert(kChunks >= 1 && ROWS * kStage / 8 % kGemmThreads == 0, "tile must divide over the workgroup");
+  uint4 c[kChunks];
+};
 template<int ROWS, bool KCONTIG>
-__device__ __forceinline__ void loadTile(bf16* __restrict__ s, const bf16* __restrict__ g, int64_t ld, int r0, int k0,
+__device__ __forceinline__ void tileLoad(TileRegs<ROWS>& t, const bf16* __restrict__ g, int64_t ld, int r0, int k0,
                                          int tid) {
-  if constexpr (KCONTIG) {
-    // 16-byte chunks along k: 4 per row
-    for (int c = tid; c < ROWS * (kBK / 8); c += kGemmThreads) {
-      const int r = c / (kBK / 8), kc = (c % (kBK / 8)) * 8;
-      *reinterpret_cast<uint4*>(s + r * kLdk + kc) =
-          *reinterpret_cast<const uint4*>(g + (int64_t)(r0 + r) * ld + k0 + kc);
-    }
-  } else {
-    // 16-byte chunks along r, transposed into LDS
-    for (int c = tid; c < kBK * (ROWS / 8); c += kGemmThreads) {
-      const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
-      const uint4 w = *reinterpret_cast<const uint4*>(g + (int64_t)(k0 + k) * ld + r0 + rc);
-      const bf16* e = reinterpret_cast<const bf16*>(&w);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) s[(rc + j) * kLdk + k] = e[j];
+  for (int j = 0; j < TileRegs<ROWS>::kChunks; ++j) {
+    const int c = tid + j * kGemmThreads;
+    if constexpr (KCONTIG) {
+      const int r = c / (kStage / 8), kc = (c % (kStage / 8)) * 8;
+      t.c[j] = *reinterpret_cast<const uint4*>(g + (int64_t)(r0 + r) * ld + k0 + kc);
+    } else {
+      const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
+      t.c[j] = *reinterpret_cast<const uint4*>(g + (int64_t)(k0 + k) * ld + r0 + rc);
+    }
+  }
+}
+template<int ROWS, bool KCONTIG>
+__device__ __forceinline__ void tileStore(const TileRegs<ROWS>& t, bf16* __restrict__ s, int tid) {
+#pragma unroll
+  for (int j = 0; j < TileRegs<ROWS>::kChunks; ++j) {
+    const int c = tid + j * kGemmThreads;
+    if constexpr (KCONTIG) {
+      const int r = c / (kStage / 8), kc = (c % (kStage / 8)) * 8;
+      *reinterpret_cast<uint4*>(s + r * kLdk + kc) = t.c[j];
+    } else {
+      const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
+      const bf16* e = reinterpret_cast<const bf16*>(&t.c[j]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s[(rc + i) * kLdk + k] = e[i];
     }
   }
 }
 
 // C tile of BM x BN per workgroup; four waves in a 2 x 2 grid, each wave BM/2 x BN/2 as
-// (BM/32) x (BN/32) MFMA blocks of 16 x 16.
+// (BM/32) x (BN/32) MFMA blocks of 16 x 16.  The next stage's operands are loaded into
+// registers while the current stage is multiplied out of LDS.
 template<int BM, int BN, bool AK, bool BK, int EPI>
 __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
@@ -94,22 +113,35 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   const int lr = lane & 15, lk = (lane >> 4) * 8;
-  for (int k0 = kBeg; k0 < kEnd; k0 += kBK) {
-    loadTile<BM, AK>(sA, g.A, g.lda, m0, k0, tid);
-    loadTile<BN, BK>(sB, g.B, g.ldb, n0, k0, tid);
+  TileRegs<BM> ra;
+  TileRegs<BN> rb;
+  tileLoad<BM, AK>(ra, g.A, g.lda, m0, kBeg, tid);
+  tileLoad<BN, BK>(rb, g.B, g.ldb, n0, kBeg, tid);
+  for (int k0 = kBeg; k0 < kEnd; k0 += kStage) {
+    tileStore<BM, AK>(ra, sA, tid);
+    tileStore<BN, BK>(rb, sB, tid);
     __syncthreads();
-    bf16x8 af[TM], bfr[TN];
+    if (k0 + kStage < kEnd) {
+      tileLoad<BM, AK>(ra, g.A, g.lda, m0, k0 + kStage, tid);
+      tileLoad<BN, BK>(rb, g.B, g.ldb, n0, k0 + kStage, tid);
+    }
 #pragma unroll
-    for (int i = 0; i < TM; ++i) af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm * WM + i * 16 + lr) * kLdk + lk);
+    for (int kk = 0; kk < kStage; kk += kBK) {
+      bf16x8 af[TM], bfr[TN];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) bfr[j] = *reinterpret_cast<const bf16x8*>(sB + (wn * WN + j * 16 + lr) * kLdk + lk);
+      for (int i = 0; i < TM; ++i)
+        af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm * WM + i * 16 + lr) * kLdk + kk + lk);
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+      for (int j = 0; j < TN; ++j)
+        bfr[j] = *reinterpret_cast<const bf16x8*>(sB + (wn * WN + j * 16 + lr) * kLdk + kk + lk);
 #pragma unroll
-      for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
     __syncthreads();
   }
-
   // lane holds C(row = 4 * (lane / 16) + e, col = lane % 16) of each 16 x 16 block
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
@@ -162,43 +194,48 @@ __device__ __forceinline__ void quatRot9(float x, float y, float z, float w, flo
   r[6] = txz - twy;       r[7] = tyz + twx;       r[8] = 1 - (txx + tyy);
 }
 
-// raw (unnormalised) feature f of configuration b; q, v, tau batch-fastest (k * ld + b)
-__device__ __forceinline__ float featureAt(const FeatMap& fm, int f, const float* q, const float* v, const float* tau,
-                                           int64_t ldq, int64_t ldv, int64_t ldt, int64_t b) {
-  if (f < fm.nrev) return sinf(q[fm.revq[f] * ldq + b]);
-  f -= fm.nrev;
-  if (f < fm.nrev) return cosf(q[fm.revq[f] * ldq + b]);
-  f -= fm.nrev;
-  if (fm.hasFF) {
-    if (f < 4) return q[(3 + f) * ldq + b];
-    f -= 4;
-    if (f < 9) {
-      float r[9];
-      quatRot9(q[3 * ldq + b], q[4 * ldq + b], q[5 * ldq + b], q[6 * ldq + b], r);
-      return r[f];
+// X[b, f] = bf16((x_f - mean_f) * invstd_f), zero for f >= nfeat.  One thread per
+// configuration (its q, v, tau reads are coalesced across the wave); a 64-configuration
+// tile of feature rows is assembled in LDS and written out as one contiguous block.
+constexpr int kMaxLdx = 256;
+__global__ __launch_bounds__(64) void k_features(FeatMap fm, int64_t B, const float* q, int64_t ldq, const float* v,
+                                                 int64_t ldv, const float* tau, int64_t ldt, const float* mean,
+                                                 const float* invstd, bf16* X, float* Xraw) {
+  __shared__ __attribute__((aligned(16))) bf16 tile[64 * kMaxLdx];
+  const int t = threadIdx.x;
+  const int64_t b0 = (int64_t)blockIdx.x * 64, b = b0 + t;
+  const bool live = b < B;
+  const int ldx = fm.ldx;
+  float r9[9] = {};
+  if (fm.hasFF && live) quatRot9(q[3 * ldq + b], q[4 * ldq + b], q[5 * ldq + b], q[6 * ldq + b], r9);
+  for (int f = 0; f < ldx; ++f) {
+    float x = 0.f;
+    if (live && f < fm.nfeat) {
+      int g = f;
+      float raw;
+      if (g < 2 * fm.nrev) {
+        const float a = q[fm.revq[g < fm.nrev ? g : g - fm.nrev] * ldq + b];
+        raw = g < fm.nrev ? sinf(a) : cosf(a);
+      } else {
+        g -= 2 * fm.nrev;
+        if (fm.hasFF && g < 13) {
+          raw = g < 4 ? q[(3 + g) * ldq + b] : r9[g - 4];
+        } else {
+          if (fm.hasFF) g -= 13;
+          raw = g < fm.nv ? v[g * ldv + b] : tau[(g - fm.nv) * ldt + b];
+        }
+      }
+      if (Xraw) Xraw[b * fm.nfeat + f] = raw;
+      x = (raw - mean[f]) * invstd[f];
     }
-    f -= 9;
+    tile[t * ldx + f] = (bf16)x;
   }
-  if (f < fm.nv) return v[f * ldv + b];
-  f -= fm.nv;
-  return tau[f * ldt + b];
-}
-
-// X[b, f] = bf16((x_f - mean_f) * invstd_f), zero for f >= nfeat (one thread per entry)
-__global__ void k_features(FeatMap fm, int64_t B, const float* q, int64_t ldq, const float* v, int64_t ldv,
-                           const float* tau, int64_t ldt, const float* mean, const float* invstd, bf16* X,
-                           float* Xraw) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t b = e / fm.ldx;
-  const int f = (int)(e - b * fm.ldx);
-  if (b >= B) return;
-  float x = 0.f;
-  if (f < fm.nfeat) {
-    const float r = featureAt(fm, f, q, v, tau, ldq, ldv, ldt, b);
-    if (Xraw) Xraw[b * fm.nfeat + f] = r;
-    x = (r - mean[f]) * invstd[f];
-  }
-  X[b * fm.ldx + f] = (bf16)x;
+  __syncthreads();
+  const int64_t rows = B - b0 < 64 ? B - b0 : 64;
+  const int chunks = (int)(rows * ldx / 8);
+  uint4* dst = reinterpret_cast<uint4*>(X + b0 * ldx);
+  const uint4* src = reinterpret_cast<const uint4*>(tile);
+  for (int c = t; c < chunks; c += 64) dst[c] = src[c];
 }
 
 // zero rows [B, Bpad) of a bf16 matrix (padding configurations of the last tile)

@@ -101,7 +101,7 @@ namespace {
 void setDev(rbd_ctx* ctx) { HIP_CHECK(hipSetDevice(ctx->device)); }
 
 template<int BM, int BN, bool AK, bool BK, int EPI> void gemm(hipStream_t st, const GemmArgs& g, int splits = 1) {
-  if (g.M % BM || g.N % BN || g.K % kBK || g.kChunk % kBK || (int64_t)g.kChunk * splits < g.K)
+  if (g.M % BM || g.N % BN || g.K % kStage || g.kChunk % kStage || (int64_t)g.kChunk * splits < g.K)
     throw Error(RBD_EINVAL, "internal: product shape does not fit its tiles");
   hipLaunchKernelGGL((k_gemm<BM, BN, AK, BK, EPI>), dim3(g.M / BM, g.N / BN, splits), dim3(kGemmThreads), 0, st, g);
 }
@@ -142,8 +142,8 @@ void checkVec(const void* p, int64_t ld, int64_t B, const char* what) {
 void features(rbd_mlp* m, hipStream_t st, int64_t B, int64_t Bp, const float* q, int64_t ldq, const float* v,
               int64_t ldv, const float* tau, int64_t ldt, float* Xraw) {
   if (B > 0)
-    hipLaunchKernelGGL(k_features, dim3(blocksFor(B * m->kin)), dim3(256), 0, st, m->fm, B, q, ldq, v, ldv, tau, ldt,
-                       m->xmean(), m->xinv(), m->X.p, Xraw);
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((B + 63) / 64)), dim3(64), 0, st, m->fm, B, q, ldq, v, ldv, tau,
+                       ldt, m->xmean(), m->xinv(), m->X.p, Xraw);
   if (Bp > B) hipLaunchKernelGGL(k_zero_rows, dim3(blocksFor((Bp - B) * m->kin)), dim3(256), 0, st, m->X.p,
                                  (int64_t)m->kin, B, Bp);
 }
@@ -174,7 +174,7 @@ void forward(rbd_mlp* m, hipStream_t st, int64_t Bp, bool keepZ) {
 
 int splitsFor(int64_t Bp) {
   int s = kMaxSplits;
-  while (s > 1 && (Bp / s) % kBK) s /= 2;
+  while (s > 1 && (Bp / s) % kStage) s /= 2;
   return s;
 }
 
@@ -286,13 +286,13 @@ int rbd_mlp_create(rbd_ctx* ctx, const rbd_model* model, int32_t hidden, int32_t
       }
     }
     fm.nfeat = 2 * fm.nrev + (fm.hasFF ? 13 : 0) + 2 * model->nv;
-    RBD_CHECK_ARG(fm.nfeat <= kMaxFeat, "too many input features");
+    RBD_CHECK_ARG(fm.nfeat <= kMaxFeat && (int)roundUp(fm.nfeat, kStage) <= kMaxLdx, "too many input features");
     m->nin = fm.nfeat;
     m->nout = model->nv;
     m->H = hidden;
     m->L = nhidden;
-    m->kin = (int)roundUp(m->nin, kBK);
-    m->kout = (int)roundUp(m->nout, kBK);
+    m->kin = (int)roundUp(m->nin, kStage);  // padded widths: whole LDS stages
+    m->kout = (int)roundUp(m->nout, kStage);
     RBD_CHECK_ARG(m->nout <= kMaxFeat, "too many outputs");
     fm.ldx = m->kin;
     for (int l = 0; l <= m->L; ++l) {

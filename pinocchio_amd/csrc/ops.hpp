@@ -25,7 +25,7 @@ using LaunchRneaD = void (*)(hipStream_t, const S* mc, int64_t B, CView<S> q, CV
 using LaunchChol32 = void (*)(hipStream_t, int64_t B, CView<float> M, View<float> U, View<float> D);
 template<class S>
 using LaunchAbaDSolve = void (*)(hipStream_t, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv,
-                                 View<S> Minv);
+                                 View<S> Minv, int jobBegin);
 
 struct TopoOps {
   const char* name;

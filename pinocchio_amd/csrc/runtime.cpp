@@ -44,6 +44,14 @@ namespace rbd_rt {
 thread_local std::string g_err;
 }  // namespace rbd_rt
 
+// batched identity matrices: element (r + c nv, b) at p[(r + c nv) rs + b bs]
+__global__ void k_identity(int nv, int64_t B, double* p, int64_t rs, int64_t bs) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = e / B, b = e - k * B;
+  if (k >= (int64_t)nv * nv) return;
+  p[k * rs + b * bs] = (k % nv == k / nv) ? 1.0 : 0.0;
+}
+
 namespace {
 
 void setDevice(rbd_ctx* ctx) { HIP_CHECK(hipSetDevice(ctx->device)); }
@@ -526,7 +534,57 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, 
     m->ops->chol32_f32(ctx->stream, B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1}, View<float>{wD, B, 1});
     m->ops->abad_solve_f32(ctx->stream, B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1},
                            View<float>{Xq, vo.rs, vo.bs}, View<float>{Xv, vo.rs, vo.bs},
-                           View<float>{Mi, vo.rs, vo.bs});
+                           View<float>{Mi, vo.rs, vo.bs}, 0);
+    finishCall(ctx, flags, st);
+  });
+}
+
+int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                         void* Minv, int64_t ldM, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && Minv), "null array argument");
+    if (!m->compact || !m->ops->crba_f32 || !m->ops->abad_solve_f32)
+      throw Error(RBD_EUNSUPPORTED, "computeMinverse: no kernels for this topology");
+    const int nv = m->nv, nnz = m->ops->nnzU;
+    if (dtype == RBD_F32) {
+      auto vq = argView<float>(m->nq, B, ldq, flags, "q");
+      auto vo = argView<float>(nv * nv, B, ldM, flags, "Minv");
+      if (B == 0) return;
+      setDevice(ctx);
+      Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+      const float* dq_ = (const float*)st.in(q, vq.extent * 4);
+      float* Mi = (float*)st.out(Minv, vo.extent * 4, false);
+      // M into the output buffer, its FP32 tree factor into the workspace, then the unit
+      // right-hand sides of the solve overwrite M with its inverse
+      const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
+      float* wU = (float*)ctx->workspace((nU + nD) * 4);
+      float* wD = wU + nU;
+      const View<float> vM{Mi, vo.rs, vo.bs};
+      m->ops->crba_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, vM, 0);
+      m->ops->chol32_f32(ctx->stream, B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1}, View<float>{wD, B, 1});
+      m->ops->abad_solve_f32(ctx->stream, B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1}, vM, vM, vM, 2 * nv);
+      finishCall(ctx, flags, st);
+      return;
+    }
+    // FP64: FP64 CRBA and tree factor, then one solve per unit column
+    auto vq = argView<double>(m->nq, B, ldq, flags, "q");
+    auto vo = argView<double>(nv * nv, B, ldM, flags, "Minv");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const double* dq_ = (const double*)st.in(q, vq.extent * 8);
+    double* Mi = (double*)st.out(Minv, vo.extent * 8, false);
+    const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
+    double* wU = (double*)ctx->workspace((nU + nD) * 8);
+    double* wD = wU + nU;
+    m->ops->crba_f64(ctx->stream, m->mc64, B, CView<double>{dq_, vq.rs, vq.bs}, View<double>{Mi, vo.rs, vo.bs}, 0);
+    m->ops->chol_f64(ctx->stream, B, CView<double>{Mi, vo.rs, vo.bs}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    hipLaunchKernelGGL(k_identity, dim3((unsigned)((nv * nv * B + 255) / 256)), dim3(256), 0, ctx->stream, nv, B, Mi,
+                       vo.rs, vo.bs);
+    for (int c = 0; c < nv; ++c)
+      m->ops->cholsolve_f64(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1},
+                            View<double>{Mi + (int64_t)c * nv * vo.rs, vo.rs, vo.bs});
     finishCall(ctx, flags, st);
   });
 }

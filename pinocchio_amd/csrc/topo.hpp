@@ -218,6 +218,39 @@ template<class TP, int G> struct WaveSplit {
   constexpr int writer(int j) const { return owner[j] < 0 ? 0 : owner[j]; }
 };
 
+// Output-emission groups for G waves sharing one tile of configurations: every wave runs
+// the forward pass and the subtree accumulations, and joint i's output entries are
+// computed and stored by wave grp[i] only.  Joints are dealt to the least-loaded wave in
+// decreasing order of their emission cost nv_i (nv_i + ancestor dofs).
+template<class TP, int G> struct EmitGroups {
+  int grp[TP::NJ];
+  constexpr EmitGroups() : grp{} {
+    constexpr auto& T = TP::T;
+    long load[8] = {};
+    bool done[TP::NJ] = {};
+    for (int it = 1; it < TP::NJ; ++it) {
+      int best = 1;
+      long bc = -1;
+      for (int j = 1; j < TP::NJ; ++j) {
+        if (done[j]) continue;
+        int anc = 0;
+        for (int a = T.parent[j]; a > 0; a = T.parent[a]) anc += T.nvj[a];
+        const long c = (long)T.nvj[j] * (anc + T.nvj[j]);
+        if (c > bc) {
+          bc = c;
+          best = j;
+        }
+      }
+      done[best] = true;
+      int g = 0;
+      for (int k = 1; k < G; ++k)
+        if (load[k] < load[g]) g = k;
+      grp[best] = g;
+      load[g] += bc;
+    }
+  }
+};
+
 // Per-joint record of model constants (device buffer, read with uniform loads).
 // Offsets in scalars; one record per joint, joint 0 included.
 enum : int {

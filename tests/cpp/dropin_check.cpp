@@ -78,6 +78,8 @@ int main(int argc, char** argv) {
     writeRaw(dir + "/out_ddq_dq.f64", d1);
     writeRaw(dir + "/out_ddq_dv.f64", d2);
     writeRaw(dir + "/out_minv.f64", d3);
+    pinocchio_amd::computeMinverseInParallel(threads, pool, Q, o1);
+    writeRaw(dir + "/out_minv64.f64", d1);
   }
   // the reference's argument errors (algorithm/parallel/rnea.hpp:53-68, aba.hpp:54-69)
   const bool e1 = throwsInvalid([&] { pinocchio_amd::rneaInParallel(threads + 1, pool, Q, V, A, MMat{tau_out.data(), nv, B, nv}); });

@@ -208,3 +208,24 @@ def test_wave_split_variants_agree(gpu_ctx, name):
             # the same arithmetic per entry, only the emitting wave differs
             a1, a4 = x1.double().numpy(), x4.double().numpy()
             assert (np.abs(a1 - a4).max(0) / np.maximum(np.abs(a4).max(0), 1e-30)).max() < 1e-6
+
+
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "manipulator6", "pendulum"])
+def test_compute_minverse(gpu_ctx, name, layout):
+    """computeMinverse against the oracle's M^-1 (FP32 1e-4, FP64 1e-10) and Minv M = I
+    (aba.cpp:265-387)."""
+    desc, model = gpu_model(gpu_ctx, name)
+    eng = pa.Engine(model, layout)
+    B = 333
+    q, v, a, tau = inputs(desc, B, k=70)
+    qf = f32_round(q)[0]
+    _, _, rmi32, _ = oracle.aba_derivatives(desc, qf, v, tau)
+    Mi32 = pa.Engine.mats_to_numpy(eng.compute_minverse(to_dev(qf, layout)), layout, desc.nv)
+    assert rel_err_mats(Mi32, rmi32).max() < 1e-4
+    _, _, rmi64, _ = oracle.aba_derivatives(desc, q, v, tau)
+    Mi64 = pa.Engine.mats_to_numpy(eng.compute_minverse(to_dev(q, layout, torch.float64)), layout, desc.nv)
+    assert rel_err_mats(Mi64, rmi64).max() < 1e-10
+    Mref = oracle.crba(desc, q)
+    eye = np.einsum("bij,bjk->bik", Mi64, Mref)
+    assert np.abs(eye - np.eye(desc.nv)).max() < 1e-9

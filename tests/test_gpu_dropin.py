@@ -1,9 +1,9 @@
 """The C++ drop-in (include/pinocchio_amd/parallel.hpp) driven like a pinocchio caller:
 tests/cpp/dropin_check builds a ModelPool from a flattened model, runs rneaInParallel,
 abaInParallel, crbaInParallel, computeRNEADerivativesInParallel and
-computeABADerivativesInParallel on column-major host matrices, and checks the reference's
+computeABADerivativesInParallel and computeMinverseInParallel on column-major host matrices, and checks the reference's
 argument errors.  Outputs against the FP64 oracle: FP64 RNEA and CRBA 1e-12, FP32 ABA
-1e-4, FP32 RNEA derivatives 1e-5, FP32 ABA derivatives 1e-4 (per configuration, inf-norm
+1e-4, FP32 RNEA derivatives 1e-5, FP32 ABA derivatives 1e-4, FP64 Minv 1e-10 (per configuration, inf-norm
 relative; oracle on the FP32-rounded inputs for the FP32 paths).
 """
 import os
@@ -62,3 +62,5 @@ def test_cpp_dropin(tmp_path, name):
     rq, rv, rmi, _ = oracle.aba_derivatives(desc, qf, vf, tf)
     for n, ref in (("ddq_dq", rq), ("ddq_dv", rv), ("minv", rmi)):
         assert rel_err_mats(blocks(np.fromfile(os.path.join(d, f"out_{n}.f64")), nv, B), ref).max() < 1e-4
+    _, _, rmi64, _ = oracle.aba_derivatives(desc, q, v, tau)
+    assert rel_err_mats(blocks(np.fromfile(os.path.join(d, "out_minv64.f64")), nv, B), rmi64).max() < 1e-10
