@@ -21,9 +21,9 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kBK = 32;        // k granularity of shapes (one MFMA's k)
-constexpr int kStage = 64;     // k extent of one LDS stage (two MFMA k-steps)
-constexpr int kLdk = kStage + 8;  // LDS row pitch in bf16 (144 B: rows stay 16-B aligned)
+constexpr int kBK = 32;       // k extent of one LDS stage (= one MFMA's k)
+constexpr int kStage = kBK;   // shapes are padded to whole stages
+constexpr int kLdk = kBK + 8; // LDS row pitch in bf16 (80 B: rows stay 16-B aligned)
 constexpr int kGemmThreads = 256;
 
 enum Epi : int {
@@ -38,8 +38,8 @@ struct GemmArgs {
   int64_t lda;  // A(m, k) = AK ? A[m * lda + k] : A[k * lda + m]
   const bf16* B;
   int64_t ldb;  // B(n, k) = BK ? B[n * ldb + k] : B[k * ldb + n]
-  int M, N, K;  // multiples of the tile extents; K a multiple of kStage
-  int kChunk;   // k range of one blockIdx.z (split-K); multiple of kStage
+  int M, N, K;  // multiples of the tile extents and of kBK
+  int kChunk;   // k range of one blockIdx.z (split-K); multiple of kBK
   const float* bias;
   const float* Z;  // EPI_SILU_GRAD: pre-activation, [M, ldz]
   float* Zout;     // EPI_SILU: pre-activation out (may be null)
@@ -52,49 +52,31 @@ struct GemmArgs {
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-// One operand tile (ROWS x kStage values, element (r, k)) moves HBM -> registers -> LDS
-// as 16-byte chunks: along k when the operand is k-contiguous, along r otherwise (then
-// transposed on its way into LDS), so every global load is a coalesced 16-byte access.
-template<int ROWS> struct TileRegs {
-  static constexpr int kChunks = ROWS * kStage / 8 / kGemmThreads;
-  static_assert(kChunks >= 1 && ROWS * kStage / 8 % kGemmThreads == 0, "tile must divide over the workgroup");
-  uint4 c[kChunks];
-};
+// Stage ROWS x kBK operand values (row r, k) into s[r * kLdk + k].
 template<int ROWS, bool KCONTIG>
-__device__ __forceinline__ void tileLoad(TileRegs<ROWS>& t, const bf16* __restrict__ g, int64_t ld, int r0, int k0,
+__device__ __forceinline__ void loadTile(bf16* __restrict__ s, const bf16* __restrict__ g, int64_t ld, int r0, int k0,
                                          int tid) {
-#pragma unroll
-  for (int j = 0; j < TileRegs<ROWS>::kChunks; ++j) {
-    const int c = tid + j * kGemmThreads;
-    if constexpr (KCONTIG) {
-      const int r = c / (kStage / 8), kc = (c % (kStage / 8)) * 8;
-      t.c[j] = *reinterpret_cast<const uint4*>(g + (int64_t)(r0 + r) * ld + k0 + kc);
-    } else {
-      const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
-      t.c[j] = *reinterpret_cast<const uint4*>(g + (int64_t)(k0 + k) * ld + r0 + rc);
+  if constexpr (KCONTIG) {
+    // 16-byte chunks along k: 4 per row
+    for (int c = tid; c < ROWS * (kBK / 8); c += kGemmThreads) {
+      const int r = c / (kBK / 8), kc = (c % (kBK / 8)) * 8;
+      *reinterpret_cast<uint4*>(s + r * kLdk + kc) =
+          *reinterpret_cast<const uint4*>(g + (int64_t)(r0 + r) * ld + k0 + kc);
     }
-  }
-}
-template<int ROWS, bool KCONTIG>
-__device__ __forceinline__ void tileStore(const TileRegs<ROWS>& t, bf16* __restrict__ s, int tid) {
-#pragma unroll
-  for (int j = 0; j < TileRegs<ROWS>::kChunks; ++j) {
-    const int c = tid + j * kGemmThreads;
-    if constexpr (KCONTIG) {
-      const int r = c / (kStage / 8), kc = (c % (kStage / 8)) * 8;
-      *reinterpret_cast<uint4*>(s + r * kLdk + kc) = t.c[j];
-    } else {
+  } else {
+    // 16-byte chunks along r, transposed into LDS
+    for (int c = tid; c < kBK * (ROWS / 8); c += kGemmThreads) {
       const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
-      const bf16* e = reinterpret_cast<const bf16*>(&t.c[j]);
+      const uint4 w = *reinterpret_cast<const uint4*>(g + (int64_t)(k0 + k) * ld + r0 + rc);
+      const bf16* e = reinterpret_cast<const bf16*>(&w);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) s[(rc + i) * kLdk + k] = e[i];
+      for (int j = 0; j < 8; ++j) s[(rc + j) * kLdk + k] = e[j];
     }
   }
 }
 
 // C tile of BM x BN per workgroup; four waves in a 2 x 2 grid, each wave BM/2 x BN/2 as
-// (BM/32) x (BN/32) MFMA blocks of 16 x 16.  The next stage's operands are loaded into
-// registers while the current stage is multiplied out of LDS.
+// (BM/32) x (BN/32) MFMA blocks of 16 x 16.
 template<int BM, int BN, bool AK, bool BK, int EPI>
 __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
@@ -113,35 +95,22 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   const int lr = lane & 15, lk = (lane >> 4) * 8;
-  TileRegs<BM> ra;
-  TileRegs<BN> rb;
-  tileLoad<BM, AK>(ra, g.A, g.lda, m0, kBeg, tid);
-  tileLoad<BN, BK>(rb, g.B, g.ldb, n0, kBeg, tid);
-  for (int k0 = kBeg; k0 < kEnd; k0 += kStage) {
-    tileStore<BM, AK>(ra, sA, tid);
-    tileStore<BN, BK>(rb, sB, tid);
+  for (int k0 = kBeg; k0 < kEnd; k0 += kBK) {
+    loadTile<BM, AK>(sA, g.A, g.lda, m0, k0, tid);
+    loadTile<BN, BK>(sB, g.B, g.ldb, n0, k0, tid);
     __syncthreads();
-    if (k0 + kStage < kEnd) {
-      tileLoad<BM, AK>(ra, g.A, g.lda, m0, k0 + kStage, tid);
-      tileLoad<BN, BK>(rb, g.B, g.ldb, n0, k0 + kStage, tid);
-    }
+    bf16x8 af[TM], bfr[TN];
 #pragma unroll
-    for (int kk = 0; kk < kStage; kk += kBK) {
-      bf16x8 af[TM], bfr[TN];
+    for (int i = 0; i < TM; ++i) af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm * WM + i * 16 + lr) * kLdk + lk);
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
-        af[i] = *reinterpret_cast<const bf16x8*>(sA + (wm * WM + i * 16 + lr) * kLdk + kk + lk);
+    for (int j = 0; j < TN; ++j) bfr[j] = *reinterpret_cast<const bf16x8*>(sB + (wn * WN + j * 16 + lr) * kLdk + lk);
 #pragma unroll
-      for (int j = 0; j < TN; ++j)
-        bfr[j] = *reinterpret_cast<const bf16x8*>(sB + (wn * WN + j * 16 + lr) * kLdk + kk + lk);
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-    }
+      for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
+
   // lane holds C(row = 4 * (lane / 16) + e, col = lane % 16) of each 16 x 16 block
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
@@ -201,41 +170,55 @@ constexpr int kMaxLdx = 256;
 __global__ __launch_bounds__(64) void k_features(FeatMap fm, int64_t B, const float* q, int64_t ldq, const float* v,
                                                  int64_t ldv, const float* tau, int64_t ldt, const float* mean,
                                                  const float* invstd, bf16* X, float* Xraw) {
-  __shared__ __attribute__((aligned(16))) bf16 tile[64 * kMaxLdx];
+  // row pitch ldx + 2: an odd number of 4-byte words, so the per-thread row writes of a
+  // wave fall in distinct LDS banks
+  __shared__ __attribute__((aligned(16))) bf16 tile[64 * (kMaxLdx + 2)];
   const int t = threadIdx.x;
   const int64_t b0 = (int64_t)blockIdx.x * 64, b = b0 + t;
   const bool live = b < B;
-  const int ldx = fm.ldx;
-  float r9[9] = {};
-  if (fm.hasFF && live) quatRot9(q[3 * ldq + b], q[4 * ldq + b], q[5 * ldq + b], q[6 * ldq + b], r9);
-  for (int f = 0; f < ldx; ++f) {
-    float x = 0.f;
-    if (live && f < fm.nfeat) {
-      int g = f;
-      float raw;
-      if (g < 2 * fm.nrev) {
-        const float a = q[fm.revq[g < fm.nrev ? g : g - fm.nrev] * ldq + b];
-        raw = g < fm.nrev ? sinf(a) : cosf(a);
-      } else {
-        g -= 2 * fm.nrev;
-        if (fm.hasFF && g < 13) {
-          raw = g < 4 ? q[(3 + g) * ldq + b] : r9[g - 4];
-        } else {
-          if (fm.hasFF) g -= 13;
-          raw = g < fm.nv ? v[g * ldv + b] : tau[(g - fm.nv) * ldt + b];
-        }
-      }
-      if (Xraw) Xraw[b * fm.nfeat + f] = raw;
-      x = (raw - mean[f]) * invstd[f];
+  const int ldx = fm.ldx, pitch = ldx + 2;
+  bf16* row = tile + t * pitch;
+  auto put = [&](int f, float raw) {
+    if (Xraw) Xraw[b * fm.nfeat + f] = raw;
+    row[f] = (bf16)((raw - mean[f]) * invstd[f]);
+  };
+  if (live) {
+    int f = 0;
+    for (int r = 0; r < fm.nrev; ++r) {
+      float sn, cs;
+      sincosf(q[fm.revq[r] * ldq + b], &sn, &cs);
+      put(r, sn);
+      put(fm.nrev + r, cs);
     }
-    tile[t * ldx + f] = (bf16)x;
+    f = 2 * fm.nrev;
+    if (fm.hasFF) {
+      const float qx = q[3 * ldq + b], qy = q[4 * ldq + b], qz = q[5 * ldq + b], qw = q[6 * ldq + b];
+      float r9[9];
+      quatRot9(qx, qy, qz, qw, r9);
+      put(f, qx);
+      put(f + 1, qy);
+      put(f + 2, qz);
+      put(f + 3, qw);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) put(f + 4 + k, r9[k]);
+      f += 13;
+    }
+    for (int k = 0; k < fm.nv; ++k) put(f + k, v[k * ldv + b]);
+    f += fm.nv;
+    for (int k = 0; k < fm.nv; ++k) put(f + k, tau[k * ldt + b]);
+    f += fm.nv;
+    for (; f < ldx; ++f) row[f] = (bf16)0.f;
   }
   __syncthreads();
+  // the tile's rows are consecutive in X: copy it out as 4-byte pairs, coalesced
   const int64_t rows = B - b0 < 64 ? B - b0 : 64;
-  const int chunks = (int)(rows * ldx / 8);
-  uint4* dst = reinterpret_cast<uint4*>(X + b0 * ldx);
-  const uint4* src = reinterpret_cast<const uint4*>(tile);
-  for (int c = t; c < chunks; c += 64) dst[c] = src[c];
+  const int half = ldx / 2;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(X + b0 * ldx);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(tile);
+  for (int c = t; c < rows * half; c += 64) {
+    const int r = c / half, k = c - r * half;
+    dst[c] = src[r * (half + 1) + k];
+  }
 }
 
 // zero rows [B, Bpad) of a bf16 matrix (padding configurations of the last tile)
