@@ -325,6 +325,17 @@ def main():
     t = time_call(lambda: em.rnea(Qm, Vm, Am, async_=True), reps=50)
     others["rnea_manipulator6_f64"] = {"batch": 4096, "configs_per_s": 4096 / t, "us": t * 1e6}
 
+    # drop-in host path: numpy matrices in and out (PCIe both ways), not the device-resident figure
+    pool = pa.ModelPool(desc, size=8)
+    qh, vh, th = (np.asfortranarray(x) for x in (q, v, tau))
+    pa.abaInParallel(8, pool, qh, vh, th)
+    t0_ = time.perf_counter()
+    for _ in range(3):
+        pa.abaInParallel(8, pool, qh, vh, th)
+    t = (time.perf_counter() - t0_) / 3
+    others["aba_humanoid36_host_arrays"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3,
+                                            "note": "abaInParallel on float64 numpy arrays: conversion, PCIe both ways"}
+
     # mixed-robot batch: RNEA + ABA per instance, one graph-replayed launch
     mix_spec = [("arm7", 40000), ("quadruped18", 40000), ("humanoid36", 15000), ("random60", 5000)]
     groups = []
