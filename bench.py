@@ -325,6 +325,16 @@ def main():
     t = time_call(lambda: em.rnea(Qm, Vm, Am, async_=True), reps=50)
     others["rnea_manipulator6_f64"] = {"batch": 4096, "configs_per_s": 4096 / t, "us": t * 1e6}
 
+    # forward kinematics (placements) and WORLD joint Jacobians, FP32, HBM-bound outputs
+    t = time_call(lambda: eng.forward_kinematics(Q, async_=True), reps=10)
+    nbytes = 4 * (desc.nq + 12 * (desc.njoints - 1))
+    others["forward_kinematics_humanoid36_f32"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3,
+                                                   "GB_s": nbytes * B / t / 1e9}
+    t = time_call(lambda: eng.joint_jacobians(Q, async_=True), reps=10)
+    nbytes = 4 * (desc.nq + 6 * nv)
+    others["joint_jacobians_humanoid36_f32"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3,
+                                                "GB_s": nbytes * B / t / 1e9}
+
     # drop-in host path: numpy matrices in and out (PCIe both ways), not the device-resident figure
     pool = pa.ModelPool(desc, size=8)
     qh, vh, th = (np.asfortranarray(x) for x in (q, v, tau))

@@ -16,6 +16,8 @@
  *   rbd_rnea_derivatives  computeRNEADerivatives    algorithm/rnea-derivatives.hxx:472-541, 756-766
  *   rbd_aba_derivatives   computeABADerivatives     algorithm/aba-derivatives.hxx:380-453, 900-923
  *   rbd_compute_minverse  computeMinverse           algorithm/aba.hxx:608-902
+ *   rbd_forward_kinematics forwardKinematics        algorithm/kinematics.hxx:121-145, 209-238
+ *   rbd_joint_jacobians   computeJointJacobians     algorithm/jacobian.hxx:23-62
  *   rbd_multi_rnea_aba    (no counterpart: one launch over instances of several models)
  *   rbd_mlp_*             (no counterpart: learned forward-dynamics surrogate, checked by
  *                          the RNEA residual with exact ABA as fallback)
@@ -143,6 +145,19 @@ int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, in
 int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
                         int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq,
                         void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags);
+
+/* forwardKinematics (kinematics.hxx:121-145, 209-238): per configuration the world
+ * placement oMi of joints 1 .. njoints-1 (12 entries per joint: R row major, then the
+ * translation; element index 12 (i - 1) + k), and, when v (and a) are given, each joint's
+ * spatial velocity (and acceleration) in its local frame (6 entries per joint, [linear;
+ * angular], element index 6 (i - 1) + k).  vout / aout / v / a may be NULL. */
+int rbd_forward_kinematics(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                           const void* v, int64_t ldv, const void* a, int64_t lda, void* oMi, int64_t ldoMi,
+                           void* vout, void* aout, int64_t ldva, uint32_t flags);
+/* computeJointJacobians (jacobian.hxx:23-62): the 6 x nv WORLD-frame Jacobian per
+ * configuration, column of dof c = oMi.act(S_c); element index r + 6 c. */
+int rbd_joint_jacobians(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                        void* J, int64_t ldJ, uint32_t flags);
 
 /* computeMinverse (aba.hxx:608-902; cholesky::computeMinv, cholesky.hxx:522-597): M(q)^-1
  * per configuration, written whole (symmetric; the reference fills the upper triangle).

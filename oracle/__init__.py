@@ -85,6 +85,33 @@ def rnea(model, q, v, a, scalar="f64", nthreads=0):
     return np.ascontiguousarray(tau)
 
 
+def forward_kinematics(model, q, v=None, a=None, scalar="f64", nthreads=0):
+    """(oMi, v, a): oMi (B, nj-1, 12) = world placement of joints 1.. (R row major, t);
+    v, a (B, nj-1, 6) local spatial velocity / acceleration (None unless v / v and a given)."""
+    q = _cols(q, model.nq)
+    B = q.shape[1]
+    nj = model.njoints
+    vv = _cols(v, model.nv) if v is not None else None
+    aa = _cols(a, model.nv) if a is not None else None
+    oM = np.zeros((B, nj - 1, 12))
+    vo = np.zeros((B, nj - 1, 6)) if v is not None else None
+    ao = np.zeros((B, nj - 1, 6)) if (v is not None and a is not None) else None
+    d = model.cstruct()
+    _check(lib().orc_forward_kinematics(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(vv), _p(aa), _p(oM), _p(vo),
+                                        _p(ao), SCALARS[scalar], nthreads))
+    return oM, vo, ao
+
+
+def joint_jacobians(model, q, scalar="f64", nthreads=0):
+    """(B, 6, nv) world-frame joint Jacobians (computeJointJacobians)."""
+    q = _cols(q, model.nq)
+    B = q.shape[1]
+    J = np.zeros((B, model.nv, 6))
+    d = model.cstruct()
+    _check(lib().orc_joint_jacobians(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(J), SCALARS[scalar], nthreads))
+    return J.transpose(0, 2, 1).copy()
+
+
 def aba(model, q, v, tau, convention="world", scalar="f64", nthreads=0):
     q, v, tau = _cols(q, model.nq), _cols(v, model.nv), _cols(tau, model.nv)
     B = q.shape[1]

@@ -154,6 +154,40 @@ int abaDerivT(const ModelDesc* d, int64_t B, const double* q, const double* v, c
     if (ddq) store(data.ddq.data(), ddq + b * nv, nv);
   });
 }
+// forward kinematics: per configuration oMi of joints 1..nj-1 as 12 numbers each (R row
+// major, then translation), and (optional) local velocities / accelerations, 6 per joint
+template<class S>
+int fkT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a, double* oM, double* vo,
+        double* ao, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv, ba;
+    const int nj = m.njoints;
+    const S* vv = v ? bv.load(v + b * m.nv, m.nv) : nullptr;
+    const S* aa = a ? ba.load(a + b * m.nv, m.nv) : nullptr;
+    forwardKinematics(m, data, bq.load(q + b * m.nq, m.nq), vv, aa);
+    for (int i = 1; i < nj; ++i) {
+      double* o = oM + (b * (nj - 1) + (i - 1)) * 12;
+      for (int k = 0; k < 9; ++k) o[k] = double(data.oMi[i].R.m[k]);
+      for (int k = 0; k < 3; ++k) o[9 + k] = double(data.oMi[i].p[k]);
+      S tmp[6];
+      if (vo) {
+        toVec(data.v[i], tmp);
+        store(tmp, vo + (b * (nj - 1) + (i - 1)) * 6, 6);
+      }
+      if (ao) {
+        toVec(data.a[i], tmp);
+        store(tmp, ao + (b * (nj - 1) + (i - 1)) * 6, 6);
+      }
+    }
+  });
+}
+template<class S> int jacT(const ModelDesc* d, int64_t B, const double* q, double* J, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq;
+    computeJointJacobians(m, data, bq.load(q + b * m.nq, m.nq));
+    storeMat(data.J, J + b * 6 * m.nv);
+  });
+}
 }  // namespace
 
 extern "C" {
@@ -197,6 +231,31 @@ int orc_topology(const ModelDesc* d, int* idx_q, int* idx_v, int* nv_subtree, in
 int orc_rnea(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a, double* tau, int scalar,
              int nthreads) {
 #define CALL(S) rneaT<S>(d, B, q, v, a, tau, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_forward_kinematics(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a,
+                           double* oM, double* vout, double* aout, int scalar, int nthreads) {
+#define CALL(S) fkT<S>(d, B, q, v, a, oM, vout, aout, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_joint_jacobians(const ModelDesc* d, int64_t B, const double* q, double* J, int scalar, int nthreads) {
+#define CALL(S) jacT<S>(d, B, q, J, nthreads)
   switch (scalar) {
     case 0: return CALL(double);
     case 1: return CALL(float);

@@ -11,6 +11,8 @@
 //   joints                   multibody/joint/joint-free-flyer.hpp:333-391,
 //                            multibody/joint/joint-revolute.hpp:793-836
 //   topology helpers         multibody/data.hxx:196-315
+//   forward kinematics       algorithm/kinematics.hxx:121-145, 209-238
+//   joint Jacobians          algorithm/jacobian.hxx:23-62
 //   rnea                     algorithm/rnea.hxx:17-161
 //   aba (WORLD, LOCAL)       algorithm/aba.hxx:79-293, 354-540
 //   crba (WORLD)             algorithm/crba.hxx:20-99, 498-548
@@ -580,6 +582,43 @@ template<class S> inline void spdInverse(const Blk36<S>& A, int n, Blk36<S>& Ain
       for (int k = i + 1; k < n; ++k) t -= L[(size_t)k * n + i] * Ainv[(size_t)k * n + c];
       Ainv[(size_t)i * n + c] = t / L[(size_t)i * n + i];
     }
+  }
+}
+
+// ----------------------------------------------------------------------------------------
+// forwardKinematics, kinematics.hxx:121-145 (first order) and 209-238 (second order):
+// oMi = oMi[parent] liMi, v_i = vJ + liMi^-1 v_parent, a_i = S qdd + c + v_i x vJ
+// + liMi^-1 a_parent (c = 0 for the joints restated here).  v and a may be null.
+// ----------------------------------------------------------------------------------------
+template<class S>
+inline void forwardKinematics(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* a) {
+  data.v[0] = zeroMotion<S>();
+  data.a[0] = zeroMotion<S>();
+  for (int i = 1; i < model.njoints; ++i) {
+    const int p = model.parents[i];
+    const JointCalc<S> jc = jointCalc(model, i, q, v);
+    data.liMi[i] = model.placements[i] * jc.M;
+    data.oMi[i] = p > 0 ? data.oMi[p] * data.liMi[i] : data.liMi[i];
+    if (v) {
+      data.v[i] = jc.v;
+      if (p > 0) data.v[i] = data.v[i] + actInv(data.liMi[i], data.v[p]);
+    }
+    if (v && a) {
+      Motion<S> ai = crossMM(data.v[i], jc.v);
+      for (int k = 0; k < jc.nv; ++k) ai = ai + a[model.idx_v[i] + k] * jc.Scol[k];
+      if (p > 0) ai = ai + actInv(data.liMi[i], data.a[p]);
+      data.a[i] = ai;
+    }
+  }
+}
+
+// computeJointJacobians, jacobian.hxx:23-62: column of dof k of joint i = oMi.act(S_k)
+// (WORLD frame); the 6 x nv result is data.J.
+template<class S> inline void computeJointJacobians(const Model<S>& model, Data<S>& data, const S* q) {
+  forwardKinematics(model, data, q, static_cast<const S*>(nullptr), static_cast<const S*>(nullptr));
+  for (int i = 1; i < model.njoints; ++i) {
+    const JointCalc<S> jc = jointCalc(model, i, q, static_cast<const S*>(nullptr));
+    for (int k = 0; k < jc.nv; ++k) setCol(data.J, model.idx_v[i] + k, act(data.oMi[i], jc.Scol[k]));
   }
 }
 

@@ -277,6 +277,38 @@ class Engine:
                                          _ptr(ddq), _i64(ldd), ctypes.c_uint32(self._flags(async_))))
         return outs[0], outs[1], outs[2], ddq
 
+    def forward_kinematics(self, q, v=None, a=None, async_=False):
+        """(oMi, v_local, a_local): oMi rows 12 per joint (R row major, then t) for joints
+        1.., velocities / accelerations 6 per joint in the joint frames (None unless given)."""
+        import torch
+        m = self.model
+        nj = m.njoints
+        B, ldq = self._vec(q, m.nq, "q")
+        oM = self._new_vec(q, 12 * (nj - 1), B)
+        _, ldo = self._vec(oM, 12 * (nj - 1), "oMi")
+        vo = self._new_vec(q, 6 * (nj - 1), B) if v is not None else None
+        ao = self._new_vec(q, 6 * (nj - 1), B) if (v is not None and a is not None) else None
+        ldv = self._vec(v, m.nv, "v")[1] if v is not None else 0
+        lda = self._vec(a, m.nv, "a")[1] if a is not None else 0
+        ldva = self._vec(vo, 6 * (nj - 1), "v out")[1] if vo is not None else 0
+        _check(lib().rbd_forward_kinematics(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                            _ptr(v) if v is not None else None, _i64(ldv),
+                                            _ptr(a) if ao is not None else None, _i64(lda), _ptr(oM), _i64(ldo),
+                                            _ptr(vo) if vo is not None else None,
+                                            _ptr(ao) if ao is not None else None, _i64(ldva),
+                                            ctypes.c_uint32(self._flags(async_))))
+        return oM, vo, ao
+
+    def joint_jacobians(self, q, async_=False):
+        """WORLD-frame joint Jacobians, 6 x nv per configuration (element r + 6 c)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        J = self._new_vec(q, 6 * m.nv, B)
+        _, ldj = self._vec(J, 6 * m.nv, "J")
+        _check(lib().rbd_joint_jacobians(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(J),
+                                         _i64(ldj), ctypes.c_uint32(self._flags(async_))))
+        return J
+
     def compute_minverse(self, q, out=None, async_=False):
         """M(q)^-1 per configuration (computeMinverse), FP32 or FP64 like q."""
         m = self.model

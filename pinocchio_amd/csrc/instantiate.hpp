@@ -7,6 +7,7 @@
 #include "k_dyn.hpp"
 #include "k_crba.hpp"
 #include "k_deriv.hpp"
+#include "k_kin.hpp"
 #include "_gen/topologies.inc"
 
 namespace rbd {
@@ -92,6 +93,12 @@ void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> 
 }
 
 
+template<class TP, class S>
+void launchKin(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> oM, View<S> vo,
+               View<S> ao, View<S> J) {
+  hipLaunchKernelGGL((k_kin<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, oM, vo, ao, J);
+}
+
 template<class TP> TopoOps makeOpsBase() {
   TopoOps o{};
   o.name = TP::name;
@@ -126,6 +133,10 @@ template<class TP> void add_chol(TopoOps& o) {
 }
 template<class TP> void add_rnead(TopoOps& o) { o.rnead_f32 = &launchRneaD<TP, float>; }
 template<class TP> void add_abads(TopoOps& o) { o.abad_solve_f32 = &launchAbaDSolve<TP, float>; }
+template<class TP> void add_kin(TopoOps& o) {
+  o.kin_f32 = &launchKin<TP, float>;
+  o.kin_f64 = &launchKin<TP, double>;
+}
 
 }  // namespace rbd
 

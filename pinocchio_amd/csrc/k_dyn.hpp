@@ -247,15 +247,15 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
         const Xform<S> X = freeflyerX(mc, i, q7, false);
         const Mot<S> ap = actInv(X, a0);  // c_root = 0, chat_root = 0
         const Frc<S> Iap = symMul6(Ia, ap);
-        double A[6][6], rhs[6], x[6];
+        double A[21], x[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
 #pragma unroll
-          for (int c = 0; c < 6; ++c) A[r][c] = double(sget(Ia, r, c));
-          A[r][r] += double(mc[mcArmature<NJ>() + iv + r]);
-          rhs[r] = double(st[(iv + r) * kPitch + L]) - double(comp(pA, r)) - double(comp(Iap, r));
+          for (int c = 0; c <= r; ++c) A[lowIdx(r, c)] = double(sget(Ia, r, c));
+          A[lowIdx(r, r)] += double(mc[mcArmature<NJ>() + iv + r]);
+          x[r] = double(st[(iv + r) * kPitch + L]) - double(comp(pA, r)) - double(comp(Iap, r));
         }
-        spdSolve6(A, rhs, x);
+        spdSolve6(A, x);
         if constexpr (WS.writer(i) == W) {
 #pragma unroll
           for (int r = 0; r < 6; ++r) so[(iv + r) * kPitch + L] = S(x[r]);

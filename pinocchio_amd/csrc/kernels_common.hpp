@@ -211,40 +211,42 @@ template<class S> __device__ __forceinline__ Mot<S> axisMot(int ax, S w) {
   return m;
 }
 
-// 6x6 SPD inverse in double through an LLT (PerformStYSInversion,
-// joint-common-operations.hpp:22-45, for the free-flyer D = S^T Ia S block).
-__device__ __forceinline__ void spdSolve6(const double (&A)[6][6], const double (&b)[6], double (&x)[6]) {
-  double L[6][6];
+// 6x6 SPD solve in double through an LLT (PerformStYSInversion,
+// joint-common-operations.hpp:22-45, for the free-flyer D = S^T Ia S block), in place on
+// the packed lower triangle a[r (r + 1) / 2 + c] (r >= c): a becomes L, b becomes the
+// solution.  Packed and in place so that the factor costs 21 doubles of registers at the
+// point of the sweep where every joint's backward-pass state is still live.
+__device__ __forceinline__ constexpr int lowIdx(int r, int c) { return r * (r + 1) / 2 + c; }
+__device__ __forceinline__ void spdSolve6(double (&a)[21], double (&b)[6]) {
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    double s = A[j][j];
+    double s = a[lowIdx(j, j)];
 #pragma unroll
-    for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+    for (int k = 0; k < j; ++k) s -= a[lowIdx(j, k)] * a[lowIdx(j, k)];
     const double ljj = sqrt(s);
     const double inv = 1.0 / ljj;
-    L[j][j] = ljj;
+    a[lowIdx(j, j)] = ljj;
 #pragma unroll
     for (int i = j + 1; i < 6; ++i) {
-      double t = A[i][j];
+      double t = a[lowIdx(i, j)];
 #pragma unroll
-      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
-      L[i][j] = t * inv;
+      for (int k = 0; k < j; ++k) t -= a[lowIdx(i, k)] * a[lowIdx(j, k)];
+      a[lowIdx(i, j)] = t * inv;
     }
   }
-  double y[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     double t = b[i];
 #pragma unroll
-    for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
-    y[i] = t / L[i][i];
+    for (int k = 0; k < i; ++k) t -= a[lowIdx(i, k)] * b[k];
+    b[i] = t / a[lowIdx(i, i)];
   }
 #pragma unroll
   for (int i = 5; i >= 0; --i) {
-    double t = y[i];
+    double t = b[i];
 #pragma unroll
-    for (int k = i + 1; k < 6; ++k) t -= L[k][i] * x[k];
-    x[i] = t / L[i][i];
+    for (int k = i + 1; k < 6; ++k) t -= a[lowIdx(k, i)] * b[k];
+    b[i] = t / a[lowIdx(i, i)];
   }
 }
 

@@ -27,6 +27,12 @@ template<class S>
 using LaunchAbaDSolve = void (*)(hipStream_t, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv,
                                  View<S> Minv, int jobBegin);
 
+// forward kinematics / joint Jacobians: outputs placements, local velocities and
+// accelerations, WORLD Jacobian columns (any may be null)
+template<class S>
+using LaunchKin = void (*)(hipStream_t, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> oM,
+                           View<S> vo, View<S> ao, View<S> J);
+
 struct TopoOps {
   const char* name;
   int njoints;
@@ -46,6 +52,8 @@ struct TopoOps {
   LaunchRneaD<float> rnead_f32;
   LaunchChol32 chol32_f32;
   LaunchAbaDSolve<float> abad_solve_f32;
+  LaunchKin<float> kin_f32;
+  LaunchKin<double> kin_f64;
   int nnzU;  // strictly-upper structural non-zeros of M
 };
 

@@ -384,6 +384,73 @@ int rbd_crba(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void*
   });
 }
 
+}  // extern "C"
+
+namespace {
+// forward kinematics / joint Jacobians through one launcher (k_kin)
+template<class S>
+void kinCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
+             const void* a, int64_t lda, void* oM, int64_t ldo, void* vo, void* ao, int64_t ldva, void* J,
+             int64_t ldJ, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const auto launch = f32 ? (void*)m->ops->kin_f32 : (void*)m->ops->kin_f64;
+  if (!launch) throw Error(RBD_EUNSUPPORTED, "kinematics: no kernel for this topology");
+  const int nj = m->njoints, nv = m->nv;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  ArgView<S> vv{}, va{}, vM{}, vva{}, vJ{};
+  if (v) vv = argView<S>(nv, B, ldv, flags, "v");
+  if (a) va = argView<S>(nv, B, lda, flags, "a");
+  if (oM) vM = argView<S>(12 * (nj - 1), B, ldo, flags, "oMi");
+  if (vo || ao) vva = argView<S>(6 * (nj - 1), B, ldva, flags, "v/a out");
+  if (J) vJ = argView<S>(6 * nv, B, ldJ, flags, "J");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* da = (const S*)st.in(a, va.extent * sizeof(S));
+  S* dM = (S*)st.out(oM, vM.extent * sizeof(S), false);
+  S* dvo = (S*)st.out(vo, vva.extent * sizeof(S), false);
+  S* dao = (S*)st.out(ao, vva.extent * sizeof(S), false);
+  S* dJ = (S*)st.out(J, vJ.extent * sizeof(S), false);
+  const S* mc;
+  if constexpr (f32) mc = m->mc32;
+  else mc = m->mc64;
+  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, mc, B, CView<S>{dq, vq.rs, vq.bs}, CView<S>{dv, vv.rs, vv.bs},
+                                        CView<S>{da, va.rs, va.bs}, View<S>{dM, vM.rs, vM.bs},
+                                        View<S>{dvo, vva.rs, vva.bs}, View<S>{dao, vva.rs, vva.bs},
+                                        View<S>{dJ, vJ.rs, vJ.bs});
+  finishCall(ctx, flags, st);
+}
+}  // namespace
+
+extern "C" {
+
+int rbd_forward_kinematics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                           const void* v, int64_t ldv, const void* a, int64_t lda, void* oMi, int64_t ldo, void* vout,
+                           void* aout, int64_t ldva, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && oMi), "null array argument");
+    RBD_CHECK_ARG(!aout || (v && a && vout), "accelerations need v, a and the velocity output");
+    RBD_CHECK_ARG(!vout || v, "velocities need v");
+    if (dtype == RBD_F32) kinCall<float>(ctx, m, B, q, ldq, v, ldv, a, lda, oMi, ldo, vout, aout, ldva, nullptr, 0, flags);
+    else kinCall<double>(ctx, m, B, q, ldq, v, ldv, a, lda, oMi, ldo, vout, aout, ldva, nullptr, 0, flags);
+  });
+}
+
+int rbd_joint_jacobians(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, void* J,
+                        int64_t ldJ, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && J), "null array argument");
+    if (dtype == RBD_F32)
+      kinCall<float>(ctx, m, B, q, ldq, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr, 0, J, ldJ, flags);
+    else
+      kinCall<double>(ctx, m, B, q, ldq, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr, 0, J, ldJ, flags);
+  });
+}
+
 int rbd_cholesky_decompose(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* M, int64_t ldM,
                            double* U, int64_t ldU, double* D, int64_t ldD, uint32_t flags) {
   return guarded([&] {

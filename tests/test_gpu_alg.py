@@ -229,3 +229,28 @@ def test_compute_minverse(gpu_ctx, name, layout):
     Mref = oracle.crba(desc, q)
     eye = np.einsum("bij,bjk->bik", Mi64, Mref)
     assert np.abs(eye - np.eye(desc.nv)).max() < 1e-9
+
+
+def test_urdf_simple_humanoid_all_algorithms(gpu_ctx):
+    """pinocchio's sample robot (tests/golden/simple_humanoid.urdf) read with a free-flyer
+    root runs on its prebuilt topology; every algorithm against the oracle."""
+    import os
+    from pinocchio_amd import urdf
+    desc = urdf.parse(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "simple_humanoid.urdf"),
+                      root="freeflyer")
+    model = pa.Model(gpu_ctx, desc)
+    eng = pa.Engine(model, "batch")
+    B = 500
+    q, v, a, tau = f32_round(*inputs(desc, B, k=80))
+    Q, V, A, T = (to_dev(x, "batch") for x in (q, v, a, tau))
+    from helpers import rel_err_cols
+    assert rel_err_cols(to_host(eng.rnea(Q, V, A), "batch"), oracle.rnea(desc, q, v, a)).max() < 1e-5
+    assert rel_err_cols(to_host(eng.aba(Q, V, T), "batch"), oracle.aba(desc, q, v, tau)).max() < 1e-4
+    nv = desc.nv
+    assert rel_err_mats(pa.Engine.mats_to_numpy(eng.crba(Q), "batch", nv), oracle.crba(desc, q)).max() < 1e-5
+    rdq, rdv, _, _ = oracle.rnea_derivatives(desc, q, v, a)
+    for x, r in zip(eng.rnea_derivatives(Q, V, A)[:2], (rdq, rdv)):
+        assert rel_err_mats(pa.Engine.mats_to_numpy(x, "batch", nv), r).max() < 1e-5
+    rq, rv, rmi, _ = oracle.aba_derivatives(desc, q, v, tau)
+    for x, r in zip(eng.aba_derivatives(Q, V, T)[:3], (rq, rv, rmi)):
+        assert rel_err_mats(pa.Engine.mats_to_numpy(x, "batch", nv), r).max() < 1e-4

@@ -260,3 +260,79 @@ def test_argument_errors():
     with pytest.raises(RuntimeError, match="compact"):
         r = mk("random60")
         oracle.crba(r, inp(r, 1)[0])
+
+
+# ---- forward kinematics and joint Jacobians (kinematics.hxx, jacobian.hxx) -------------
+def _support(desc):
+    """dof sets of each joint's ancestors-or-self."""
+    iv, nvs = desc.idx_v, desc.nvs
+    sup = {}
+    for i in range(1, desc.njoints):
+        s, j = set(), i
+        while j > 0:
+            s |= set(range(iv[j], iv[j] + nvs[j]))
+            j = desc.parents[j]
+        sup[i] = s
+    return sup
+
+
+def test_fk_pendulum_closed_form():
+    d = M.pendulum(axis="Y")
+    q = np.array([[0.3, -1.2]])
+    oM, _, _ = oracle.forward_kinematics(d, q)
+    for b, th in enumerate(q[0]):
+        c, s = np.cos(th), np.sin(th)
+        Ry = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+        assert np.allclose(oM[b, 0, :9], Ry.ravel()) and np.allclose(oM[b, 0, 9:], 0)
+
+
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "random60"])
+def test_jacobian_is_fk_derivative(name):
+    """Column c of the WORLD Jacobian is the motion of every frame supported by dof c,
+    measured by central differences of the placements along q (+) eps e_c; frames not
+    supported by c do not move (jacobian.hxx:23-62 against kinematics.hxx:52-71)."""
+    d = M.SAMPLE_MODELS[name]()
+    q, _, _, _ = M.sample_inputs(d, 1, M.INPUT_SEED_BASE + 90)
+    J = oracle.joint_jacobians(d, q)[0]
+    sup = _support(d)
+    eps = 1e-6
+    worst = 0.0
+    for c in range(d.nv):
+        e = np.zeros((d.nv, 1))
+        e[c] = eps
+        Mp, _, _ = oracle.forward_kinematics(d, oracle.integrate(d, q, e))
+        Mm, _, _ = oracle.forward_kinematics(d, oracle.integrate(d, q, -e))
+        M0, _, _ = oracle.forward_kinematics(d, q)
+        for i in range(1, d.njoints):
+            R0, p0 = M0[0, i - 1, :9].reshape(3, 3), M0[0, i - 1, 9:]
+            dR = (Mp[0, i - 1, :9] - Mm[0, i - 1, :9]).reshape(3, 3) / (2 * eps)
+            dp = (Mp[0, i - 1, 9:] - Mm[0, i - 1, 9:]) / (2 * eps)
+            W = dR @ R0.T
+            w = np.array([W[2, 1], W[0, 2], W[1, 0]])
+            fd = np.concatenate([dp - np.cross(w, p0), w])
+            want = J[:, c] if c in sup[i] else np.zeros(6)
+            worst = max(worst, np.abs(fd - want).max())
+    assert worst < 1e-7
+
+
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18"])
+def test_fk_velocity_and_acceleration(name):
+    """FK velocities: oMi.act(v_i) = sum over the support of J[:, c] v_c.  FK accelerations:
+    a_i = d/dt v_i along q(t) = q (+) (t v + t^2 a / 2), v(t) = v + t a (central differences)."""
+    d = M.SAMPLE_MODELS[name]()
+    q, v, a, _ = M.sample_inputs(d, 1, M.INPUT_SEED_BASE + 91)
+    oM, vo, ao = oracle.forward_kinematics(d, q, v, a)
+    J = oracle.joint_jacobians(d, q)[0]
+    sup = _support(d)
+    for i in range(1, d.njoints):
+        R, p = oM[0, i - 1, :9].reshape(3, 3), oM[0, i - 1, 9:]
+        vl, va = vo[0, i - 1, :3], vo[0, i - 1, 3:]
+        ang = R @ va
+        lin = R @ vl + np.cross(p, ang)
+        cols = sorted(sup[i])
+        want = J[:, cols] @ v[cols, 0]
+        assert np.allclose(np.concatenate([lin, ang]), want, atol=1e-10)
+    h = 1e-5
+    _, vp, _ = oracle.forward_kinematics(d, oracle.integrate(d, q, h * v + 0.5 * h * h * a), v + h * a)
+    _, vm, _ = oracle.forward_kinematics(d, oracle.integrate(d, q, -h * v + 0.5 * h * h * a), v - h * a)
+    assert np.abs((vp - vm) / (2 * h) - ao).max() < 1e-6
