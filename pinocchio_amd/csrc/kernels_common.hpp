@@ -33,9 +33,12 @@ __device__ __forceinline__ int laneId() { return threadIdx.x & 63; }
 // Scheduling fence between per-joint steps of an unrolled sweep: keeps the machine
 // scheduler from hoisting the next joint's constant loads across the whole tree, which
 // would stretch live ranges far beyond the open path.
+#ifndef RBD_FENCE_MASK
+#define RBD_FENCE_MASK 0  // instruction classes allowed across (sched_barrier mask); 0: none
+#endif
 __device__ __forceinline__ void stepFence() {
 #ifndef RBD_NO_FENCE
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(RBD_FENCE_MASK);
 #endif
 }
 
@@ -80,19 +83,48 @@ template<class S> __device__ __forceinline__ S* laneLaunder(S* p) {
 }
 
 // Copy rows [0, R) of the tile starting at configuration b0 into lds[k * kPitch + b].
-// Lanes past the batch end get zeros (keeps every lane on the fast math paths).
+// Lanes past the batch end get zeros (keeps every lane on the fast math paths).  kStageU
+// loads per lane are issued before the first is waited on (the addresses of lanes past
+// the batch end are clamped to the last configuration, so no load needs a branch): with
+// one wave per SIMD, a load-wait-store loop would expose the full memory latency once per
+// row.
+constexpr int kStageU = 8;
 template<int R, class S>
 __device__ __forceinline__ void stageIn(S* __restrict__ lds, CView<S> g, int64_t b0, int nb) {
   const int t = laneId();
   if (g.rs == 1) {  // configuration-contiguous (column major, Eigen default)
-    for (int e = t; e < R * kTile; e += kTile) {
-      const int b = e / R, k = e - b * R;
-      lds[k * kPitch + b] = b < nb ? g.p[(b0 + b) * g.bs + k] : S(0);
+#pragma unroll
+    for (int e0 = 0; e0 < R * kTile; e0 += kStageU * kTile) {
+      S x[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int e = e0 + u * kTile + t;
+        const int ec = e < R * kTile ? e : R * kTile - 1;
+        const int b = ec / R, k = ec - b * R;
+        const int bc = b < nb ? b : nb - 1;
+        x[u] = g.p[(b0 + bc) * g.bs + k];
+        if (b >= nb) x[u] = S(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int e = e0 + u * kTile + t;
+        if (e < R * kTile) {
+          const int b = e / R, k = e - b * R;
+          lds[k * kPitch + b] = x[u];
+        }
+      }
     }
-  } else {  // batch-fastest or general strides
-    for (int e = t; e < R * kTile; e += kTile) {
-      const int k = e >> 6, b = e & 63;
-      lds[k * kPitch + b] = b < nb ? g.p[k * g.rs + (b0 + b) * g.bs] : S(0);
+  } else {  // batch-fastest or general strides: row k of the tile is one strided segment
+    const int bc = t < nb ? t : nb - 1;
+#pragma unroll
+    for (int k0 = 0; k0 < R; k0 += kStageU) {
+      S x[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u)
+        if (k0 + u < R) x[u] = t < nb ? g.p[(k0 + u) * g.rs + (b0 + bc) * g.bs] : S(0);
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u)
+        if (k0 + u < R) lds[(k0 + u) * kPitch + t] = x[u];
     }
   }
 }
@@ -100,14 +132,33 @@ template<int R, class S>
 __device__ __forceinline__ void stageOut(const S* __restrict__ lds, View<S> g, int64_t b0, int nb) {
   const int t = laneId();
   if (g.rs == 1) {
-    for (int e = t; e < R * kTile; e += kTile) {
-      const int b = e / R, k = e - b * R;
-      if (b < nb) g.p[(b0 + b) * g.bs + k] = lds[k * kPitch + b];
+#pragma unroll
+    for (int e0 = 0; e0 < R * kTile; e0 += kStageU * kTile) {
+      S x[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int e = e0 + u * kTile + t;
+        const int ec = e < R * kTile ? e : R * kTile - 1;
+        const int b = ec / R, k = ec - b * R;
+        x[u] = lds[k * kPitch + b];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const int e = e0 + u * kTile + t;
+        const int b = e / R, k = e - b * R;
+        if (e < R * kTile && b < nb) g.p[(b0 + b) * g.bs + k] = x[u];
+      }
     }
   } else {
-    for (int e = t; e < R * kTile; e += kTile) {
-      const int k = e >> 6, b = e & 63;
-      if (b < nb) g.p[k * g.rs + (b0 + b) * g.bs] = lds[k * kPitch + b];
+#pragma unroll
+    for (int k0 = 0; k0 < R; k0 += kStageU) {
+      S x[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u)
+        if (k0 + u < R) x[u] = lds[(k0 + u) * kPitch + t];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u)
+        if (k0 + u < R && t < nb) g.p[(k0 + u) * g.rs + (b0 + t) * g.bs] = x[u];
     }
   }
 }
