@@ -38,7 +38,9 @@ template<class S> __device__ __forceinline__ Mot<S> mcGravityNeg(const S* __rest
 template<class TP> constexpr int dynLdsRows() { return TP::T.nq + 2 * TP::T.nv; }
 
 // One 64-configuration tile starting at b0; `lds` holds dynLdsRows<TP>() * kPitch scalars.
-template<class TP, class S>
+// (kU: loads in flight per lane while staging; the fused mixed-batch kernel, already at
+// the register limit, stages one row at a time)
+template<class TP, class S, int kU = kStageU>
 __device__ __forceinline__ void rneaTile(const S* __restrict__ mc_in, int64_t B, int64_t b0, CView<S> gq,
                                          CView<S> gv, CView<S> ga, View<S> gtau, S* __restrict__ lds) {
   using C = DynCfg<TP, S>;
@@ -48,9 +50,9 @@ __device__ __forceinline__ void rneaTile(const S* __restrict__ mc_in, int64_t B,
   S* sv = lds + NQ * kPitch;
   S* sa = lds + (NQ + NV) * kPitch;  // a in, tau out
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  stageIn<NQ>(sq, gq, b0, nb);
-  stageIn<NV>(sv, gv, b0, nb);
-  stageIn<NV>(sa, ga, b0, nb);
+  stageIn<NQ, S, kU>(sq, gq, b0, nb);
+  stageIn<NV, S, kU>(sv, gv, b0, nb);
+  stageIn<NV, S, kU>(sa, ga, b0, nb);
   __syncthreads();
   const int L = laneId();
   const Mot<S> a0 = mcGravityNeg(mc_in, mcGravity<NJ, NV>());
@@ -127,7 +129,7 @@ __device__ __forceinline__ void rneaTile(const S* __restrict__ mc_in, int64_t B,
     }
   });
   __syncthreads();
-  stageOut<NV>(sa, gtau, b0, nb);
+  stageOut<NV, S, kU>(sa, gtau, b0, nb);
 }
 
 template<class TP, class S>
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc, int64_t B
 constexpr int kAbaSlot = 27;  // floats per slot: 21 (articulated inertia) + 6 (bias force)
 template<class TP> constexpr int abaLdsRows(int G) { return TP::T.nq + (G > 1 ? 3 : 2) * TP::T.nv; }
 
-template<class TP, class S, int G = 1, int W = 0>
+template<class TP, class S, int G = 1, int W = 0, int kU = kStageU>
 __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, int64_t b0, CView<S> gq,
                                         CView<S> gv, CView<S> gt, View<S> gddq, S* __restrict__ lds,
                                         S* __restrict__ xs = nullptr) {
@@ -164,9 +166,9 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
   S* so = G > 1 ? lds + (NQ + 2 * NV) * kPitch : st;  // ddq out
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
   if (W == 0) {
-    stageIn<NQ>(sq, gq, b0, nb);
-    stageIn<NV>(sv, gv, b0, nb);
-    stageIn<NV>(st, gt, b0, nb);
+    stageIn<NQ, S, kU>(sq, gq, b0, nb);
+    stageIn<NV, S, kU>(sv, gv, b0, nb);
+    stageIn<NV, S, kU>(st, gt, b0, nb);
   }
   __syncthreads();
   const int L = laneId();
@@ -336,7 +338,7 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
     }
   });
   __syncthreads();
-  if (W == 0) stageOut<NV>(so, gddq, b0, nb);
+  if (W == 0) stageOut<NV, S, kU>(so, gddq, b0, nb);
 }
 
 template<class TP, class S>
@@ -408,10 +410,10 @@ __global__ __launch_bounds__(64) void k_multi(const MultiGroup* __restrict__ gro
 #pragma unroll 1
     for (int c = 0; c < t.nchunks; ++c) {
       const int64_t b0 = int64_t(t.chunk0 + c) * kTile;
-      rneaTile<TP, float>(g.mc, n, b0, CView<float>{g.q, n, 1}, CView<float>{g.v, n, 1}, CView<float>{g.a, n, 1},
+      rneaTile<TP, float, 1>(g.mc, n, b0, CView<float>{g.q, n, 1}, CView<float>{g.v, n, 1}, CView<float>{g.a, n, 1},
                           View<float>{g.tau_out, n, 1}, lds);
       __syncthreads();
-      abaTile<TP, float>(g.mc, n, b0, CView<float>{g.q, n, 1}, CView<float>{g.v, n, 1}, CView<float>{g.tau, n, 1},
+      abaTile<TP, float, 1, 0, 1>(g.mc, n, b0, CView<float>{g.q, n, 1}, CView<float>{g.v, n, 1}, CView<float>{g.tau, n, 1},
                          View<float>{g.ddq_out, n, 1}, lds);
       __syncthreads();
     }

@@ -89,77 +89,108 @@ template<class S> __device__ __forceinline__ S* laneLaunder(S* p) {
 // one wave per SIMD, a load-wait-store loop would expose the full memory latency once per
 // row.
 constexpr int kStageU = 8;
-template<int R, class S>
+// (the per-batch loops below use static_for, so the kStageU values always live in
+// registers: a plain loop over a local array may be left rolled in very large kernels,
+// which puts the array on the stack)
+template<int R, class S, int kU = kStageU>
 __device__ __forceinline__ void stageIn(S* __restrict__ lds, CView<S> g, int64_t b0, int nb) {
   const int t = laneId();
+  if constexpr (kU == 1) {  // compact loop: one row in flight
+    if (g.rs == 1) {
+      for (int e = t; e < R * kTile; e += kTile) {
+        const int b = e / R, k = e - b * R;
+        lds[k * kPitch + b] = b < nb ? g.p[(b0 + b) * g.bs + k] : S(0);
+      }
+    } else {
+      for (int e = t; e < R * kTile; e += kTile) {
+        const int k = e >> 6, b = e & 63;
+        lds[k * kPitch + b] = b < nb ? g.p[k * g.rs + (b0 + b) * g.bs] : S(0);
+      }
+    }
+    return;
+  }
   if (g.rs == 1) {  // configuration-contiguous (column major, Eigen default)
-#pragma unroll
-    for (int e0 = 0; e0 < R * kTile; e0 += kStageU * kTile) {
-      S x[kStageU];
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const int e = e0 + u * kTile + t;
+    static_for<0, (R + kU - 1) / kU>([&](auto I) {
+      constexpr int e0 = int(I) * kU * kTile;
+      S x[kU];
+      static_for<0, kU>([&](auto U) {
+        const int e = e0 + int(U) * kTile + t;
         const int ec = e < R * kTile ? e : R * kTile - 1;
         const int b = ec / R, k = ec - b * R;
         const int bc = b < nb ? b : nb - 1;
-        x[u] = g.p[(b0 + bc) * g.bs + k];
-        if (b >= nb) x[u] = S(0);
-      }
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const int e = e0 + u * kTile + t;
+        const S v = g.p[(b0 + bc) * g.bs + k];
+        x[U] = b < nb ? v : S(0);
+      });
+      static_for<0, kU>([&](auto U) {
+        const int e = e0 + int(U) * kTile + t;
         if (e < R * kTile) {
           const int b = e / R, k = e - b * R;
-          lds[k * kPitch + b] = x[u];
+          lds[k * kPitch + b] = x[U];
         }
-      }
-    }
+      });
+    });
   } else {  // batch-fastest or general strides: row k of the tile is one strided segment
     const int bc = t < nb ? t : nb - 1;
-#pragma unroll
-    for (int k0 = 0; k0 < R; k0 += kStageU) {
-      S x[kStageU];
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u)
-        if (k0 + u < R) x[u] = t < nb ? g.p[(k0 + u) * g.rs + (b0 + bc) * g.bs] : S(0);
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u)
-        if (k0 + u < R) lds[(k0 + u) * kPitch + t] = x[u];
-    }
+    static_for<0, (R + kU - 1) / kU>([&](auto I) {
+      constexpr int k0 = int(I) * kU;
+      S x[kU];
+      static_for<0, kU>([&](auto U) {
+        if constexpr (k0 + int(U) < R) {
+          const S v = g.p[(k0 + int(U)) * g.rs + (b0 + bc) * g.bs];
+          x[U] = t < nb ? v : S(0);
+        }
+      });
+      static_for<0, kU>([&](auto U) {
+        if constexpr (k0 + int(U) < R) lds[(k0 + int(U)) * kPitch + t] = x[U];
+      });
+    });
   }
 }
-template<int R, class S>
+template<int R, class S, int kU = kStageU>
 __device__ __forceinline__ void stageOut(const S* __restrict__ lds, View<S> g, int64_t b0, int nb) {
   const int t = laneId();
+  if constexpr (kU == 1) {  // compact loop: one row in flight
+    if (g.rs == 1) {
+      for (int e = t; e < R * kTile; e += kTile) {
+        const int b = e / R, k = e - b * R;
+        if (b < nb) g.p[(b0 + b) * g.bs + k] = lds[k * kPitch + b];
+      }
+    } else {
+      for (int e = t; e < R * kTile; e += kTile) {
+        const int k = e >> 6, b = e & 63;
+        if (b < nb) g.p[k * g.rs + (b0 + b) * g.bs] = lds[k * kPitch + b];
+      }
+    }
+    return;
+  }
   if (g.rs == 1) {
-#pragma unroll
-    for (int e0 = 0; e0 < R * kTile; e0 += kStageU * kTile) {
+    static_for<0, (R + kStageU - 1) / kStageU>([&](auto I) {
+      constexpr int e0 = int(I) * kStageU * kTile;
       S x[kStageU];
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const int e = e0 + u * kTile + t;
+      static_for<0, kStageU>([&](auto U) {
+        const int e = e0 + int(U) * kTile + t;
         const int ec = e < R * kTile ? e : R * kTile - 1;
         const int b = ec / R, k = ec - b * R;
-        x[u] = lds[k * kPitch + b];
-      }
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const int e = e0 + u * kTile + t;
+        x[U] = lds[k * kPitch + b];
+      });
+      static_for<0, kStageU>([&](auto U) {
+        const int e = e0 + int(U) * kTile + t;
         const int b = e / R, k = e - b * R;
-        if (e < R * kTile && b < nb) g.p[(b0 + b) * g.bs + k] = x[u];
-      }
-    }
+        if (e < R * kTile && b < nb) g.p[(b0 + b) * g.bs + k] = x[U];
+      });
+    });
   } else {
-#pragma unroll
-    for (int k0 = 0; k0 < R; k0 += kStageU) {
+    static_for<0, (R + kStageU - 1) / kStageU>([&](auto I) {
+      constexpr int k0 = int(I) * kStageU;
       S x[kStageU];
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u)
-        if (k0 + u < R) x[u] = lds[(k0 + u) * kPitch + t];
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u)
-        if (k0 + u < R && t < nb) g.p[(k0 + u) * g.rs + (b0 + t) * g.bs] = x[u];
-    }
+      static_for<0, kStageU>([&](auto U) {
+        if constexpr (k0 + int(U) < R) x[U] = lds[(k0 + int(U)) * kPitch + t];
+      });
+      static_for<0, kStageU>([&](auto U) {
+        if constexpr (k0 + int(U) < R)
+          if (t < nb) g.p[(k0 + int(U)) * g.rs + (b0 + t) * g.bs] = x[U];
+      });
+    });
   }
 }
 

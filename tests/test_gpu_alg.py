@@ -254,3 +254,28 @@ def test_urdf_simple_humanoid_all_algorithms(gpu_ctx):
     rq, rv, rmi, _ = oracle.aba_derivatives(desc, q, v, tau)
     for x, r in zip(eng.aba_derivatives(Q, V, T)[:3], (rq, rv, rmi)):
         assert rel_err_mats(pa.Engine.mats_to_numpy(x, "batch", nv), r).max() < 1e-4
+
+
+def test_largest_batches_use_64bit_offsets(gpu_ctx):
+    """A batch whose matrix output exceeds 4 GiB (CRBA, 1,000,003 humanoid configurations,
+    5.2 GB of M) and a million-configuration ABA: sampled configurations -- first, last,
+    across the 2^31-byte boundary, random -- against the oracle."""
+    desc, model = gpu_model(gpu_ctx, "humanoid36")
+    eng = pa.Engine(model, "batch")
+    B = 1_000_003
+    q, v, a, tau = f32_round(*M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 33))
+    Q, V, T = (to_dev(x, "batch") for x in (q, v, tau))
+    rng = np.random.default_rng(5)
+    nv = desc.nv
+    cross = (2 ** 31) // (4 * B) + 1  # first matrix row whose byte offset exceeds 2^31
+    cols = np.unique(np.concatenate([[0, 1, 63, 64, B - 65, B - 2, B - 1], rng.integers(0, B, 200)]))
+    Mg = eng.crba(Q)
+    assert Mg.shape == (nv * nv, B) and cross < nv * nv
+    sub = Mg[:, torch.tensor(cols, device="cuda")].double().cpu().numpy()
+    ref = oracle.crba(desc, q[:, cols])
+    assert rel_err_mats(pa.Engine.mats_to_numpy(torch.tensor(sub), "batch", nv), ref).max() < 1e-5
+    del Mg
+    torch.cuda.empty_cache()
+    dd = eng.aba(Q, V, T)[:, torch.tensor(cols, device="cuda")].double().cpu().numpy()
+    from helpers import rel_err_cols
+    assert rel_err_cols(dd, oracle.aba(desc, q[:, cols], v[:, cols], tau[:, cols])).max() < 1e-4
