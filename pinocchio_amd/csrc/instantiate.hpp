@@ -35,6 +35,16 @@ template<class TP, int G> constexpr bool abaSplitPays() {
   for (int k = 0; k < G; ++k) busiest = WS.load[k] > busiest ? WS.load[k] : busiest;
   return 5 * (trunk + busiest) <= 3 * (TP::NJ - 1) && abaWavesLdsBytes<TP, G>() <= 160 * 1024;
 }
+// RNEA derivatives: the same test, with the LDS of the staged inputs and the 43-float
+// subtree slots
+template<class TP, int G> constexpr bool rneadSplitPays() {
+  constexpr WaveSplit<TP, G> WS{};
+  int trunk = 0, busiest = 0;
+  for (int j = 1; j < TP::NJ; ++j) trunk += WS.owner[j] < 0 ? 1 : 0;
+  for (int k = 0; k < G; ++k) busiest = WS.load[k] > busiest ? WS.load[k] : busiest;
+  constexpr int lds = ((TP::T.nq + 2 * TP::T.nv) * kPitch + WS.nslots * kRneadSlot * kTile) * 4;
+  return 5 * (trunk + busiest) <= 3 * (TP::NJ - 1) && lds <= 160 * 1024;
+}
 // RBD_ABA_WAVES=1|2|4 fixes the waves per tile (default: chosen from the batch size)
 template<class TP, class S>
 void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
@@ -80,10 +90,14 @@ template<class TP, class S>
 void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
                  View<S> M, int mask, int waves) {
   if (waves <= 0) waves = wavesForBatch(B);
-  if (waves >= 2)
-    hipLaunchKernelGGL((k_rnead<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, a, dq, dv, M, mask);
-  else
-    hipLaunchKernelGGL((k_rnead<TP, S, 1>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, dq, dv, M, mask);
+  if constexpr (rneadSplitPays<TP, 4>()) {
+    if (waves >= 2) {
+      hipLaunchKernelGGL((k_rnead<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, a, dq, dv, M,
+                         mask);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_rnead<TP, S, 1>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, dq, dv, M, mask);
 }
 template<class TP, class S>
 void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv,

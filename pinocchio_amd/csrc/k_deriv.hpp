@@ -85,20 +85,22 @@ template<class S> struct ColData {
 // ---------------------------------------------------------------------------------------
 // computeRNEADerivatives: dtau_dq, dtau_dv, M (any may be skipped) and optionally tau
 // ---------------------------------------------------------------------------------------
-template<class TP, class S, int G = 1>
-__global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
-                                                  CView<S> ga, View<S> gdq, View<S> gdv, View<S> gM, int writeMask) {
-  static_assert(G >= 1 && G <= 8, "1 to 8 waves per tile");
+// G > 1 (small batches): G waves share the tile and split the tree by subtrees
+// (WaveSplit, as in ABA): a wave runs the events of its subtrees and of the replicated
+// trunk, emits the entries of the joints it owns (wave 0 those of the trunk), and hands
+// a subtree's composite inertia, its variation and its force (43 floats) to the trunk
+// parent through an LDS slot read after a barrier.  This is the body of wave W.
+constexpr int kRneadSlot = 43;
+template<class TP, class S, int G, int W>
+__device__ __forceinline__ void rneadTile(const S* __restrict__ mc_in, int64_t B, int64_t b0, CView<S> gq,
+                                          CView<S> gv, CView<S> ga, View<S> gdq, View<S> gdv, View<S> gM,
+                                          int writeMask, S* __restrict__ sq, S* __restrict__ sv, S* __restrict__ sa,
+                                          S* __restrict__ xs) {
   constexpr auto& T = TP::T;
   constexpr int NJ = TP::NJ, NQ = T.nq, NV = T.nv;
-  constexpr EmitGroups<TP, G> EG{};
-  __shared__ S sq[NQ * kPitch];
-  __shared__ S sv[NV * kPitch];
-  __shared__ S sa[NV * kPitch];
-  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  constexpr WaveSplit<TP, G> WS{};
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  const int w = G > 1 ? (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
-  if (w == 0) {
+  if (W == 0) {
     stageIn<NQ>(sq, gq, b0, nb);
     stageIn<NV>(sv, gv, b0, nb);
     stageIn<NV>(sa, ga, b0, nb);
@@ -135,9 +137,10 @@ __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, i
 
   static_for<0, T.nev>([&](auto E) {
     constexpr int e = E;
+    constexpr int i = TP::T.evJoint[e];
+    if constexpr (WS.runs(i, W)) {
     stepFence();
     const S* __restrict__ mc = stepConsts(mc_in);
-    constexpr int i = TP::T.evJoint[e];
     constexpr int p = TP::T.parent[i];
     constexpr int ty = TP::T.type[i];
     constexpr int iq = TP::T.iq[i], iv = TP::T.iv[i], nvi = TP::T.nvj[i];
@@ -183,12 +186,37 @@ __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, i
       const Frc<S> oh = cmul(Y, ov[i]);
       Frc<S> of = cmul(Y, oa[i]) + crossMF(ov[i], oh);
       DY<S> DYc = inertiaVariation(Y, ov[i], oh);
-      if constexpr (TP::T.nchild[i] > 0) {
+      if constexpr (WS.nReg[i] > 0) {
         cadd(Y, Yacc[i]);
         dyAdd(DYc, DYacc[i]);
         of = of + Facc[i];
       }
-      if (G == 1 || w == EG.grp[i]) {
+      if constexpr (G > 1 && WS.ldsIn[i]) {
+        // subtrees of other waves (fixed order, so every wave sums identically)
+        __syncthreads();
+        static_for<1, NJ>([&](auto Cc) {
+          constexpr int c = Cc;
+          if constexpr (TP::T.parent[c] == i && WS.slot[c] >= 0) {
+            const S* x = xs + WS.slot[c] * kRneadSlot * kTile + L;
+            CInertia<S> Yc;
+            DY<S> Dc;
+            Frc<S> Fc;
+            S* yc = reinterpret_cast<S*>(&Yc);
+            S* dc = reinterpret_cast<S*>(&Dc);
+            S* fc = reinterpret_cast<S*>(&Fc);
+#pragma unroll
+            for (int k = 0; k < 10; ++k) yc[k] = x[k * kTile];
+#pragma unroll
+            for (int k = 0; k < 27; ++k) dc[k] = x[(10 + k) * kTile];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) fc[k] = x[(37 + k) * kTile];
+            cadd(Y, Yc);
+            dyAdd(DYc, Dc);
+            of = of + Fc;
+          }
+        });
+      }
+      if constexpr (WS.writer(i) == W) {
 #pragma unroll
       for (int k = 0; k < nvi; ++k) {
         const int c = iv + k;
@@ -235,7 +263,18 @@ __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, i
       }
       }
       if constexpr (p > 0) {
-        if constexpr (TP::T.childIdx[i] == 0) {
+        if constexpr (G > 1 && WS.slot[i] >= 0) {
+          S* x = xs + WS.slot[i] * kRneadSlot * kTile + L;
+          const S* yc = reinterpret_cast<const S*>(&Y);
+          const S* dc = reinterpret_cast<const S*>(&DYc);
+          const S* fc = reinterpret_cast<const S*>(&of);
+#pragma unroll
+          for (int k = 0; k < 10; ++k) x[k * kTile] = yc[k];
+#pragma unroll
+          for (int k = 0; k < 27; ++k) x[(10 + k) * kTile] = dc[k];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) x[(37 + k) * kTile] = fc[k];
+        } else if constexpr (WS.regIdx[i] == 0) {
           Yacc[p] = Y;
           DYacc[p] = DYc;
           Facc[p] = of;
@@ -246,16 +285,39 @@ __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, i
         }
       }
     }
+    }
   });
   if (live) {
     static_for<0, NV>([&](auto C) {
       static_for<0, NV>([&](auto R) {
-        if (constexpr int c = C; !CrbaPattern<TP>::nz(R, C) && (G == 1 || w == c % G)) {
+        if constexpr (!CrbaPattern<TP>::nz(R, C) && int(C) % G == W) {
           if (wq) stMat(gdq, b, NV, R, C, S(0));
           if (wv) stMat(gdv, b, NV, R, C, S(0));
           if (wM) stMat(gM, b, NV, R, C, S(0));
         }
       });
+    });
+  }
+}
+
+template<class TP, class S, int G = 1>
+__global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, int64_t B, CView<S> gq, CView<S> gv,
+                                                  CView<S> ga, View<S> gdq, View<S> gdv, View<S> gM, int writeMask) {
+  static_assert(G >= 1 && G <= 8, "1 to 8 waves per tile");
+  constexpr int NQ = TP::T.nq, NV = TP::T.nv;
+  constexpr WaveSplit<TP, G> WS{};
+  __shared__ S sq[NQ * kPitch];
+  __shared__ S sv[NV * kPitch];
+  __shared__ S sa[NV * kPitch];
+  __shared__ S xs[G > 1 && WS.nslots > 0 ? WS.nslots * kRneadSlot * kTile : 1];
+  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  if constexpr (G == 1) {
+    rneadTile<TP, S, 1, 0>(mc_in, B, b0, gq, gv, ga, gdq, gdv, gM, writeMask, sq, sv, sa, xs);
+  } else {
+    const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    static_for<0, G>([&](auto Wc) {
+      constexpr int W = Wc;
+      if (w == W) rneadTile<TP, S, G, W>(mc_in, B, b0, gq, gv, ga, gdq, gdv, gM, writeMask, sq, sv, sa, xs);
     });
   }
 }
@@ -272,7 +334,7 @@ __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, i
 // every column load and store is one coalesced 256-byte row in the batch-fastest
 // layout and every factor read is a conflict-free LDS row.
 // ---------------------------------------------------------------------------------------
-constexpr int kSolveWaves = 8;
+constexpr int kSolveWaves = 12;  // 168 VGPRs: three waves per SIMD at one workgroup per CU
 
 template<class TP, class S>
 __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CView<S> gU, CView<S> gD, View<S> gXq,
@@ -286,11 +348,27 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
   const int L = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  for (int e = threadIdx.x; e < (NNZ + NV) * kTile; e += 64 * kSolveWaves) {
-    const int k = e >> 6, b = e & 63;
-    S x = S(1);
-    if (b < nb) x = k < NNZ ? gU.p[(int64_t)k * gU.rs + (b0 + b) * gU.bs] : S(1) / gD.p[(int64_t)(k - NNZ) * gD.rs + (b0 + b) * gD.bs];
-    sU[e] = x;
+  // staged kStageU entries per thread at a time, all loads issued before the first is
+  // used (lanes past the batch end read the last configuration's entry and store 1)
+  {
+    constexpr int kTot = (NNZ + NV) * kTile, kStep = 64 * kSolveWaves;
+    const int b = threadIdx.x & 63;
+    const int bc = b < nb ? b : nb - 1;
+    static_for<0, (kTot + kStep * kStageU - 1) / (kStep * kStageU)>([&](auto I) {
+      S x[kStageU];
+      static_for<0, kStageU>([&](auto U) {
+        const int e = (int(I) * kStageU + int(U)) * kStep + (int)threadIdx.x;
+        const int k = (e < kTot ? e : kTot - 1) >> 6;
+        x[U] = k < NNZ ? gU.p[(int64_t)k * gU.rs + (b0 + bc) * gU.bs] : gD.p[(int64_t)(k - NNZ) * gD.rs + (b0 + bc) * gD.bs];
+      });
+      static_for<0, kStageU>([&](auto U) {
+        const int e = (int(I) * kStageU + int(U)) * kStep + (int)threadIdx.x;
+        if (e < kTot) {
+          const int k = e >> 6;
+          sU[e] = b >= nb ? S(1) : (k < NNZ ? x[U] : S(1) / x[U]);
+        }
+      });
+    });
   }
   __syncthreads();
   const bool live = L < nb;
