@@ -45,11 +45,31 @@ template<class TP, int G> constexpr bool rneadSplitPays() {
   constexpr int lds = ((TP::T.nq + 2 * TP::T.nv) * kPitch + WS.nslots * kRneadSlot * kTile) * 4;
   return 5 * (trunk + busiest) <= 3 * (TP::NJ - 1) && lds <= 160 * 1024;
 }
-// RBD_ABA_WAVES=1|2|4 fixes the waves per tile (default: chosen from the batch size)
+// waves of one kernel resident per CU with `threads` threads per workgroup (0 if unknown)
+inline int residentWaves(const void* kernel, int threads) {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, threads, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return blocks * (threads / 64);
+}
+// RBD_ABA_WAVES=1|2|4 fixes the waves per tile.  Default: chosen from the batch size, and
+// four waves at every size when the four-wave kernel keeps at least twice as many waves
+// resident per CU as the one-wave kernel (humanoid: 389 -> 243 registers per lane, one ->
+// two waves per SIMD; ABA at 65,536 configurations 67.7 -> 61.7 us).
 template<class TP, class S>
 void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
   static const int forced = envInt("RBD_ABA_WAVES");
-  const int waves = forced > 0 ? forced : wavesForBatch(B);
+  bool denser = false;
+  if constexpr (abaSplitPays<TP, 4>()) {
+    static const bool d = [] {
+      const int one = residentWaves((const void*)&k_aba<TP, S>, kTile);
+      return one > 0 && residentWaves((const void*)&k_aba_waves<TP, S, 4>, kTile * 4) >= 2 * one;
+    }();
+    denser = d;
+  }
+  const int waves = forced > 0 ? forced : denser ? 4 : wavesForBatch(B);
   if constexpr (abaSplitPays<TP, 4>()) {
     if (waves >= 4) {
       hipLaunchKernelGGL((k_aba_waves<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, tau, out);
