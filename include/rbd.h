@@ -18,6 +18,9 @@
  *   rbd_compute_minverse  computeMinverse           algorithm/aba.hxx:608-902
  *   rbd_forward_kinematics forwardKinematics        algorithm/kinematics.hxx:121-145, 209-238
  *   rbd_joint_jacobians   computeJointJacobians     algorithm/jacobian.hxx:23-62
+ *   rbd_constrained_forward_dynamics
+ *                         forwardDynamics(q, v, tau, J, gamma, inv_damping)
+ *                                                   algorithm/contact-dynamics.hxx:90-167
  *   rbd_multi_rnea_aba    (no counterpart: one launch over instances of several models)
  *   rbd_mlp_*             (no counterpart: learned forward-dynamics surrogate, checked by
  *                          the RNEA residual with exact ABA as fallback)
@@ -164,6 +167,22 @@ int rbd_joint_jacobians(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int
  * FP32: CRBA, FP32 tree factor, unit right-hand sides; FP64: the same in double. */
 int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
                          void* Minv, int64_t ldMinv, uint32_t flags);
+
+/* forwardDynamics with acceleration constraints J ddq + gamma = 0 (contact-dynamics.hpp:
+ * 66-75, contact-dynamics.hxx:90-167): M(q) and nle(q, v) as computeAllTerms gives them,
+ * then ddq = M^-1 (tau - nle + J^T lambda) with
+ * lambda = (J M^-1 J^T + inv_damping I)^-1 (-J M^-1 (tau - nle) - gamma), the reference's
+ * data.lambda_c.  Per configuration J is nc x nv (1 <= nc <= 12), stored column major
+ * like an Eigen matrix: element (i, k) is row i + k nc of the J argument (nc nv rows per
+ * configuration, leading dimension ldJ, same layout flags as the vectors); gamma (nc),
+ * lambda (nc) and ddq (nv) are vectors.  FP32: CRBA and RNEA in FP32, the tree factor and
+ * everything after it in FP64; FP64: all in double.  Like the reference's LLT, a
+ * rank-deficient J with zero damping gives non-finite results. */
+int rbd_constrained_forward_dynamics(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                                     int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, int nc,
+                                     const void* J, int64_t ldJ, const void* gamma, int64_t ldgamma,
+                                     double inv_damping, void* ddq, int64_t ldddq, void* lambda, int64_t ldlambda,
+                                     uint32_t flags);
 
 /* Mixed-robot batch (no reference counterpart: the reference batches one model per
  * call).  Each group is `count` instances of one model, with FP32 device arrays in the

@@ -212,6 +212,42 @@ def flops(model, alg, q=None, v=None, x=None):
     return out.value
 
 
+def constrained_forward_dynamics(model, q, v, tau, J, gamma, inv_damping=0.0, scalar="f64", nthreads=0):
+    """forwardDynamics(model, data, q, v, tau, J, gamma, inv_damping) per column
+    (contact-dynamics.hxx:90-167).  J: (B, nc, nv); gamma: (nc, B).  Returns (ddq, lambda)
+    as (nv, B) and (nc, B)."""
+    q, v, tau = _cols(q, model.nq), _cols(v, model.nv), _cols(tau, model.nv)
+    B, nv = q.shape[1], model.nv
+    J = np.asarray(J, dtype=np.float64)
+    assert J.ndim == 3 and J.shape[0] == B and J.shape[2] == nv, J.shape
+    nc = J.shape[1]
+    Jc = np.ascontiguousarray(J.transpose(0, 2, 1))  # per configuration column major
+    gamma = _cols(gamma, nc)
+    ddq = np.zeros((nv, B), order="F")
+    lam = np.zeros((nc, B), order="F")
+    d = model.cstruct()
+    _check(lib().orc_constrained_forward_dynamics(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(v), _p(tau),
+                                                  int(nc), _p(Jc), _p(gamma), ctypes.c_double(inv_damping),
+                                                  _p(ddq), _p(lam), SCALARS[scalar], nthreads))
+    return np.ascontiguousarray(ddq), np.ascontiguousarray(lam)
+
+
+def time_constrained(model, q, v, tau, J, gamma, inv_damping=0.0, nthreads=0, reps=5):
+    """Median wall time (s) of the OpenMP batch loop of constrained forward dynamics (FP64)."""
+    q, v, tau = _cols(q, model.nq), _cols(v, model.nv), _cols(tau, model.nv)
+    B, nv = q.shape[1], model.nv
+    J = np.asarray(J, dtype=np.float64)
+    nc = J.shape[1]
+    Jc = np.ascontiguousarray(J.transpose(0, 2, 1))
+    gamma = _cols(gamma, nc)
+    sec = ctypes.c_double()
+    d = model.cstruct()
+    _check(lib().orc_time_constrained(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(v), _p(tau), int(nc), _p(Jc),
+                                      _p(gamma), ctypes.c_double(inv_damping), int(nthreads), int(reps),
+                                      ctypes.byref(sec)))
+    return sec.value
+
+
 TIME_ALGS = {"rnea": 0, "aba": 1, "crba": 2, "crba_cholesky": 3, "step": 4, "rnea_derivatives": 5,
              "aba_derivatives": 6}
 

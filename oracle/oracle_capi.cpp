@@ -406,6 +406,75 @@ extern "C" int orc_flops(const ModelDesc* d, int alg, const double* q, const dou
 // 6 computeABADerivatives.  Each thread owns one Data (ModelPool semantics) and writes
 // results into it; inputs are the caller's column-major double arrays.
 #include <chrono>
+namespace {
+template<class S>
+int cfdT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau, int nc, const double* J,
+         const double* gamma, double damping, double* ddq, double* lambda, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    if (!m.isCompact()) throw std::invalid_argument("constrained dynamics requires a compact (DFS-ordered) model");
+    static thread_local Buf<S> bq, bv, bt, bJ, bg;
+    static thread_local std::vector<S> a, l;
+    a.resize(m.nv);
+    l.resize(nc);
+    constrainedForwardDynamics(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * m.nv, m.nv),
+                               bt.load(tau + b * m.nv, m.nv), nc, bJ.load(J + b * nc * m.nv, nc * m.nv),
+                               bg.load(gamma + b * nc, nc), S(damping), a.data(), l.data());
+    store(a.data(), ddq + b * m.nv, m.nv);
+    store(l.data(), lambda + b * nc, nc);
+  });
+}
+}  // namespace
+
+extern "C" int orc_constrained_forward_dynamics(const ModelDesc* d, int64_t B, const double* q, const double* v,
+                                                const double* tau, int nc, const double* J, const double* gamma,
+                                                double damping, double* ddq, double* lambda, int scalar,
+                                                int nthreads) {
+  if (nc < 1) {
+    g_err = "nc must be positive";
+    return 1;
+  }
+#define CALL(S) cfdT<S>(d, B, q, v, tau, nc, J, gamma, damping, ddq, lambda, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+// median wall time of the OpenMP batch loop of constrainedForwardDynamics (FP64)
+extern "C" int orc_time_constrained(const ModelDesc* d, int64_t B, const double* q, const double* v,
+                                    const double* tau, int nc, const double* J, const double* gamma, double damping,
+                                    int nthreads, int reps, double* seconds) {
+  try {
+    const Model<double> m(*d);
+    if (!m.isCompact()) throw std::invalid_argument("model is not compact");
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    std::vector<double> times;
+    for (int r = 0; r < reps; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel num_threads(nt)
+      {
+        Data<double> data(m);
+        std::vector<double> a(m.nv), l(nc);
+#pragma omp for schedule(static)
+        for (int64_t b = 0; b < B; ++b)
+          constrainedForwardDynamics(m, data, q + b * m.nq, v + b * m.nv, tau + b * m.nv, nc, J + b * nc * m.nv,
+                                     gamma + b * nc, damping, a.data(), l.data());
+      }
+      times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    std::sort(times.begin(), times.end());
+    *seconds = times[times.size() / 2];
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
 extern "C" int orc_time(const ModelDesc* d, int alg, int64_t B, const double* q, const double* v, const double* x,
                         int nthreads, int reps, double* seconds) {
   try {

@@ -926,6 +926,75 @@ template<class S> inline void choleskyComputeMinv(const Model<S>& model, const D
 }
 
 // ----------------------------------------------------------------------------------------
+// forwardDynamics with acceleration constraints J ddq + gamma = 0
+// (contact-dynamics.hxx:90-167): computeAllTerms gives M (CRBA) and nle (RNEA at a = 0),
+// then cholesky::decompose and
+//   torque_residual = M^-1 (tau - nle)
+//   sDUiJt = D^-1/2 U^-1 J^T,  JMinvJt = sDUiJt^T sDUiJt + inv_damping I
+//   lambda_c = JMinvJt^-1 (-J torque_residual - gamma)        (LLT, no pivoting)
+//   ddq = M^-1 J^T lambda_c + torque_residual
+// J is nc x nv, column major (element (i, k) at J[i + k nc]).
+// ----------------------------------------------------------------------------------------
+template<class S>
+inline void constrainedForwardDynamics(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* tau,
+                                       int nc, const S* J, const S* gamma, S inv_damping, S* ddq, S* lambda) {
+  using std::sqrt;
+  const int nv = model.nv;
+  std::vector<S> r(nv), zero(nv, S(0));
+  rnea(model, data, q, v, zero.data(), r.data());
+  for (int k = 0; k < nv; ++k) r[k] = tau[k] - r[k];
+  crbaWorld(model, data, q);
+  choleskyDecompose(model, data);
+  choleskySolve(model, data, r.data());
+  std::vector<S> sd((size_t)nv * nc), G((size_t)nc * nc), L((size_t)nc * nc, S(0));
+  for (int i = 0; i < nc; ++i) {
+    S* col = &sd[(size_t)i * nv];
+    for (int k = 0; k < nv; ++k) col[k] = J[i + (size_t)k * nc];
+    choleskyUiv(model, data, col);
+    for (int k = 0; k < nv; ++k) col[k] /= sqrt(data.D[k]);
+  }
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j < nc; ++j) {
+      S g = S(0);
+      for (int k = 0; k < nv; ++k) g += sd[(size_t)i * nv + k] * sd[(size_t)j * nv + k];
+      G[i + (size_t)j * nc] = i == j ? g + inv_damping : g;
+    }
+  for (int i = 0; i < nc; ++i) {
+    S s = S(0);
+    for (int k = 0; k < nv; ++k) s += J[i + (size_t)k * nc] * r[k];
+    lambda[i] = -s - gamma[i];
+  }
+  // G = L L^T, then the two triangular solves
+  for (int j = 0; j < nc; ++j) {
+    S d = G[j + (size_t)j * nc];
+    for (int k = 0; k < j; ++k) d -= L[j + (size_t)k * nc] * L[j + (size_t)k * nc];
+    L[j + (size_t)j * nc] = sqrt(d);
+    for (int i = j + 1; i < nc; ++i) {
+      S t = G[i + (size_t)j * nc];
+      for (int k = 0; k < j; ++k) t -= L[i + (size_t)k * nc] * L[j + (size_t)k * nc];
+      L[i + (size_t)j * nc] = t / L[j + (size_t)j * nc];
+    }
+  }
+  for (int i = 0; i < nc; ++i) {
+    S s = lambda[i];
+    for (int k = 0; k < i; ++k) s -= L[i + (size_t)k * nc] * lambda[k];
+    lambda[i] = s / L[i + (size_t)i * nc];
+  }
+  for (int i = nc - 1; i >= 0; --i) {
+    S s = lambda[i];
+    for (int k = i + 1; k < nc; ++k) s -= L[k + (size_t)i * nc] * lambda[k];
+    lambda[i] = s / L[i + (size_t)i * nc];
+  }
+  for (int k = 0; k < nv; ++k) {
+    S s = S(0);
+    for (int i = 0; i < nc; ++i) s += J[i + (size_t)k * nc] * lambda[i];
+    ddq[k] = s;
+  }
+  choleskySolve(model, data, ddq);
+  for (int k = 0; k < nv; ++k) ddq[k] += r[k];
+}
+
+// ----------------------------------------------------------------------------------------
 // computeRNEADerivatives (rnea-derivatives.hxx:242-541)
 // ----------------------------------------------------------------------------------------
 template<class S>

@@ -319,6 +319,33 @@ class Engine:
                                           _ptr(out), _i64(ldm), ctypes.c_uint32(self._flags(async_))))
         return out
 
+    def constrained_forward_dynamics(self, q, v, tau, J, gamma, inv_damping=0.0, out=None, lambda_out=None,
+                                     async_=False):
+        """forwardDynamics(model, data, q, v, tau, J, gamma, inv_damping) per configuration
+        (contact-dynamics.hxx:90-167): ddq under J ddq + gamma = 0 and the multipliers lambda.
+        J holds nc * nv rows per configuration, element (i, k) of that configuration's
+        nc x nv matrix at row i + k * nc (Eigen's column-major memory); gamma and lambda have
+        nc rows.  Returns (ddq, lambda)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        _, ldt = self._vec(tau, m.nv, "tau")
+        rows = J.shape[1] if self.layout == "col" else J.shape[0]
+        nc = rows // m.nv
+        if nc * m.nv != rows:
+            raise ValueError(f"J: {rows} rows per configuration is not a multiple of nv = {m.nv}")
+        _, ldJ = self._vec(J, nc * m.nv, "J")
+        _, ldg = self._vec(gamma, nc, "gamma")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        lambda_out = self._new_vec(q, nc, B) if lambda_out is None else lambda_out
+        _, lda = self._vec(out, m.nv, "ddq")
+        _, ldl = self._vec(lambda_out, nc, "lambda")
+        _check(lib().rbd_constrained_forward_dynamics(
+            m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v), _i64(ldv), _ptr(tau),
+            _i64(ldt), int(nc), _ptr(J), _i64(ldJ), _ptr(gamma), _i64(ldg), ctypes.c_double(inv_damping), _ptr(out),
+            _i64(lda), _ptr(lambda_out), _i64(ldl), ctypes.c_uint32(self._flags(async_))))
+        return out, lambda_out
+
     @staticmethod
     def mats_to_numpy(t, layout, nv):
         """Device matrix output -> (B, nv, nv) numpy array with [b] the configuration's matrix."""

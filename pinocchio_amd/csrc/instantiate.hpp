@@ -8,6 +8,7 @@
 #include "k_crba.hpp"
 #include "k_deriv.hpp"
 #include "k_kin.hpp"
+#include "k_kkt.hpp"
 #include "_gen/topologies.inc"
 
 namespace rbd {
@@ -104,8 +105,8 @@ template<class TP, class S>
 void launchCholSolve(hipStream_t st, int64_t B, CView<double> U, CView<double> D, View<S> x) {
   hipLaunchKernelGGL((k_chol64_solve<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, U, D, x);
 }
-// RNEA derivatives: with few tiles, 4 waves per tile, each emitting the output entries of
-// a share of the joints (EmitGroups), shorten the critical path
+// RNEA derivatives: with few tiles, 4 waves per tile, each running a share of the subtrees
+// (WaveSplit), shorten the critical path
 template<class TP, class S>
 void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
                  View<S> M, int mask, int waves) {
@@ -131,6 +132,15 @@ template<class TP, class S>
 void launchKin(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> oM, View<S> vo,
                View<S> ao, View<S> J) {
   hipLaunchKernelGGL((k_kin<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, oM, vo, ao, J);
+}
+
+template<class TP, class S>
+void launchKkt(hipStream_t st, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J, CView<S> tau,
+               CView<S> nle, CView<S> gamma, double damping, View<double> Y, View<S> ddq, View<S> lam) {
+  hipLaunchKernelGGL((k_kkt_solve<TP, S>), dim3((unsigned)((B + kKktTile - 1) / kKktTile)), dim3(64 * kKktWaves), 0,
+                     st, B, U, D, nc, J, tau, nle, Y);
+  hipLaunchKernelGGL((k_kkt_finish<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, nc, J, gamma, damping,
+                     CView<double>{Y.p, Y.rs, Y.bs}, ddq, lam);
 }
 
 template<class TP> TopoOps makeOpsBase() {
@@ -167,6 +177,13 @@ template<class TP> void add_chol(TopoOps& o) {
 }
 template<class TP> void add_rnead(TopoOps& o) { o.rnead_f32 = &launchRneaD<TP, float>; }
 template<class TP> void add_abads(TopoOps& o) { o.abad_solve_f32 = &launchAbaDSolve<TP, float>; }
+// (topologies whose staged FP64 factor fits the LDS)
+template<class TP> void add_kkt(TopoOps& o) {
+  if constexpr (kktSolveLdsBytes<TP>() <= 160 * 1024) {
+    o.kkt_f32 = &launchKkt<TP, float>;
+    o.kkt_f64 = &launchKkt<TP, double>;
+  }
+}
 template<class TP> void add_kin(TopoOps& o) {
   o.kin_f32 = &launchKin<TP, float>;
   o.kin_f64 = &launchKin<TP, double>;

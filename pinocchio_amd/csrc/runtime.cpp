@@ -16,6 +16,7 @@
 
 #include "../../include/rbd.h"
 #include "k_dyn.hpp"
+#include "k_kkt.hpp"
 #include "ops.hpp"
 #include "runtime.hpp"
 
@@ -653,6 +654,93 @@ int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
       m->ops->cholsolve_f64(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1},
                             View<double>{Mi + (int64_t)c * nv * vo.rs, vo.rs, vo.bs});
     finishCall(ctx, flags, st);
+  });
+}
+
+}  // extern "C"
+
+namespace {
+template<class S> __global__ void k_fill_zero(S* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = S(0);
+}
+
+// forwardDynamics(model, data, q, v, tau, J, gamma, inv_damping): CRBA, FP64 tree factor,
+// nle = RNEA at a = 0, then the KKT kernels (k_kkt.hpp)
+template<class S>
+void cfdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
+             const void* tau, int64_t ldtau, int nc, const void* J, int64_t ldJ, const void* gamma, int64_t ldg,
+             double damping, void* ddq, int64_t ldddq, void* lambda, int64_t ldl, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const int nv = m->nv, nnz = m->ops->nnzU;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v");
+  auto vt = argView<S>(nv, B, ldtau, flags, "tau");
+  auto vJ = argView<S>(nc * nv, B, ldJ, flags, "J");
+  auto vg = argView<S>(nc, B, ldg, flags, "gamma");
+  auto va = argView<S>(nv, B, ldddq, flags, "ddq");
+  auto vl = argView<S>(nc, B, ldl, flags, "lambda");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const bool batchLayout = (flags & RBD_LAYOUT_BATCH) != 0;
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* dt = (const S*)st.in(tau, vt.extent * sizeof(S));
+  const S* dJ = (const S*)st.in(J, vJ.extent * sizeof(S));
+  const S* dg = (const S*)st.in(gamma, vg.extent * sizeof(S));
+  S* da = (S*)st.out(ddq, va.extent * sizeof(S), ldddq != nv || batchLayout);
+  S* dl = (S*)st.out(lambda, vl.extent * sizeof(S), ldl != nc || batchLayout);
+  // workspace, batch-fastest: U, D, Y (FP64), then M, nle and one zero (S)
+  const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B, nY = (size_t)(nc + 1) * nv * B;
+  const size_t nM = (size_t)nv * nv * B, nN = (size_t)nv * B;
+  double* wU = (double*)ctx->workspace((nU + nD + nY) * 8 + (nM + nN + 1) * sizeof(S));
+  double* wD = wU + nU;
+  double* wY = wD + nD;
+  S* wM = (S*)(wY + nY);
+  S* wN = wM + nM;
+  S* wZ = wN + nN;
+  const S* mc = f32 ? (const S*)m->mc32 : (const S*)m->mc64;
+  const TopoOps& o = *m->ops;
+  const CView<S> cq{dq, vq.rs, vq.bs}, cv{dv, vv.rs, vv.bs};
+  if constexpr (f32) {
+    o.crba_f32(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f32(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+  } else {
+    o.crba_f64(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f64(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+  }
+  // nle = rnea(q, v, 0): the acceleration view repeats one zero (both strides 0)
+  hipLaunchKernelGGL(k_fill_zero<S>, dim3(1), dim3(64), 0, ctx->stream, wZ, (int64_t)1);
+  if constexpr (f32) o.rnea_f32(ctx->stream, mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
+  else o.rnea_f64(ctx->stream, mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
+  const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
+  kkt(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
+      CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{dg, vg.rs, vg.bs}, damping, View<double>{wY, B, 1},
+      View<S>{da, va.rs, va.bs}, View<S>{dl, vl.rs, vl.bs});
+  finishCall(ctx, flags, st);
+}
+}  // namespace
+
+extern "C" {
+
+int rbd_constrained_forward_dynamics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q,
+                                     int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, int nc,
+                                     const void* J, int64_t ldJ, const void* gamma, int64_t ldgamma,
+                                     double inv_damping, void* ddq, int64_t ldddq, void* lambda, int64_t ldlambda,
+                                     uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(nc >= 1 && nc <= kKktMaxConstraints, "the constraint count must be between 1 and 12");
+    RBD_CHECK_ARG(B == 0 || (q && v && tau && J && gamma && ddq && lambda), "null array argument");
+    if (!m->compact || !m->ops->kkt_f32 || !m->ops->crba_f32)
+      throw Error(RBD_EUNSUPPORTED, "constrained forward dynamics: no kernels for this topology");
+    if (dtype == RBD_F32)
+      cfdCall<float>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, nc, J, ldJ, gamma, ldgamma, inv_damping, ddq, ldddq,
+                     lambda, ldlambda, flags);
+    else
+      cfdCall<double>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, nc, J, ldJ, gamma, ldgamma, inv_damping, ddq, ldddq,
+                      lambda, ldlambda, flags);
   });
 }
 
