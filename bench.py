@@ -133,7 +133,7 @@ def main():
     ap.add_argument("--sur-samples", type=int, default=1 << 22)
     ap.add_argument("--sur-epochs", type=int, default=8)
     ap.add_argument("--only", default=None,
-                    help="one part: step|aba|crba|chol|rnea|rnead|abad|mlp|surrogate|copy")
+                    help="one part: step|aba|crba|chol|rnea|rnead|abad|cfd|mlp|surrogate|copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; the batch shards with no "
@@ -195,6 +195,23 @@ def main():
     nnz = len(rows)
     Ubuf = torch.empty((nnz, B), dtype=torch.float64, device=dev)
     Dbuf = torch.empty((nv, B), dtype=torch.float64, device=dev)
+    # constrained forward dynamics (forwardDynamics with J, gamma): both feet of the humanoid
+    # in contact, J = their WORLD Jacobians from the engine's own kernel (support columns),
+    # gamma = 0; element (i, k) of a configuration's 12 x nv J at row i + 12 k
+    feet = (7, 13)
+    ncon = 6 * len(feet)
+    J6 = eng.joint_jacobians(Q).view(nv, 6, B)
+    Jc = torch.zeros((nv, ncon, B), device=dev)
+    for f, j in enumerate(feet):
+        sup, i = [], j
+        while i > 0:
+            sup += list(range(desc.idx_v[i], desc.idx_v[i] + desc.nvs[i]))
+            i = int(desc.parents[i])
+        Jc[sup, 6 * f:6 * f + 6, :] = J6[sup]
+    Jrows = Jc.reshape(nv * ncon, B).contiguous()
+    gam0 = torch.zeros((ncon, B), device=dev)
+    cddq = torch.empty((nv, B), device=dev)
+    clam = torch.empty((ncon, B), device=dev)
     L = pa.lib()
     import ctypes
 
@@ -248,12 +265,14 @@ def main():
         elapsed = time.perf_counter() - t0
     if only is not None:
         extra = {}
-        if only in ("rnea", "rnead", "abad"):
+        if only in ("rnea", "rnead", "abad", "cfd"):
             Bd = args.deriv_batch
             Qd, Vd, Ad, Td = Q[:, :Bd].contiguous(), V[:, :Bd].contiguous(), A[:, :Bd].contiguous(), T[:, :Bd].contiguous()
             f = {"rnea": lambda: eng.rnea(Q, V, A, async_=True),
                  "rnead": lambda: eng.rnea_derivatives(Qd, Vd, Ad, async_=True),
-                 "abad": lambda: eng.aba_derivatives(Qd, Vd, Td, async_=True)}[only]
+                 "abad": lambda: eng.aba_derivatives(Qd, Vd, Td, async_=True),
+                 "cfd": lambda: eng.constrained_forward_dynamics(Q, V, T, Jrows, gam0, out=cddq, lambda_out=clam,
+                                                                 async_=True)}[only]
             for _ in range(args.warmup):
                 f()
             ctx.synchronize()
@@ -360,6 +379,12 @@ def main():
     others["joint_jacobians_humanoid36_f32"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3,
                                                 "GB_s": nbytes * B / t / 1e9}
 
+    t = time_call(lambda: eng.constrained_forward_dynamics(Q, V, T, Jrows, gam0, out=cddq, lambda_out=clam,
+                                                           async_=True), reps=10)
+    others["constrained_forward_dynamics_humanoid36_f32"] = {
+        "batch": B, "constraints": ncon, "configs_per_s": B / t, "ms": t * 1e3,
+        "note": "CRBA, RNEA(a=0) in FP32; FP64 tree factor, KKT solve and LLT; J = both feet's 6D Jacobians"}
+
     # drop-in host path: numpy matrices in and out (PCIe both ways), not the device-resident figure
     pool = pa.ModelPool(desc, size=8)
     qh, vh, th = (np.asfortranarray(x) for x in (q, v, tau))
@@ -411,6 +436,11 @@ def main():
         r = oracle.aba_derivatives(desc, qs[:, :nd], vs[:, :nd], ts[:, :nd])
         for name, x, ref in zip(("ddq_dq", "ddq_dv", "Minv"), o[:3], r[:3]):
             errors[name] = rel_mats(pa.Engine.mats_to_numpy(x, "batch", nv), ref)
+        Jh = Jrows[:, :nd].double().cpu().numpy().reshape(nv, ncon, nd).transpose(2, 1, 0)
+        ra, rl = oracle.constrained_forward_dynamics(desc, qs[:, :nd], vs[:, :nd], ts[:, :nd], Jh,
+                                                     np.zeros((ncon, nd)))
+        errors["constrained_ddq"] = rel_cols(cddq[:, :nd].double().cpu().numpy(), ra)
+        errors["constrained_lambda"] = rel_cols(clam[:, :nd].double().cpu().numpy(), rl)
         rm = oracle.rnea(man, qm[:, :1024], vm[:, :1024], am[:, :1024])
         errors["rnea_manipulator6_f64"] = rel_cols(em.rnea(Qm[:, :1024].contiguous(), Vm[:, :1024].contiguous(),
                                                            Am[:, :1024].contiguous()).cpu().numpy(), rm)
@@ -436,6 +466,12 @@ def main():
         gpu_key = {"rnea": "rnea_humanoid36_f32", "aba": "aba_humanoid36_f32",
                    "rnea_derivatives": "rnea_derivatives_humanoid36_f32",
                    "aba_derivatives": "aba_derivatives_humanoid36_f32"}
+        nbc = 16384
+        Jh = Jrows[:, :nbc].double().cpu().numpy().reshape(nv, ncon, nbc).transpose(2, 1, 0)
+        s_ = oracle.time_constrained(desc, q[:, :nbc], v[:, :nbc], tau[:, :nbc], Jh, np.zeros((ncon, nbc)),
+                                     nthreads=cores, reps=3)
+        cpu_alg["constrained_forward_dynamics"] = {"batch": nbc, "configs_per_s": nbc / s_, "cores": cores}
+        gpu_key["constrained_forward_dynamics"] = "constrained_forward_dynamics_humanoid36_f32"
         for alg, c in cpu_alg.items():
             c["gpu_over_cpu"] = others[gpu_key[alg]]["configs_per_s"] / c["configs_per_s"]
         others["cpu_fp64_oracle"] = cpu_alg

@@ -185,6 +185,34 @@ inline void computeMinverseInParallel(size_t num_threads, ModelPool& pool, const
   detail::fromF32(fo, Minv);
 }
 
+// forwardDynamics(model, data, q, v, tau, J, gamma, inv_damping) per column
+// (contact-dynamics.hxx:90-167): J is nc x (nv * B), the configurations' nc x nv constraint
+// Jacobians side by side (contiguous, ld = nc); gamma and lambda are nc x B, ddq nv x B.
+// FP64 throughout.
+inline void forwardDynamicsInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
+                                      const CMat& tau, const CMat& J, const CMat& gamma, double inv_damping,
+                                      const MMat& ddq, const MMat& lambda) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(v, pool.nv(), "The velocity matrix");
+  detail::checkRows(tau, pool.nv(), "The joint torque matrix");
+  detail::checkRows(ddq, pool.nv(), "The output matrix");
+  detail::checkCols(v, q.cols, "The velocity matrix");
+  detail::checkCols(tau, q.cols, "The joint torque matrix");
+  detail::checkCols(ddq, q.cols, "The output matrix");
+  const int nc = (int)J.rows;
+  detail::checkCols(J, q.cols * pool.nv(), "The constraint Jacobian");
+  if (J.ld != J.rows) throw std::invalid_argument("The constraint Jacobian must be contiguous");
+  detail::checkRows(gamma, nc, "The constraint drift");
+  detail::checkCols(gamma, q.cols, "The constraint drift");
+  detail::checkRows(lambda, nc, "The multiplier matrix");
+  detail::checkCols(lambda, q.cols, "The multiplier matrix");
+  check(rbd_constrained_forward_dynamics(pool.ctx(), pool.model(), q.cols, RBD_F64, q.data, q.ld, v.data, v.ld,
+                                         tau.data, tau.ld, nc, J.data, (int64_t)nc * pool.nv(), gamma.data,
+                                         gamma.ld, inv_damping, ddq.data, ddq.ld, lambda.data, lambda.ld,
+                                         RBD_HOST_PTRS));
+}
+
 // computeRNEADerivatives per column: three nv x (nv * B) outputs (blocks side by side).
 inline void computeRNEADerivativesInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
                                              const CMat& a, const MMat& dtau_dq, const MMat& dtau_dv,

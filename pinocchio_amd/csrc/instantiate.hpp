@@ -136,11 +136,10 @@ void launchKin(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, C
 
 template<class TP, class S>
 void launchKkt(hipStream_t st, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J, CView<S> tau,
-               CView<S> nle, CView<S> gamma, double damping, View<double> Y, View<S> ddq, View<S> lam) {
-  hipLaunchKernelGGL((k_kkt_solve<TP, S>), dim3((unsigned)((B + kKktTile - 1) / kKktTile)), dim3(64 * kKktWaves), 0,
-                     st, B, U, D, nc, J, tau, nle, Y);
-  hipLaunchKernelGGL((k_kkt_finish<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, nc, J, gamma, damping,
-                     CView<double>{Y.p, Y.rs, Y.bs}, ddq, lam);
+               CView<S> nle, CView<S> gamma, double damping, View<S> ddq, View<S> lam) {
+  using K = KktShape<S>;
+  hipLaunchKernelGGL((k_kkt<TP, S>), dim3((unsigned)((B + K::T - 1) / K::T)), dim3(64 * K::W), 0, st, B, U, D, nc, J,
+                     tau, nle, gamma, damping, ddq, lam);
 }
 
 template<class TP> TopoOps makeOpsBase() {
@@ -179,7 +178,7 @@ template<class TP> void add_rnead(TopoOps& o) { o.rnead_f32 = &launchRneaD<TP, f
 template<class TP> void add_abads(TopoOps& o) { o.abad_solve_f32 = &launchAbaDSolve<TP, float>; }
 // (topologies whose staged FP64 factor fits the LDS)
 template<class TP> void add_kkt(TopoOps& o) {
-  if constexpr (kktSolveLdsBytes<TP>() <= 160 * 1024) {
+  if constexpr (kktLdsBytes<TP, float>() <= 160 * 1024 && kktLdsBytes<TP, double>() <= 160 * 1024) {
     o.kkt_f32 = &launchKkt<TP, float>;
     o.kkt_f64 = &launchKkt<TP, double>;
   }

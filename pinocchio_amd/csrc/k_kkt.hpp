@@ -3,14 +3,18 @@
 // contact-dynamics.hxx:90-167):
 //
 //   M = crba(q), nle = rnea(q, v, 0), M = U D U^T                 (existing kernels)
-//   r = M^-1 (tau - nle),  Y = M^-1 J^T                           (k_kkt_solve)
-//   lambda = (J Y + damping I)^-1 (-J r - gamma)                   (k_kkt_finish)
-//   ddq = r + Y lambda
+//   r = M^-1 (tau - nle),  Y = M^-1 J^T                           (k_kkt, phase 2)
+//   G = J Y + damping I,  lambda = G^-1 (-J r - gamma)             (phases 3-4)
+//   ddq = r + Y lambda                                             (phase 5)
 //
 // The reference forms J M^-1 J^T as S^T S with S = D^-1/2 U^-1 J^T and adds M^-1 J^T lambda
-// by one more solve; here the nc + 1 full solves run side by side (two per wave, the FP64
-// factor of a 32-configuration tile staged once in LDS), so the small dense part needs no
-// further pass over the factor.  Arithmetic is FP64 from the factor on.
+// by one more solve.  Here one workgroup owns a tile of T configurations: the FP64 factor
+// and J of the tile are staged in LDS once, the nc + 1 solves run side by side (T lanes
+// per right-hand side, so the lanes of different right-hand sides read the same LDS
+// words), each solve applies J to its own result, and the small dense part and the final
+// combination go through LDS, so nothing but the inputs and ddq, lambda touches HBM.
+// Arithmetic is FP64 from the factor on; Y is kept in LDS in the input precision (FP32 for
+// FP32 inputs, whose outputs are FP32 too).
 #pragma once
 
 #include "k_crba.hpp"
@@ -18,173 +22,221 @@
 namespace rbd {
 
 constexpr int kKktMaxConstraints = 12;
-constexpr int kKktWaves = (kKktMaxConstraints + 2) / 2;  // nc + 1 right-hand sides, two per wave
-constexpr int kKktTile = 32;                              // configurations per workgroup of the solve
 
-// LDS bytes of the solve's staged factor
-template<class TP> constexpr int kktSolveLdsBytes() { return (nnzOf<TP>() + TP::T.nv) * kKktTile * 8; }
+// T configurations per tile: small enough that two (FP32) or three (FP64) workgroups fit a
+// CU's LDS, so one tile's staging overlaps another's arithmetic
+template<class S> struct KktShape {
+  static constexpr int T = sizeof(S) == 4 ? 16 : 8;
+  static constexpr int RPW = 64 / T;  // right-hand sides per wave
+  static constexpr int W = (kKktMaxConstraints + 1 + RPW - 1) / RPW;
+  static constexpr int kU = 16;       // staging loads in flight per thread
+};
 
-// Y[:, j] = M^-1 J[j, :]^T for j < nc and Y[:, nc] = M^-1 (tau - nle).  Lanes 0-31 of wave w
-// take right-hand side 2w and lanes 32-63 right-hand side 2w + 1, both for the tile's 32
-// configurations, so the two halves read the same LDS words.  J is nc x nv per
-// configuration, column major (element (j, k) at row j + k nc of the view); Y is
-// (nc + 1) nv FP64 rows per configuration (element (k, j) at row k + j nv).
+// LDS bytes of the kernel: the factor (U entries then 1/D, FP64) and J (S) of a tile; after
+// the solves the space holds Y = M^-1 [J^T, tau - nle] (S), J [Y r] and lambda (FP64)
+template<class TP, class S> constexpr int kktLdsBytes() {
+  constexpr int T = KktShape<S>::T, NV = TP::T.nv, NC = kKktMaxConstraints;
+  const int staged = (nnzOf<TP>() + NV) * T * 8 + NC * NV * T * (int)sizeof(S);
+  const int later = (NC + 1) * NV * T * (int)sizeof(S) + NC * (NC + 1) * T * 8 + NC * T * 8;
+  return staged > later ? staged : later;
+}
+
+// J is nc x nv per configuration, column major (element (i, k) at row i + k nc of gJ).
 template<class TP, class S>
-__global__ __launch_bounds__(64 * kKktWaves) void k_kkt_solve(int64_t B, CView<double> gU, CView<double> gD, int nc,
-                                                             CView<S> gJ, CView<S> gtau, CView<S> gnle,
-                                                             View<double> gY) {
+__global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<double> gU, CView<double> gD, int nc,
+                                                            CView<S> gJ, CView<S> gtau, CView<S> gnle, CView<S> gg,
+                                                            double damping, View<S> gddq, View<S> glam) {
+  using K = KktShape<S>;
+  constexpr int T = K::T, NT = 64 * K::W;
   constexpr int NV = TP::T.nv;
   constexpr int NNZ = nnzOf<TP>();
+  constexpr int NC = kKktMaxConstraints;
   using P = CholPattern<TP>;
-  __shared__ double sU[(NNZ + NV) * kKktTile];  // U entries, then 1/D; [entry][configuration]
-  const int64_t b0 = int64_t(blockIdx.x) * kKktTile;
-  const int nb = (int)(B - b0 < kKktTile ? B - b0 : kKktTile);
+  __shared__ double smem[kktLdsBytes<TP, S>() / 8];
+  double* sF = smem;                                     // factor: [entry][configuration]
+  S* sJ = reinterpret_cast<S*>(smem + (NNZ + NV) * T);  // J: element (i, k) at [(i + k NC) T + c]
+  // after the solves: Y (element (k, j) at [(k + j NV) T + c]), J [Y r], lambda
+  S* sY = reinterpret_cast<S*>(smem);
+  constexpr int kYDoubles = ((NC + 1) * NV * T * (int)sizeof(S) + 7) / 8;
+  double* sP = smem + kYDoubles;
+  double* sL = sP + NC * (NC + 1) * T;
+  const int64_t b0 = int64_t(blockIdx.x) * T;
+  const int nb = (int)(B - b0 < T ? B - b0 : T);
+  const int t = (int)threadIdx.x;
+
+  // ---- phase 1: stage the factor and J (K::kU loads in flight per thread; columns past
+  // the batch end repeat the last configuration and are never stored)
   {
-    // kStageU loads in flight per thread; configurations past the batch end repeat the
-    // last one (their results are not stored)
-    constexpr int kTot = (NNZ + NV) * kKktTile, kStep = 64 * kKktWaves;
-    const int t = (int)threadIdx.x;
-    static_for<0, (kTot + kStep * kStageU - 1) / (kStep * kStageU)>([&](auto I) {
-      double x[kStageU];
-      static_for<0, kStageU>([&](auto U) {
-        const int e = (int(I) * kStageU + int(U)) * kStep + t;
+    constexpr int kTot = (NNZ + NV) * T;
+    static_for<0, (kTot + NT * K::kU - 1) / (NT * K::kU)>([&](auto I) {
+      double xs[K::kU];
+      static_for<0, K::kU>([&](auto U) {
+        const int e = (int(I) * K::kU + int(U)) * NT + t;
         const int ec = e < kTot ? e : kTot - 1;
-        const int k = ec / kKktTile, c = ec % kKktTile;
+        const int k = ec / T, c = ec % T;
         const int64_t bc = b0 + (c < nb ? c : nb - 1);
-        x[U] = k < NNZ ? gU.p[(int64_t)k * gU.rs + bc * gU.bs] : gD.p[(int64_t)(k - NNZ) * gD.rs + bc * gD.bs];
+        xs[U] = k < NNZ ? gU.p[(int64_t)k * gU.rs + bc * gU.bs] : gD.p[(int64_t)(k - NNZ) * gD.rs + bc * gD.bs];
       });
-      static_for<0, kStageU>([&](auto U) {
-        const int e = (int(I) * kStageU + int(U)) * kStep + t;
-        if (e < kTot) sU[e] = e / kKktTile < NNZ ? x[U] : 1.0 / x[U];
+      static_for<0, K::kU>([&](auto U) {
+        const int e = (int(I) * K::kU + int(U)) * NT + t;
+        if (e < kTot) sF[e] = e / T < NNZ ? xs[U] : 1.0 / xs[U];
+      });
+    });
+    // J padded to NC rows per column with zeros, so the products below need no guard
+    constexpr int jTot = NC * NV * T;
+    static_for<0, (jTot + NT * K::kU - 1) / (NT * K::kU)>([&](auto I) {
+      S xs[K::kU];
+      static_for<0, K::kU>([&](auto U) {
+        const int e = (int(I) * K::kU + int(U)) * NT + t;
+        const int ec = e < jTot ? e : jTot - 1;
+        const int rp = ec / T, c = ec % T, i = rp % NC, k = rp / NC;
+        const int64_t bc = b0 + (c < nb ? c : nb - 1);
+        xs[U] = i < nc ? gJ.p[(int64_t)(i + k * nc) * gJ.rs + bc * gJ.bs] : S(0);
+      });
+      static_for<0, K::kU>([&](auto U) {
+        const int e = (int(I) * K::kU + int(U)) * NT + t;
+        if (e < jTot) sJ[e] = xs[U];
       });
     });
   }
   __syncthreads();
-  // no barrier below: idle half-waves may leave
-  const int L = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-  const int c = L & (kKktTile - 1);
-  const int j = 2 * w + (L >> 5);
-  if (j > nc || c >= nb) return;
+
+  // ---- phase 2: right-hand side j of configuration c (j < nc: row j of J; j = nc: tau - nle)
+  const int L = t & 63, w = t >> 6;
+  const int c = L % T;
+  const int j = w * K::RPW + L / T;
+  const bool active = j <= nc && c < nb;
   const int64_t b = b0 + c;
   double x[NV];
-  if (j < nc) {
+  double p[NC];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) x[k] = double(gJ.p[(int64_t)(j + k * nc) * gJ.rs + b * gJ.bs]);
-  } else {
+  for (int i = 0; i < NC; ++i) p[i] = 0.0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      x[k] = double(gtau.p[(int64_t)k * gtau.rs + b * gtau.bs]) - double(gnle.p[(int64_t)k * gnle.rs + b * gnle.bs]);
+  for (int k = 0; k < NV; ++k) x[k] = 0.0;
+  if (active) {
+    if (j < nc) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) x[k] = double(sJ[(j + k * NC) * T + c]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        x[k] = double(gtau.p[(int64_t)k * gtau.rs + b * gtau.bs]) - double(gnle.p[(int64_t)k * gnle.rs + b * gnle.bs]);
+    }
+    // x <- M^-1 x: Uiv, D^-1, Utiv (cholesky.hxx:193-317, 466-520).  A compiler memory
+    // clobber at every elimination step keeps each factor read next to its use.
+    const double* su = sF + c;
+    static_for<0, NV>([&](auto KK) {
+      constexpr int k = NV - 1 - KK;
+      stepFence();
+      asm volatile("" ::: "memory");
+      static_for<k + 1, NV>([&](auto EE) {
+        constexpr int e2 = EE;
+        if constexpr (P::anc(k, e2)) {
+          constexpr int pk = P::packed(k, e2);  // (forced to a compile-time constant)
+          x[k] -= su[pk * T] * x[e2];
+        }
+      });
+    });
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int k = 0; k < NV; ++k) x[k] *= su[(NNZ + k) * T];
+    static_for<0, NV>([&](auto KK) {
+      constexpr int k = KK;
+      stepFence();
+      asm volatile("" ::: "memory");
+      static_for<k + 1, NV>([&](auto EE) {
+        constexpr int e2 = EE;
+        if constexpr (P::anc(k, e2)) {
+          constexpr int pk = P::packed(k, e2);
+          x[e2] -= su[pk * T] * x[k];
+        }
+      });
+    });
+    // p = J x over the zero-padded rows, one column of J per step (compile-time LDS offsets)
+    const S* sj = sJ + c;
+    static_for<0, NV>([&](auto KK) {
+      constexpr int k = KK;
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int i = 0; i < NC; ++i) p[i] += double(sj[(i + k * NC) * T]) * x[k];
+    });
   }
-  // x <- M^-1 x: Uiv, D^-1, Utiv (cholesky.hxx:193-317, 466-520).  A compiler memory
-  // clobber at every elimination step keeps each factor read next to its use (otherwise
-  // the reads are scheduled together and the factor is held in registers).
-  const double* su = sU + c;
-  static_for<0, NV>([&](auto KK) {
-    constexpr int k = NV - 1 - KK;
-    stepFence();
-    asm volatile("" ::: "memory");
-    static_for<k + 1, NV>([&](auto EE) {
-      constexpr int e2 = EE;
-      if constexpr (P::anc(k, e2)) {
-        constexpr int pk = P::packed(k, e2);  // (forced to a compile-time constant)
-        x[k] -= su[pk * kKktTile] * x[e2];
-      }
-    });
-  });
-  asm volatile("" ::: "memory");
-#pragma unroll
-  for (int k = 0; k < NV; ++k) x[k] *= su[(NNZ + k) * kKktTile];
-  static_for<0, NV>([&](auto KK) {
-    constexpr int k = KK;
-    stepFence();
-    asm volatile("" ::: "memory");
-    static_for<k + 1, NV>([&](auto EE) {
-      constexpr int e2 = EE;
-      if constexpr (P::anc(k, e2)) {
-        constexpr int pk = P::packed(k, e2);
-        x[e2] -= su[pk * kKktTile] * x[k];
-      }
-    });
-  });
-  double* py = gY.p + b * gY.bs + (int64_t)(j * NV) * gY.rs;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) stOut(&py[(int64_t)k * gY.rs], x[k]);
-}
+  __syncthreads();  // J and the factor are dead from here on
 
-// lambda and ddq from Y; one lane per configuration, the packed lower triangle of the
-// nc x nc Delassus matrix J M^-1 J^T and lambda in LDS (run-time nc).  Its Cholesky
-// factorisation has no pivoting and no rank check, like the reference's LLT: a rank-deficient
-// J with zero damping gives non-finite results.
-template<class TP, class S>
-__global__ __launch_bounds__(64) void k_kkt_finish(int64_t B, int nc, CView<S> gJ, CView<S> gg, double damping,
-                                                  CView<double> gY, View<S> gddq, View<S> glam) {
-  constexpr int NV = TP::T.nv;
-  constexpr int kG = kKktMaxConstraints * (kKktMaxConstraints + 1) / 2;
-  __shared__ double sG[kG * kTile];
-  __shared__ double sl[kKktMaxConstraints * kTile];
-  const int L = laneId();
-  const int64_t b = int64_t(blockIdx.x) * kTile + L;
-  if (b >= B) return;  // no barrier below
-  double* G = sG + L;
-  double* lam = sl + L;
-  auto gi = [](int r, int c) { return (r * (r + 1) / 2 + c) * kTile; };  // (r, c), c <= r
-  const double* py = gY.p + b * gY.bs;
-  auto yv = [&](int j, int k) { return py[(int64_t)(k + j * NV) * gY.rs]; };
-  double r[NV];
+  // ---- phase 3: column j of Y and of J [Y r] into LDS (element (i, j) of the latter at
+  // [(i + j NC) T + c])
+  if (active) {
 #pragma unroll
-  for (int k = 0; k < NV; ++k) r[k] = yv(nc, k);
-#pragma unroll 1
-  for (int i = 0; i < nc; ++i) {
-    double Ji[NV];
+    for (int k = 0; k < NV; ++k) sY[(k + j * NV) * T + c] = S(x[k]);
 #pragma unroll
-    for (int k = 0; k < NV; ++k) Ji[k] = double(gJ.p[(int64_t)(i + k * nc) * gJ.rs + b * gJ.bs]);
-    double s = 0;
+    for (int i = 0; i < NC; ++i)
+      if (i < nc) sP[(i + j * NC) * T + c] = p[i];
+  }
+  __syncthreads();
+
+  // ---- phase 4: one lane per configuration: G = J Y + damping I (padded to NC with a unit
+  // diagonal), LLT, lambda = G^-1 (-J r - gamma).  No pivoting and no rank check, like the
+  // reference's LLT: a rank-deficient J with zero damping gives non-finite results.
+  if (t < nb) {
+    constexpr auto gi = [](int r, int cc) { return r * (r + 1) / 2 + cc; };  // (r, cc), cc <= r
+    const int64_t bt = b0 + t;
+    double G[NC * (NC + 1) / 2], l[NC];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) s += Ji[k] * r[k];
-    lam[i * kTile] = -s - double(gg.p[(int64_t)i * gg.rs + b * gg.bs]);
-#pragma unroll 1
-    for (int d = 0; d <= i; ++d) {
-      double g = 0;
+    for (int i = 0; i < NC; ++i) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k) g += Ji[k] * yv(d, k);
-      G[gi(i, d)] = d == i ? g + damping : g;
+      for (int d = 0; d <= i; ++d)
+        G[gi(i, d)] = i < nc ? sP[(i + d * NC) * T + t] + (d == i ? damping : 0.0) : (d == i ? 1.0 : 0.0);
+      l[i] = i < nc ? -sP[(i + nc * NC) * T + t] - double(gg.p[(int64_t)i * gg.rs + bt * gg.bs]) : 0.0;
+    }
+    // column by column: L_dd = sqrt(.), then the column below scaled by 1 / L_dd (kept)
+    double rd[NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+      double s = G[gi(d, d)];
+#pragma unroll
+      for (int k = 0; k < d; ++k) s -= G[gi(d, k)] * G[gi(d, k)];
+      const double ld = sqrt(s);
+      rd[d] = 1.0 / ld;
+#pragma unroll
+      for (int i = d + 1; i < NC; ++i) {
+        double t2 = G[gi(i, d)];
+#pragma unroll
+        for (int k = 0; k < d; ++k) t2 -= G[gi(i, k)] * G[gi(d, k)];
+        G[gi(i, d)] = t2 * rd[d];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      double s = l[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= G[gi(i, k)] * l[k];
+      l[i] = s * rd[i];
+    }
+#pragma unroll
+    for (int i = NC - 1; i >= 0; --i) {
+      double s = l[i];
+#pragma unroll
+      for (int k = i + 1; k < NC; ++k) s -= G[gi(k, i)] * l[k];
+      l[i] = s * rd[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+      if (i < nc) {
+        sL[i * T + t] = l[i];
+        stOut(&glam.p[(int64_t)i * glam.rs + bt * glam.bs], S(l[i]));
+      }
+  }
+  __syncthreads();
+
+  // ---- phase 5: ddq = r + Y lambda, fixed order
+  for (int e = t; e < NV * T; e += NT) {
+    const int k = e / T, cc = e % T;
+    if (cc < nb) {
+      double a = double(sY[(k + nc * NV) * T + cc]);
+      for (int i = 0; i < nc; ++i) a += sL[i * T + cc] * double(sY[(k + i * NV) * T + cc]);
+      stOut(&gddq.p[(int64_t)k * gddq.rs + (b0 + cc) * gddq.bs], S(a));
     }
   }
-  // G = L L^T, then L L^T lambda = rhs
-#pragma unroll 1
-  for (int i = 0; i < nc; ++i) {
-#pragma unroll 1
-    for (int d = 0; d <= i; ++d) {
-      double s = G[gi(i, d)];
-#pragma unroll 1
-      for (int k = 0; k < d; ++k) s -= G[gi(i, k)] * G[gi(d, k)];
-      G[gi(i, d)] = d == i ? sqrt(s) : s / G[gi(d, d)];
-    }
-  }
-#pragma unroll 1
-  for (int i = 0; i < nc; ++i) {
-    double s = lam[i * kTile];
-#pragma unroll 1
-    for (int k = 0; k < i; ++k) s -= G[gi(i, k)] * lam[k * kTile];
-    lam[i * kTile] = s / G[gi(i, i)];
-  }
-#pragma unroll 1
-  for (int i = nc - 1; i >= 0; --i) {
-    double s = lam[i * kTile];
-#pragma unroll 1
-    for (int k = i + 1; k < nc; ++k) s -= G[gi(k, i)] * lam[k * kTile];
-    lam[i * kTile] = s / G[gi(i, i)];
-  }
-  // ddq = r + Y lambda
-#pragma unroll 1
-  for (int i = 0; i < nc; ++i) {
-    const double li = lam[i * kTile];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) r[k] += li * yv(i, k);
-    stOut(&glam.p[(int64_t)i * glam.rs + b * glam.bs], S(li));
-  }
-#pragma unroll
-  for (int k = 0; k < NV; ++k) stOut(&gddq.p[(int64_t)k * gddq.rs + b * gddq.bs], S(r[k]));
 }
 
 }  // namespace rbd

@@ -691,13 +691,12 @@ void cfdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
   const S* dg = (const S*)st.in(gamma, vg.extent * sizeof(S));
   S* da = (S*)st.out(ddq, va.extent * sizeof(S), ldddq != nv || batchLayout);
   S* dl = (S*)st.out(lambda, vl.extent * sizeof(S), ldl != nc || batchLayout);
-  // workspace, batch-fastest: U, D, Y (FP64), then M, nle and one zero (S)
-  const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B, nY = (size_t)(nc + 1) * nv * B;
+  // workspace, batch-fastest: U, D (FP64), then M, nle and one zero (S)
+  const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
   const size_t nM = (size_t)nv * nv * B, nN = (size_t)nv * B;
-  double* wU = (double*)ctx->workspace((nU + nD + nY) * 8 + (nM + nN + 1) * sizeof(S));
+  double* wU = (double*)ctx->workspace((nU + nD) * 8 + (nM + nN + 1) * sizeof(S));
   double* wD = wU + nU;
-  double* wY = wD + nD;
-  S* wM = (S*)(wY + nY);
+  S* wM = (S*)(wD + nD);
   S* wN = wM + nM;
   S* wZ = wN + nN;
   const S* mc = f32 ? (const S*)m->mc32 : (const S*)m->mc64;
@@ -716,8 +715,8 @@ void cfdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
   else o.rnea_f64(ctx->stream, mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
   const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
   kkt(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
-      CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{dg, vg.rs, vg.bs}, damping, View<double>{wY, B, 1},
-      View<S>{da, va.rs, va.bs}, View<S>{dl, vl.rs, vl.bs});
+      CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{dg, vg.rs, vg.bs}, damping, View<S>{da, va.rs, va.bs},
+      View<S>{dl, vl.rs, vl.bs});
   finishCall(ctx, flags, st);
 }
 }  // namespace
