@@ -5,7 +5,8 @@
 // its outputs against the oracle.
 //
 //   dropin_check DIR   reads DIR/{dims.txt, parents.i32, jtype.i32, placement.f64,
-//                      inertia.f64, q.f64, v.f64, a.f64, tau.f64}, writes DIR/out_*.f64
+//                      inertia.f64, q.f64, v.f64, a.f64, tau.f64, J.f64, gamma.f64},
+//                      writes DIR/out_*.f64
 #include <cstdio>
 #include <fstream>
 #include <stdexcept>
@@ -40,11 +41,11 @@ int main(int argc, char** argv) {
     return 2;
   }
   const std::string dir = argv[1];
-  int nj = 0, nq = 0, nv = 0;
+  int nj = 0, nq = 0, nv = 0, nc = 0;
   long B = 0;
   {
     std::ifstream d(dir + "/dims.txt");
-    d >> nj >> nq >> nv >> B;
+    d >> nj >> nq >> nv >> B >> nc;
   }
   auto parents = readRaw<int32_t>(dir + "/parents.i32", nj);
   auto jtype = readRaw<int32_t>(dir + "/jtype.i32", nj);
@@ -80,6 +81,15 @@ int main(int argc, char** argv) {
     writeRaw(dir + "/out_minv.f64", d3);
     pinocchio_amd::computeMinverseInParallel(threads, pool, Q, o1);
     writeRaw(dir + "/out_minv64.f64", d1);
+    // constrained forwardDynamics: J is nc x (nv B), the configurations' blocks side by side
+    auto J = readRaw<double>(dir + "/J.f64", (size_t)nc * nv * B);
+    auto gamma = readRaw<double>(dir + "/gamma.f64", (size_t)nc * B);
+    std::vector<double> cddq((size_t)nv * B), lam((size_t)nc * B);
+    pinocchio_amd::forwardDynamicsInParallel(threads, pool, Q, V, T, CMat{J.data(), nc, nv * B, nc},
+                                             CMat{gamma.data(), nc, B, nc}, 0.0, MMat{cddq.data(), nv, B, nv},
+                                             MMat{lam.data(), nc, B, nc});
+    writeRaw(dir + "/out_cddq.f64", cddq);
+    writeRaw(dir + "/out_lambda.f64", lam);
   }
   // the reference's argument errors (algorithm/parallel/rnea.hpp:53-68, aba.hpp:54-69)
   const bool e1 = throwsInvalid([&] { pinocchio_amd::rneaInParallel(threads + 1, pool, Q, V, A, MMat{tau_out.data(), nv, B, nv}); });

@@ -1,10 +1,12 @@
 """The C++ drop-in (include/pinocchio_amd/parallel.hpp) driven like a pinocchio caller:
 tests/cpp/dropin_check builds a ModelPool from a flattened model, runs rneaInParallel,
-abaInParallel, crbaInParallel, computeRNEADerivativesInParallel and
-computeABADerivativesInParallel and computeMinverseInParallel on column-major host matrices, and checks the reference's
-argument errors.  Outputs against the FP64 oracle: FP64 RNEA and CRBA 1e-12, FP32 ABA
-1e-4, FP32 RNEA derivatives 1e-5, FP32 ABA derivatives 1e-4, FP64 Minv 1e-10 (per configuration, inf-norm
-relative; oracle on the FP32-rounded inputs for the FP32 paths).
+abaInParallel, crbaInParallel, computeRNEADerivativesInParallel,
+computeABADerivativesInParallel, computeMinverseInParallel and forwardDynamicsInParallel
+(constrained) on column-major host matrices, and checks the reference's argument errors.
+Outputs against the FP64 oracle: FP64 RNEA and CRBA 1e-12, FP32 ABA 1e-4, FP32 RNEA
+derivatives 1e-5, FP32 ABA derivatives 1e-4, FP64 Minv 1e-10, FP64 constrained ddq and
+lambda 1e-8 with a random J (per configuration, inf-norm relative; oracle on the
+FP32-rounded inputs for the FP32 paths).
 """
 import os
 import subprocess
@@ -34,8 +36,13 @@ def test_cpp_dropin(tmp_path, name):
     B = 300
     q, v, a, tau = inputs(desc, B, k=60)
     d = str(tmp_path)
+    nc = 6 if desc.nv > 12 else 3  # random J: the error grows with the conditioning of J M^-1 J^T
+    J = np.random.default_rng(61).standard_normal((B, nc, desc.nv))
+    gamma = np.random.default_rng(62).standard_normal((nc, B))
+    J.transpose(0, 2, 1).astype(np.float64).tofile(os.path.join(d, "J.f64"))  # per configuration column major
+    np.asfortranarray(gamma).T.astype(np.float64).tofile(os.path.join(d, "gamma.f64"))
     with open(os.path.join(d, "dims.txt"), "w") as f:
-        f.write(f"{desc.njoints} {desc.nq} {desc.nv} {B}\n")
+        f.write(f"{desc.njoints} {desc.nq} {desc.nv} {B} {nc}\n")
     desc.parents.astype(np.int32).tofile(os.path.join(d, "parents.i32"))
     desc.jtype.astype(np.int32).tofile(os.path.join(d, "jtype.i32"))
     desc.placement.astype(np.float64).tofile(os.path.join(d, "placement.f64"))
@@ -64,3 +71,7 @@ def test_cpp_dropin(tmp_path, name):
         assert rel_err_mats(blocks(np.fromfile(os.path.join(d, f"out_{n}.f64")), nv, B), ref).max() < 1e-4
     _, _, rmi64, _ = oracle.aba_derivatives(desc, q, v, tau)
     assert rel_err_mats(blocks(np.fromfile(os.path.join(d, "out_minv64.f64")), nv, B), rmi64).max() < 1e-10
+    ca, cl = oracle.constrained_forward_dynamics(desc, q, v, tau, J, gamma)
+    cddq = np.fromfile(os.path.join(d, "out_cddq.f64")).reshape(B, nv).T
+    clam = np.fromfile(os.path.join(d, "out_lambda.f64")).reshape(B, nc).T
+    assert rel_err_cols(cddq, ca).max() < 1e-8 and rel_err_cols(clam, cl).max() < 1e-8
