@@ -378,21 +378,23 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
   // A compiler memory clobber at every elimination step keeps the factor reads where they
   // are: without it they are hoisted out of the job loop and all 337 values stay live.
   const S* su = sU + L;
+  // U^-1 runs by columns: once x[jc] is final it updates its ancestors' entries
+  // (independent multiply-adds, where the reference's row dot products chain).
   auto solve2 = [&](S (&x)[NV], S (&y)[NV]) {
-    static_for<0, NV>([&](auto KK) {
-      constexpr int k = NV - 1 - KK;
+    static_for<0, NV>([&](auto JJ) {
+      constexpr int jc = NV - 1 - JJ;
       stepFence();
       asm volatile("" ::: "memory");
-      static_for<k + 1, NV>([&](auto EE) {
-        constexpr int e2 = EE;
-        if constexpr (P::anc(k, e2)) {
-          constexpr int pk = P::packed(k, e2);
+      static_for<0, jc>([&](auto II) {
+        constexpr int i = II;
+        if constexpr (P::anc(i, jc)) {
+          constexpr int pk = P::packed(i, jc);
           const S u = su[pk * kTile];
-          x[k] -= u * x[e2];
-          y[k] -= u * y[e2];
+          x[i] -= u * x[jc];
+          y[i] -= u * y[jc];
         }
       });
-      pin(x[k], y[k]);
+      pin(x[jc], y[jc]);
     });
     asm volatile("" ::: "memory");
 #pragma unroll

@@ -122,18 +122,20 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
       for (int k = 0; k < NV; ++k)
         x[k] = double(gtau.p[(int64_t)k * gtau.rs + b * gtau.bs]) - double(gnle.p[(int64_t)k * gnle.rs + b * gnle.bs]);
     }
-    // x <- M^-1 x: Uiv, D^-1, Utiv (cholesky.hxx:193-317, 466-520).  A compiler memory
-    // clobber at every elimination step keeps each factor read next to its use.
+    // x <- M^-1 x: Uiv, D^-1, Utiv (cholesky.hxx:193-317, 466-520).  U^-1 runs by columns:
+    // once x[jc] is final it updates its ancestors' entries, independent multiply-adds
+    // instead of the reference's row dot products (one dependent chain per row).  A
+    // compiler memory clobber at every step keeps each factor read next to its use.
     const double* su = sF + c;
-    static_for<0, NV>([&](auto KK) {
-      constexpr int k = NV - 1 - KK;
+    static_for<0, NV>([&](auto JJ) {
+      constexpr int jc = NV - 1 - JJ;
       stepFence();
       asm volatile("" ::: "memory");
-      static_for<k + 1, NV>([&](auto EE) {
-        constexpr int e2 = EE;
-        if constexpr (P::anc(k, e2)) {
-          constexpr int pk = P::packed(k, e2);  // (forced to a compile-time constant)
-          x[k] -= su[pk * T] * x[e2];
+      static_for<0, jc>([&](auto II) {
+        constexpr int i = II;
+        if constexpr (P::anc(i, jc)) {
+          constexpr int pk = P::packed(i, jc);  // (forced to a compile-time constant)
+          x[i] -= su[pk * T] * x[jc];
         }
       });
     });
