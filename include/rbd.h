@@ -18,6 +18,8 @@
  *   rbd_compute_minverse  computeMinverse           algorithm/aba.hxx:608-902
  *   rbd_forward_kinematics forwardKinematics        algorithm/kinematics.hxx:121-145, 209-238
  *   rbd_joint_jacobians   computeJointJacobians     algorithm/jacobian.hxx:23-62
+ *   rbd_frames            updateFramePlacements,    algorithm/frames.hxx:15-35,
+ *                         getFrameJacobian          algorithm/frames.hxx:156-175
  *   rbd_constrained_forward_dynamics
  *                         forwardDynamics(q, v, tau, J, gamma, inv_damping)
  *                                                   algorithm/contact-dynamics.hxx:90-167
@@ -167,6 +169,20 @@ int rbd_joint_jacobians(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int
  * FP32: CRBA, FP32 tree factor, unit right-hand sides; FP64: the same in double. */
 int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
                          void* Minv, int64_t ldMinv, uint32_t flags);
+
+/* Frames (operational points): for each frame f given by its parent joint frame_joint[f]
+ * and its placement in that joint's frame (frame_placement[12 f ..]: R row major, then the
+ * translation), the placement oMf = oMi[parent] * placement (updateFramePlacements,
+ * frames.hxx:15-35; 12 rows per frame, element 12 f + k) and the 6 x nv frame Jacobian
+ * (getFrameJacobian, frames.hxx:156-175): the support columns of the parent joint's WORLD
+ * Jacobian in the requested reference frame (RBD_WORLD as they are; RBD_LOCAL_WORLD_ALIGNED
+ * with the linear part taken at the frame origin; RBD_LOCAL in the frame's axes), zero
+ * elsewhere (element f 6 nv + r + 6 c).  oMf or J may be NULL.  frame_joint and
+ * frame_placement are host arrays (the call copies them and always synchronises). */
+enum { RBD_WORLD = 0, RBD_LOCAL = 1, RBD_LOCAL_WORLD_ALIGNED = 2 };
+int rbd_frames(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+               int nframes, const int32_t* frame_joint, const double* frame_placement, int reference_frame,
+               void* oMf, int64_t ldoMf, void* J, int64_t ldJ, uint32_t flags);
 
 /* forwardDynamics with acceleration constraints J ddq + gamma = 0 (contact-dynamics.hpp:
  * 66-75, contact-dynamics.hxx:90-167): M(q) and nle(q, v) as computeAllTerms gives them,

@@ -112,6 +112,25 @@ def joint_jacobians(model, q, scalar="f64", nthreads=0):
     return J.transpose(0, 2, 1).copy()
 
 
+REFERENCE_FRAMES = {"world": 0, "local": 1, "local_world_aligned": 2}
+
+
+def frames(model, q, parents, placements, reference_frame="world", scalar="f64", nthreads=0):
+    """updateFramePlacements + getFrameJacobian per column for frames given as (parent joint,
+    placement [R row-major 9, p 3]).  Returns oMf (B, nf, 12) and J (B, nf, 6, nv)."""
+    q = _cols(q, model.nq)
+    B = q.shape[1]
+    par = np.ascontiguousarray(parents, dtype=np.int32)
+    pl = np.ascontiguousarray(placements, dtype=np.float64).reshape(len(par), 12)
+    nf = len(par)
+    oMf = np.zeros((B, nf, 12))
+    J = np.zeros((B, nf, model.nv, 6))
+    d = model.cstruct()
+    _check(lib().orc_frames(ctypes.byref(d), ctypes.c_int64(B), _p(q), int(nf), _p(par), _p(pl),
+                            REFERENCE_FRAMES[reference_frame], _p(oMf), _p(J), SCALARS[scalar], nthreads))
+    return oMf, J.transpose(0, 1, 3, 2).copy()
+
+
 def aba(model, q, v, tau, convention="world", scalar="f64", nthreads=0):
     q, v, tau = _cols(q, model.nq), _cols(v, model.nv), _cols(tau, model.nv)
     B = q.shape[1]

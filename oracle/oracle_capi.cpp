@@ -188,6 +188,30 @@ template<class S> int jacT(const ModelDesc* d, int64_t B, const double* q, doubl
     storeMat(data.J, J + b * 6 * m.nv);
   });
 }
+template<class S>
+int framesT(const ModelDesc* d, int64_t B, const double* q, int nf, const int32_t* parent, const double* placement,
+            int rf, double* oMf, double* J, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq;
+    static thread_local std::vector<S> Jf;
+    computeJointJacobians(m, data, bq.load(q + b * m.nq, m.nq));
+    Jf.resize((size_t)6 * m.nv);
+    for (int f = 0; f < nf; ++f) {
+      if (parent[f] < 0 || parent[f] >= m.njoints) throw std::invalid_argument("frame parent out of range");
+      SE3<S> X;
+      for (int k = 0; k < 9; ++k) X.R.m[k] = S(placement[12 * f + k]);
+      for (int k = 0; k < 3; ++k) X.p[k] = S(placement[12 * f + 9 + k]);
+      SE3<S> oMfr;
+      frameJacobian(m, data, parent[f], X, rf, oMfr, Jf.data());
+      if (oMf) {
+        double* o = oMf + (b * nf + f) * 12;
+        for (int k = 0; k < 9; ++k) o[k] = double(oMfr.R.m[k]);
+        for (int k = 0; k < 3; ++k) o[9 + k] = double(oMfr.p[k]);
+      }
+      if (J) store(Jf.data(), J + (b * nf + f) * 6 * m.nv, 6 * m.nv);
+    }
+  });
+}
 }  // namespace
 
 extern "C" {
@@ -244,6 +268,19 @@ int orc_rnea(const ModelDesc* d, int64_t B, const double* q, const double* v, co
 int orc_forward_kinematics(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a,
                            double* oM, double* vout, double* aout, int scalar, int nthreads) {
 #define CALL(S) fkT<S>(d, B, q, v, a, oM, vout, aout, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_frames(const ModelDesc* d, int64_t B, const double* q, int nf, const int32_t* parent,
+               const double* placement, int rf, double* oMf, double* J, int scalar, int nthreads) {
+#define CALL(S) framesT<S>(d, B, q, nf, parent, placement, rf, oMf, J, nthreads)
   switch (scalar) {
     case 0: return CALL(double);
     case 1: return CALL(float);

@@ -926,6 +926,29 @@ template<class S> inline void choleskyComputeMinv(const Model<S>& model, const D
 }
 
 // ----------------------------------------------------------------------------------------
+// Frames: updateFramePlacements (frames.hxx:15-35), oMf = oMi[parent] * placement, and
+// getFrameJacobian (frames.hxx:156-175 with translateJointJacobian, jacobian.hxx:175-300):
+// the support columns of the parent joint's WORLD Jacobian, expressed in WORLD (as they
+// are), LOCAL_WORLD_ALIGNED (linear part moved to the frame origin: lin -= p x ang) or LOCAL
+// (oMf^-1 applied); the other columns are zero.  Needs computeJointJacobians first.
+// ----------------------------------------------------------------------------------------
+enum { RF_WORLD = 0, RF_LOCAL = 1, RF_LOCAL_WORLD_ALIGNED = 2 };
+template<class S>
+inline void frameJacobian(const Model<S>& model, const Data<S>& data, int parent, const SE3<S>& placement, int rf,
+                          SE3<S>& oMf, S* J) {
+  oMf = data.oMi[parent] * placement;
+  std::fill(J, J + 6 * model.nv, S(0));
+  for (int j = parent; j > 0; j = model.parents[j])
+    for (int k = 0; k < model.nvs[j]; ++k) {
+      const int c = model.idx_v[j] + k;
+      Motion<S> m = colM(data.J, c);
+      if (rf == RF_LOCAL_WORLD_ALIGNED) m.lin = m.lin - cross(oMf.p, m.ang);
+      else if (rf == RF_LOCAL) m = actInv(oMf, m);
+      toVec(m, J + 6 * c);
+    }
+}
+
+// ----------------------------------------------------------------------------------------
 // forwardDynamics with acceleration constraints J ddq + gamma = 0
 // (contact-dynamics.hxx:90-167): computeAllTerms gives M (CRBA) and nle (RNEA at a = 0),
 // then cholesky::decompose and

@@ -309,6 +309,28 @@ class Engine:
                                          _i64(ldj), ctypes.c_uint32(self._flags(async_))))
         return J
 
+    REFERENCE_FRAMES = {"world": 0, "local": 1, "local_world_aligned": 2}
+
+    def frames(self, q, frame_joint, frame_placement, reference_frame="world", placements=True, jacobians=True):
+        """Frame placements oMf (12 rows per frame: R row major, p) and 6 x nv frame Jacobians
+        (rows f 6 nv + r + 6 c) for frames given by parent joint and placement (12 numbers
+        each) -- updateFramePlacements and getFrameJacobian per configuration.  Synchronous."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        fj = np.ascontiguousarray(frame_joint, dtype=np.int32).ravel()
+        nf = len(fj)
+        fp = np.ascontiguousarray(frame_placement, dtype=np.float64).reshape(nf, 12)
+        oMf = self._new_vec(q, 12 * nf, B) if placements else None
+        J = self._new_vec(q, 6 * m.nv * nf, B) if jacobians else None
+        ldo = self._vec(oMf, 12 * nf, "oMf")[1] if placements else 0
+        ldj = self._vec(J, 6 * m.nv * nf, "J")[1] if jacobians else 0
+        _check(lib().rbd_frames(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), int(nf),
+                                fj.ctypes.data_as(ctypes.c_void_p), fp.ctypes.data_as(ctypes.c_void_p),
+                                int(self.REFERENCE_FRAMES[reference_frame]), _ptr(oMf) if placements else None,
+                                _i64(ldo), _ptr(J) if jacobians else None, _i64(ldj),
+                                ctypes.c_uint32(self._flags(False))))
+        return oMf, J
+
     def compute_minverse(self, q, out=None, async_=False):
         """M(q)^-1 per configuration (computeMinverse), FP32 or FP64 like q."""
         m = self.model

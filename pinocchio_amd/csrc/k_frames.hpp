@@ -1,0 +1,91 @@
+// k_frames.hpp -- frame placements and frame Jacobians from the joint placements oMi and
+// the WORLD joint Jacobian that k_kin writes:
+//   oMf = oMi[parent] * placement                       (updateFramePlacements, frames.hxx:15-35)
+//   J_f = support columns of the WORLD Jacobian, in     (getFrameJacobian, frames.hxx:156-175;
+//         WORLD, LOCAL_WORLD_ALIGNED or LOCAL            translateJointJacobian, jacobian.hxx:175-300)
+// One thread per (configuration, frame): blockIdx.y is the frame, threads run along the
+// batch so every load and store is coalesced in the batch-fastest workspace.
+#pragma once
+
+#include "kernels_common.hpp"
+
+namespace rbd {
+
+enum { kRfWorld = 0, kRfLocal = 1, kRfLocalWorldAligned = 2 };
+
+// oMi: 12 (j - 1) + k rows (R row major, then p); Jw: r + 6 c rows; frame tables: parent
+// joint and placement (12 per frame, S); support: njoints x nv flags (1 where column c is a
+// dof of joint j or of one of its ancestors).  Outputs: oMf 12 f + k rows, Jf f 6 nv + r + 6 c
+// rows; either may be absent.
+template<class S>
+__global__ __launch_bounds__(256) void k_frames(int64_t B, int nv, const int32_t* __restrict__ fpar,
+                                                const S* __restrict__ fpl, const uint8_t* __restrict__ support,
+                                                CView<S> oMi, CView<S> Jw, int rf, View<S> oMf, View<S> Jf) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (b >= B) return;
+  const int p = fpar[f];
+  const S* X = fpl + 12 * f;
+  S R[9], t[3];
+  if (p > 0) {
+    S Ri[9], ti[3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ri[k] = oMi.p[(int64_t)(12 * (p - 1) + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ti[k] = oMi.p[(int64_t)(12 * (p - 1) + 9 + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) R[3 * r + c] = Ri[3 * r] * X[c] + Ri[3 * r + 1] * X[3 + c] + Ri[3 * r + 2] * X[6 + c];
+      t[r] = Ri[3 * r] * X[9] + Ri[3 * r + 1] * X[10] + Ri[3 * r + 2] * X[11] + ti[r];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = X[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] = X[9 + k];
+  }
+  if (oMf.p) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) oMf.p[(int64_t)(12 * f + k) * oMf.rs + b * oMf.bs] = R[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) oMf.p[(int64_t)(12 * f + 9 + k) * oMf.rs + b * oMf.bs] = t[k];
+  }
+  if (!Jf.p) return;
+  const uint8_t* sup = support + (int64_t)p * nv;
+  const int64_t base = (int64_t)f * 6 * nv;
+  for (int c = 0; c < nv; ++c) {
+    S o[6] = {S(0), S(0), S(0), S(0), S(0), S(0)};
+    if (sup[c]) {
+      S l[3], a[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        l[k] = Jw.p[(int64_t)(6 * c + k) * Jw.rs + b * Jw.bs];
+        a[k] = Jw.p[(int64_t)(6 * c + 3 + k) * Jw.rs + b * Jw.bs];
+      }
+      if (rf != kRfWorld) {  // linear part at the frame origin: l - t x a
+        const S cx = t[1] * a[2] - t[2] * a[1], cy = t[2] * a[0] - t[0] * a[2], cz = t[0] * a[1] - t[1] * a[0];
+        l[0] -= cx;
+        l[1] -= cy;
+        l[2] -= cz;
+      }
+      if (rf == kRfLocal) {  // then rotated into the frame: R^T
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          o[k] = R[k] * l[0] + R[3 + k] * l[1] + R[6 + k] * l[2];
+          o[3 + k] = R[k] * a[0] + R[3 + k] * a[1] + R[6 + k] * a[2];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          o[k] = l[k];
+          o[3 + k] = a[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) Jf.p[(base + 6 * c + r) * Jf.rs + b * Jf.bs] = o[r];
+  }
+}
+
+}  // namespace rbd

@@ -16,6 +16,7 @@
 
 #include "../../include/rbd.h"
 #include "k_dyn.hpp"
+#include "k_frames.hpp"
 #include "k_kkt.hpp"
 #include "ops.hpp"
 #include "runtime.hpp"
@@ -410,10 +411,13 @@ void kinCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
   const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
   const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
   const S* da = (const S*)st.in(a, va.extent * sizeof(S));
-  S* dM = (S*)st.out(oM, vM.extent * sizeof(S), false);
-  S* dvo = (S*)st.out(vo, vva.extent * sizeof(S), false);
-  S* dao = (S*)st.out(ao, vva.extent * sizeof(S), false);
-  S* dJ = (S*)st.out(J, vJ.extent * sizeof(S), false);
+  // host outputs with gaps between columns (ld > rows, or the batch-fastest layout) are
+  // preloaded so the copy back leaves the gaps as the caller had them
+  const bool bl = (flags & RBD_LAYOUT_BATCH) != 0;
+  S* dM = (S*)st.out(oM, vM.extent * sizeof(S), bl || ldo != 12 * (nj - 1));
+  S* dvo = (S*)st.out(vo, vva.extent * sizeof(S), bl || ldva != 6 * (nj - 1));
+  S* dao = (S*)st.out(ao, vva.extent * sizeof(S), bl || ldva != 6 * (nj - 1));
+  S* dJ = (S*)st.out(J, vJ.extent * sizeof(S), bl || ldJ != 6 * nv);
   const S* mc;
   if constexpr (f32) mc = m->mc32;
   else mc = m->mc64;
@@ -660,6 +664,60 @@ int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
 }  // extern "C"
 
 namespace {
+// frame placements and Jacobians: k_kin's oMi and WORLD Jacobian into the workspace, then
+// k_frames.  The frame tables are host arrays, copied during the call (which therefore
+// always synchronises).
+template<class S>
+void framesCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, int nf,
+                const int32_t* fjoint, const double* fplace, int rf, void* oMf, int64_t ldo, void* J, int64_t ldJ,
+                uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const auto launch = f32 ? (void*)m->ops->kin_f32 : (void*)m->ops->kin_f64;
+  if (!launch) throw Error(RBD_EUNSUPPORTED, "frames: no kinematics kernel for this topology");
+  const int nj = m->njoints, nv = m->nv;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  ArgView<S> vM{}, vJ{};
+  if (oMf) vM = argView<S>(12 * nf, B, ldo, flags, "oMf");
+  if (J) vJ = argView<S>(6 * nv * nf, B, ldJ, flags, "J");
+  if (B == 0 || nf == 0 || (!oMf && !J)) return;
+  // tables: parent joint, placement (in S), support flags of every joint
+  std::vector<S> pl((size_t)12 * nf);
+  for (size_t k = 0; k < pl.size(); ++k) pl[k] = S(fplace[k]);
+  std::vector<uint8_t> sup((size_t)nj * nv, 0);
+  for (int j = 1; j < nj; ++j)
+    for (int a = j; a > 0; a = m->parents[a]) {
+      const int nva = m->jtype[a] == RBD_JOINT_FREEFLYER ? 6 : 1;
+      for (int k = 0; k < nva; ++k) sup[(size_t)j * nv + m->iv[a] + k] = 1;
+    }
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const bool bl = (flags & RBD_LAYOUT_BATCH) != 0;
+  S* dM = (S*)st.out(oMf, vM.extent * sizeof(S), bl || ldo != 12 * nf);
+  S* dJ = (S*)st.out(J, vJ.extent * sizeof(S), bl || ldJ != 6 * nv * nf);
+  auto table = [&](const void* h, size_t bytes) {
+    void* d = ctx->stageBuf(st.slot++, bytes);
+    HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return d;
+  };
+  const int32_t* dpar = (const int32_t*)table(fjoint, sizeof(int32_t) * nf);
+  const S* dpl = (const S*)table(pl.data(), sizeof(S) * pl.size());
+  const uint8_t* dsup = (const uint8_t*)table(sup.data(), sup.size());
+  // workspace, batch-fastest: oMi (12 (nj - 1) rows), then the WORLD Jacobian (6 nv rows)
+  const size_t nO = (size_t)12 * (nj - 1) * B, nJ = J ? (size_t)6 * nv * B : 0;
+  S* wO = (S*)ctx->workspace((nO + nJ) * sizeof(S));
+  S* wJ = J ? wO + nO : nullptr;
+  const S* mc;
+  if constexpr (f32) mc = m->mc32;
+  else mc = m->mc64;
+  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, mc, B, CView<S>{dq, vq.rs, vq.bs}, CView<S>{}, CView<S>{},
+                                        View<S>{wO, B, 1}, View<S>{}, View<S>{}, View<S>{wJ, B, 1});
+  hipLaunchKernelGGL(k_frames<S>, dim3((unsigned)((B + 255) / 256), (unsigned)nf), dim3(256), 0, ctx->stream, B, nv,
+                     dpar, dpl, dsup, CView<S>{wO, B, 1}, CView<S>{wJ, B, 1}, rf, View<S>{dM, vM.rs, vM.bs},
+                     View<S>{dJ, vJ.rs, vJ.bs});
+  finishCall(ctx, flags & ~(uint32_t)RBD_ASYNC, st);
+}
+
 template<class S> __global__ void k_fill_zero(S* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = S(0);
@@ -722,6 +780,27 @@ void cfdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
 }  // namespace
 
 extern "C" {
+
+int rbd_frames(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, int nframes,
+               const int32_t* frame_joint, const double* frame_placement, int reference_frame, void* oMf,
+               int64_t ldoMf, void* J, int64_t ldJ, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(nframes >= 0 && nframes <= 65535, "the frame count must be between 0 and 65535");
+    RBD_CHECK_ARG(nframes == 0 || (frame_joint && frame_placement), "null frame table");
+    RBD_CHECK_ARG(reference_frame >= RBD_WORLD && reference_frame <= RBD_LOCAL_WORLD_ALIGNED,
+                  "reference_frame must be RBD_WORLD, RBD_LOCAL or RBD_LOCAL_WORLD_ALIGNED");
+    for (int f = 0; f < nframes; ++f)
+      RBD_CHECK_ARG(frame_joint[f] >= 0 && frame_joint[f] < m->njoints, "The index of the Joint is outside the bounds.");
+    RBD_CHECK_ARG(B == 0 || nframes == 0 || q, "null array argument");
+    if (dtype == RBD_F32)
+      framesCall<float>(ctx, m, B, q, ldq, nframes, frame_joint, frame_placement, reference_frame, oMf, ldoMf, J, ldJ,
+                        flags);
+    else
+      framesCall<double>(ctx, m, B, q, ldq, nframes, frame_joint, frame_placement, reference_frame, oMf, ldoMf, J,
+                         ldJ, flags);
+  });
+}
 
 int rbd_constrained_forward_dynamics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q,
                                      int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, int nc,

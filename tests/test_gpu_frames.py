@@ -1,0 +1,76 @@
+"""GPU parity of frame placements and frame Jacobians (updateFramePlacements,
+frames.hxx:15-35; getFrameJacobian, frames.hxx:156-175) against the FP64 oracle, through
+the C ABI, in the three reference frames and both layouts.  Frames sit on random joints
+(one on the universe) with random placements.
+Bars (max-abs relative to the oracle's max-abs, per configuration): FP64 1e-12, FP32 1e-5
+(oracle on the FP32-rounded inputs).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import pinocchio_amd as pa
+from pinocchio_amd import models as M
+from helpers import f32_round
+from test_oracle_frames import random_frames
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": 1e-5, "f64": 1e-12}
+_models = {}
+
+
+def gpu_model(ctx, name):
+    if name not in _models:
+        desc = getattr(M, name)()
+        _models[name] = (desc, pa.Model(ctx, desc))
+    return _models[name]
+
+
+def rel(x, ref):
+    d = np.abs(x - ref).reshape(len(ref), -1).max(axis=1)
+    return float((d / np.maximum(np.abs(ref).reshape(len(ref), -1).max(axis=1), 1e-30)).max())
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("rf", ["world", "local", "local_world_aligned"])
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18"])
+def test_frames_parity(gpu_ctx, name, rf, layout, prec):
+    desc, model = gpu_model(gpu_ctx, name)
+    B, nf = 301, 5
+    q, _, _, _ = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 880)
+    par, pl = random_frames(desc, nf, 3)
+    if prec == "f32":
+        (q,) = f32_round(q)
+        pl = np.asarray(pl, dtype=np.float32).astype(np.float64)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    eng = pa.Engine(model, layout)
+    Q = torch.tensor(np.ascontiguousarray(q.T if layout == "col" else q), dtype=dt, device="cuda")
+    oMf, J = eng.frames(Q, par, pl, rf)
+    torch.cuda.synchronize()
+    oMf = oMf.double().cpu().numpy()
+    J = J.double().cpu().numpy()
+    if layout == "batch":
+        oMf, J = oMf.T, J.T
+    oMf = oMf.reshape(B, nf, 12)
+    J = J.reshape(B, nf, desc.nv, 6).transpose(0, 1, 3, 2)
+    ref_o, ref_J = oracle.frames(desc, q, par, pl, rf)
+    eo, ej = rel(oMf, ref_o), rel(J, ref_J)
+    print(f"{name} {rf} {layout} {prec}: oMf {eo:.2e} J {ej:.2e}")
+    assert eo < TOL[prec] and ej < TOL[prec]
+
+
+def test_frames_arguments(gpu_ctx):
+    desc, model = gpu_model(gpu_ctx, "arm7")
+    eng = pa.Engine(model, "batch")
+    Q = torch.zeros((desc.nq, 64), device="cuda")
+    pl = np.zeros((1, 12))
+    with pytest.raises(ValueError, match="outside the bounds"):
+        eng.frames(Q, [desc.njoints], pl)
+    oMf, J = eng.frames(Q, [0], np.concatenate([np.eye(3).ravel(), [1.0, 2.0, 3.0]])[None])
+    torch.cuda.synchronize()
+    o = oMf.cpu().numpy()
+    assert np.allclose(o[:9].T, np.eye(3).ravel()) and np.allclose(o[9:].T, [1.0, 2.0, 3.0])
+    assert float(J.abs().max()) == 0.0  # the universe supports no dof
