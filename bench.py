@@ -385,6 +385,20 @@ def main():
         "batch": B, "constraints": ncon, "configs_per_s": B / t, "ms": t * 1e3,
         "note": "CRBA, RNEA(a=0) in FP32; FP64 tree factor, KKT solve and LLT; J = both feet's 6D Jacobians"}
 
+    idq = torch.empty_like(cddq)
+    iimp = torch.empty_like(clam)
+    t = time_call(lambda: eng.impulse_dynamics(Q, V, Jrows, r_coeff=0.0, out=idq, impulse_out=iimp, async_=True),
+                  reps=10)
+    others["impulse_dynamics_humanoid36_f32"] = {"batch": B, "constraints": ncon, "configs_per_s": B / t,
+                                                 "ms": t * 1e3}
+    # frames: both feet and both hands (end joints) with a 5 cm offset, placements and
+    # LOCAL_WORLD_ALIGNED Jacobians (synchronous call: tables copied from the host)
+    fj = [7, 13, 22, 29]
+    fpl = np.tile(np.concatenate([np.eye(3).ravel(), [0.0, 0.0, -0.05]]), (len(fj), 1))
+    t = time_call(lambda: eng.frames(Q, fj, fpl, "local_world_aligned"), reps=10)
+    others["frames_humanoid36_f32"] = {"batch": B, "frames": len(fj), "configs_per_s": B / t, "ms": t * 1e3,
+                                       "GB_s": 4 * (desc.nq + len(fj) * (12 + 6 * nv)) * B / t / 1e9}
+
     # drop-in host path: numpy matrices in and out (PCIe both ways), not the device-resident figure
     pool = pa.ModelPool(desc, size=8)
     qh, vh, th = (np.asfortranarray(x) for x in (q, v, tau))

@@ -20,6 +20,8 @@
  *   rbd_joint_jacobians   computeJointJacobians     algorithm/jacobian.hxx:23-62
  *   rbd_frames            updateFramePlacements,    algorithm/frames.hxx:15-35,
  *                         getFrameJacobian          algorithm/frames.hxx:156-175
+ *   rbd_impulse_dynamics  impulseDynamics(q, v_before, J, r_coeff, inv_damping)
+ *                                                   algorithm/contact-dynamics.hxx:264-331
  *   rbd_constrained_forward_dynamics
  *                         forwardDynamics(q, v, tau, J, gamma, inv_damping)
  *                                                   algorithm/contact-dynamics.hxx:90-167
@@ -199,6 +201,16 @@ int rbd_constrained_forward_dynamics(rbd_ctx* ctx, const rbd_model* model, int64
                                      const void* J, int64_t ldJ, const void* gamma, int64_t ldgamma,
                                      double inv_damping, void* ddq, int64_t ldddq, void* lambda, int64_t ldlambda,
                                      uint32_t flags);
+
+/* impulseDynamics (contact-dynamics.hpp:241-300, contact-dynamics.hxx:264-331): the velocity
+ * after an impact, dq_after = v_before + M^-1 J^T impulse with
+ * impulse = (J M^-1 J^T + inv_damping I)^-1 (-(1 + r_coeff) J v_before), the reference's
+ * data.impulse_c.  J as in rbd_constrained_forward_dynamics (1 <= nc <= 12, nc nv rows
+ * per configuration); CRBA in the input precision, the factor and the rest in FP64. */
+int rbd_impulse_dynamics(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                         const void* v_before, int64_t ldv, int nc, const void* J, int64_t ldJ, double r_coeff,
+                         double inv_damping, void* dq_after, int64_t ldda, void* impulse, int64_t ldimpulse,
+                         uint32_t flags);
 
 /* Mixed-robot batch (no reference counterpart: the reference batches one model per
  * call).  Each group is `count` instances of one model, with FP32 device arrays in the

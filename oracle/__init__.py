@@ -251,6 +251,24 @@ def constrained_forward_dynamics(model, q, v, tau, J, gamma, inv_damping=0.0, sc
     return np.ascontiguousarray(ddq), np.ascontiguousarray(lam)
 
 
+def impulse_dynamics(model, q, v_before, J, r_coeff=0.0, inv_damping=0.0, scalar="f64", nthreads=0):
+    """impulseDynamics(model, data, q, v_before, J, r_coeff, inv_damping) per column
+    (contact-dynamics.hxx:264-331).  J: (B, nc, nv).  Returns (dq_after (nv, B), impulse (nc, B))."""
+    q, v = _cols(q, model.nq), _cols(v_before, model.nv)
+    B, nv = q.shape[1], model.nv
+    J = np.asarray(J, dtype=np.float64)
+    assert J.ndim == 3 and J.shape[0] == B and J.shape[2] == nv, J.shape
+    nc = J.shape[1]
+    Jc = np.ascontiguousarray(J.transpose(0, 2, 1))
+    dqa = np.zeros((nv, B), order="F")
+    imp = np.zeros((nc, B), order="F")
+    d = model.cstruct()
+    _check(lib().orc_impulse_dynamics(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(v), int(nc), _p(Jc),
+                                      ctypes.c_double(r_coeff), ctypes.c_double(inv_damping), _p(dqa), _p(imp),
+                                      SCALARS[scalar], nthreads))
+    return np.ascontiguousarray(dqa), np.ascontiguousarray(imp)
+
+
 def time_constrained(model, q, v, tau, J, gamma, inv_damping=0.0, nthreads=0, reps=5):
     """Median wall time (s) of the OpenMP batch loop of constrained forward dynamics (FP64)."""
     q, v, tau = _cols(q, model.nq), _cols(v, model.nv), _cols(tau, model.nv)

@@ -481,6 +481,42 @@ extern "C" int orc_constrained_forward_dynamics(const ModelDesc* d, int64_t B, c
   return 1;
 }
 
+namespace {
+template<class S>
+int impT(const ModelDesc* d, int64_t B, const double* q, const double* v, int nc, const double* J, double rc,
+         double damping, double* dqa, double* imp, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    if (!m.isCompact()) throw std::invalid_argument("impulse dynamics requires a compact (DFS-ordered) model");
+    static thread_local Buf<S> bq, bv, bJ;
+    static thread_local std::vector<S> a, l;
+    a.resize(m.nv);
+    l.resize(nc);
+    impulseDynamics(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * m.nv, m.nv), nc,
+                    bJ.load(J + b * nc * m.nv, nc * m.nv), S(rc), S(damping), a.data(), l.data());
+    store(a.data(), dqa + b * m.nv, m.nv);
+    store(l.data(), imp + b * nc, nc);
+  });
+}
+}  // namespace
+
+extern "C" int orc_impulse_dynamics(const ModelDesc* d, int64_t B, const double* q, const double* v, int nc,
+                                    const double* J, double r_coeff, double damping, double* dq_after,
+                                    double* impulse, int scalar, int nthreads) {
+  if (nc < 1) {
+    g_err = "nc must be positive";
+    return 1;
+  }
+#define CALL(S) impT<S>(d, B, q, v, nc, J, r_coeff, damping, dq_after, impulse, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
 // median wall time of the OpenMP batch loop of constrainedForwardDynamics (FP64)
 extern "C" int orc_time_constrained(const ModelDesc* d, int64_t B, const double* q, const double* v,
                                     const double* tau, int nc, const double* J, const double* gamma, double damping,

@@ -1018,6 +1018,66 @@ inline void constrainedForwardDynamics(const Model<S>& model, Data<S>& data, con
 }
 
 // ----------------------------------------------------------------------------------------
+// impulseDynamics (contact-dynamics.hxx:264-331): crba, cholesky::decompose, then
+//   sDUiJt = D^-1/2 U^-1 J^T,  JMinvJt = sDUiJt^T sDUiJt + inv_damping I
+//   impulse_c = JMinvJt^-1 (-(1 + r_coeff) J v_before)         (LLT)
+//   dq_after = M^-1 J^T impulse_c + v_before
+// ----------------------------------------------------------------------------------------
+template<class S>
+inline void impulseDynamics(const Model<S>& model, Data<S>& data, const S* q, const S* v_before, int nc, const S* J,
+                            S r_coeff, S inv_damping, S* dq_after, S* impulse) {
+  using std::sqrt;
+  const int nv = model.nv;
+  crbaWorld(model, data, q);
+  choleskyDecompose(model, data);
+  std::vector<S> sd((size_t)nv * nc), G((size_t)nc * nc), L((size_t)nc * nc, S(0));
+  for (int i = 0; i < nc; ++i) {
+    S* col = &sd[(size_t)i * nv];
+    for (int k = 0; k < nv; ++k) col[k] = J[i + (size_t)k * nc];
+    choleskyUiv(model, data, col);
+    for (int k = 0; k < nv; ++k) col[k] /= sqrt(data.D[k]);
+  }
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j < nc; ++j) {
+      S g = S(0);
+      for (int k = 0; k < nv; ++k) g += sd[(size_t)i * nv + k] * sd[(size_t)j * nv + k];
+      G[i + (size_t)j * nc] = i == j ? g + inv_damping : g;
+    }
+  for (int i = 0; i < nc; ++i) {
+    S s = S(0);
+    for (int k = 0; k < nv; ++k) s += J[i + (size_t)k * nc] * v_before[k];
+    impulse[i] = (-r_coeff - S(1)) * s;
+  }
+  for (int j = 0; j < nc; ++j) {
+    S d = G[j + (size_t)j * nc];
+    for (int k = 0; k < j; ++k) d -= L[j + (size_t)k * nc] * L[j + (size_t)k * nc];
+    L[j + (size_t)j * nc] = sqrt(d);
+    for (int i = j + 1; i < nc; ++i) {
+      S t = G[i + (size_t)j * nc];
+      for (int k = 0; k < j; ++k) t -= L[i + (size_t)k * nc] * L[j + (size_t)k * nc];
+      L[i + (size_t)j * nc] = t / L[j + (size_t)j * nc];
+    }
+  }
+  for (int i = 0; i < nc; ++i) {
+    S s = impulse[i];
+    for (int k = 0; k < i; ++k) s -= L[i + (size_t)k * nc] * impulse[k];
+    impulse[i] = s / L[i + (size_t)i * nc];
+  }
+  for (int i = nc - 1; i >= 0; --i) {
+    S s = impulse[i];
+    for (int k = i + 1; k < nc; ++k) s -= L[k + (size_t)i * nc] * impulse[k];
+    impulse[i] = s / L[i + (size_t)i * nc];
+  }
+  for (int k = 0; k < nv; ++k) {
+    S s = S(0);
+    for (int i = 0; i < nc; ++i) s += J[i + (size_t)k * nc] * impulse[i];
+    dq_after[k] = s;
+  }
+  choleskySolve(model, data, dq_after);
+  for (int k = 0; k < nv; ++k) dq_after[k] += v_before[k];
+}
+
+// ----------------------------------------------------------------------------------------
 // computeRNEADerivatives (rnea-derivatives.hxx:242-541)
 // ----------------------------------------------------------------------------------------
 template<class S>

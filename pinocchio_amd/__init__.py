@@ -368,6 +368,29 @@ class Engine:
             _i64(lda), _ptr(lambda_out), _i64(ldl), ctypes.c_uint32(self._flags(async_))))
         return out, lambda_out
 
+    def impulse_dynamics(self, q, v_before, J, r_coeff=0.0, inv_damping=0.0, out=None, impulse_out=None,
+                         async_=False):
+        """impulseDynamics(model, data, q, v_before, J, r_coeff, inv_damping) per configuration
+        (contact-dynamics.hxx:264-331): the velocity after impact and the impulses.  J as in
+        constrained_forward_dynamics.  Returns (dq_after, impulse)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v_before, m.nv, "v_before")
+        rows = J.shape[1] if self.layout == "col" else J.shape[0]
+        nc = rows // m.nv
+        if nc * m.nv != rows:
+            raise ValueError(f"J: {rows} rows per configuration is not a multiple of nv = {m.nv}")
+        _, ldJ = self._vec(J, nc * m.nv, "J")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        impulse_out = self._new_vec(q, nc, B) if impulse_out is None else impulse_out
+        _, lda = self._vec(out, m.nv, "dq_after")
+        _, ldi = self._vec(impulse_out, nc, "impulse")
+        _check(lib().rbd_impulse_dynamics(
+            m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v_before), _i64(ldv), int(nc),
+            _ptr(J), _i64(ldJ), ctypes.c_double(r_coeff), ctypes.c_double(inv_damping), _ptr(out), _i64(lda),
+            _ptr(impulse_out), _i64(ldi), ctypes.c_uint32(self._flags(async_))))
+        return out, impulse_out
+
     @staticmethod
     def mats_to_numpy(t, layout, nv):
         """Device matrix output -> (B, nv, nv) numpy array with [b] the configuration's matrix."""

@@ -42,10 +42,14 @@ template<class TP, class S> constexpr int kktLdsBytes() {
 }
 
 // J is nc x nv per configuration, column major (element (i, k) at row i + k nc of gJ).
+// impulse != 0: impulseDynamics (contact-dynamics.hxx:288-331) -- gtau holds v_before, which
+// is taken as r unsolved, the right-hand side is -(1 + rcoef) J v_before (gnle and gg
+// unused), and gddq receives dq_after = v_before + M^-1 J^T lambda.
 template<class TP, class S>
 __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<double> gU, CView<double> gD, int nc,
                                                             CView<S> gJ, CView<S> gtau, CView<S> gnle, CView<S> gg,
-                                                            double damping, View<S> gddq, View<S> glam) {
+                                                            double damping, int impulse, double rcoef, View<S> gddq,
+                                                            View<S> glam) {
   using K = KktShape<S>;
   constexpr int T = K::T, NT = 64 * K::W;
   constexpr int NV = TP::T.nv;
@@ -101,7 +105,8 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
   }
   __syncthreads();
 
-  // ---- phase 2: right-hand side j of configuration c (j < nc: row j of J; j = nc: tau - nle)
+  // ---- phase 2: right-hand side j of configuration c (j < nc: row j of J; j = nc: tau - nle,
+  // or v_before, unsolved, for impulses)
   const int L = t & 63, w = t >> 6;
   const int c = L % T;
   const int j = w * K::RPW + L / T;
@@ -117,11 +122,16 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
     if (j < nc) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) x[k] = double(sJ[(j + k * NC) * T + c]);
+    } else if (impulse) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) x[k] = double(gtau.p[(int64_t)k * gtau.rs + b * gtau.bs]);
     } else {
 #pragma unroll
       for (int k = 0; k < NV; ++k)
         x[k] = double(gtau.p[(int64_t)k * gtau.rs + b * gtau.bs]) - double(gnle.p[(int64_t)k * gnle.rs + b * gnle.bs]);
     }
+  }
+  if (active && !(impulse && j == nc)) {
     // x <- M^-1 x: Uiv, D^-1, Utiv (cholesky.hxx:193-317, 466-520).  U^-1 runs by columns:
     // once x[jc] is final it updates its ancestors' entries, independent multiply-adds
     // instead of the reference's row dot products (one dependent chain per row).  A
@@ -154,6 +164,8 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         }
       });
     });
+  }
+  if (active) {
     // p = J x over the zero-padded rows, one column of J per step (compile-time LDS offsets)
     const S* sj = sJ + c;
     static_for<0, NV>([&](auto KK) {
@@ -188,7 +200,9 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
 #pragma unroll
       for (int d = 0; d <= i; ++d)
         G[gi(i, d)] = i < nc ? sP[(i + d * NC) * T + t] + (d == i ? damping : 0.0) : (d == i ? 1.0 : 0.0);
-      l[i] = i < nc ? -sP[(i + nc * NC) * T + t] - double(gg.p[(int64_t)i * gg.rs + bt * gg.bs]) : 0.0;
+      l[i] = i >= nc ? 0.0
+             : impulse ? -(1.0 + rcoef) * sP[(i + nc * NC) * T + t]
+                       : -sP[(i + nc * NC) * T + t] - double(gg.p[(int64_t)i * gg.rs + bt * gg.bs]);
     }
     // column by column: L_dd = sqrt(.), then the column below scaled by 1 / L_dd (kept)
     double rd[NC];

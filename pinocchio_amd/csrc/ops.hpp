@@ -33,10 +33,12 @@ template<class S>
 using LaunchKin = void (*)(hipStream_t, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> oM,
                            View<S> vo, View<S> ao, View<S> J);
 
-// forward dynamics under J ddq + gamma = 0 from the FP64 tree factor of M and nle: ddq, lambda
+// forward dynamics under J ddq + gamma = 0 from the FP64 tree factor of M and nle: ddq,
+// lambda; impulse != 0: impulse dynamics with v_before in `tau` (nle and gamma unused)
 template<class S>
 using LaunchKkt = void (*)(hipStream_t, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J,
-                           CView<S> tau, CView<S> nle, CView<S> gamma, double damping, View<S> ddq, View<S> lambda);
+                           CView<S> tau, CView<S> nle, CView<S> gamma, double damping, int impulse, double rcoef,
+                           View<S> ddq, View<S> lambda);
 
 struct TopoOps {
   const char* name;

@@ -773,8 +773,51 @@ void cfdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
   else o.rnea_f64(ctx->stream, mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
   const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
   kkt(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
-      CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{dg, vg.rs, vg.bs}, damping, View<S>{da, va.rs, va.bs},
-      View<S>{dl, vl.rs, vl.bs});
+      CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{dg, vg.rs, vg.bs}, damping, 0, 0.0,
+      View<S>{da, va.rs, va.bs}, View<S>{dl, vl.rs, vl.bs});
+  finishCall(ctx, flags, st);
+}
+
+// impulseDynamics(model, data, q, v_before, J, r_coeff, inv_damping): CRBA, FP64 tree
+// factor, then k_kkt in impulse mode
+template<class S>
+void impulseCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v,
+                 int64_t ldv, int nc, const void* J, int64_t ldJ, double rcoef, double damping, void* dqa,
+                 int64_t lda, void* imp, int64_t ldi, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const int nv = m->nv, nnz = m->ops->nnzU;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v_before");
+  auto vJ = argView<S>(nc * nv, B, ldJ, flags, "J");
+  auto va = argView<S>(nv, B, lda, flags, "dq_after");
+  auto vi = argView<S>(nc, B, ldi, flags, "impulse");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const bool batchLayout = (flags & RBD_LAYOUT_BATCH) != 0;
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* dJ = (const S*)st.in(J, vJ.extent * sizeof(S));
+  S* da = (S*)st.out(dqa, va.extent * sizeof(S), lda != nv || batchLayout);
+  S* di = (S*)st.out(imp, vi.extent * sizeof(S), ldi != nc || batchLayout);
+  const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B, nM = (size_t)nv * nv * B;
+  double* wU = (double*)ctx->workspace((nU + nD) * 8 + nM * sizeof(S));
+  double* wD = wU + nU;
+  S* wM = (S*)(wD + nD);
+  const S* mc = f32 ? (const S*)m->mc32 : (const S*)m->mc64;
+  const TopoOps& o = *m->ops;
+  const CView<S> cq{dq, vq.rs, vq.bs};
+  if constexpr (f32) {
+    o.crba_f32(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f32(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+  } else {
+    o.crba_f64(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f64(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+  }
+  const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
+  kkt(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
+      CView<S>{dv, vv.rs, vv.bs}, CView<S>{}, CView<S>{}, damping, 1, rcoef, View<S>{da, va.rs, va.bs},
+      View<S>{di, vi.rs, vi.bs});
   finishCall(ctx, flags, st);
 }
 }  // namespace
@@ -799,6 +842,25 @@ int rbd_frames(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const voi
     else
       framesCall<double>(ctx, m, B, q, ldq, nframes, frame_joint, frame_placement, reference_frame, oMf, ldoMf, J,
                          ldJ, flags);
+  });
+}
+
+int rbd_impulse_dynamics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                         const void* v_before, int64_t ldv, int nc, const void* J, int64_t ldJ, double r_coeff,
+                         double inv_damping, void* dq_after, int64_t ldda, void* impulse, int64_t ldimpulse,
+                         uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(nc >= 1 && nc <= kKktMaxConstraints, "the constraint count must be between 1 and 12");
+    RBD_CHECK_ARG(B == 0 || (q && v_before && J && dq_after && impulse), "null array argument");
+    if (!m->compact || !m->ops->kkt_f32 || !m->ops->crba_f32)
+      throw Error(RBD_EUNSUPPORTED, "impulse dynamics: no kernels for this topology");
+    if (dtype == RBD_F32)
+      impulseCall<float>(ctx, m, B, q, ldq, v_before, ldv, nc, J, ldJ, r_coeff, inv_damping, dq_after, ldda, impulse,
+                         ldimpulse, flags);
+    else
+      impulseCall<double>(ctx, m, B, q, ldq, v_before, ldv, nc, J, ldJ, r_coeff, inv_damping, dq_after, ldda,
+                          impulse, ldimpulse, flags);
   });
 }
 

@@ -157,3 +157,28 @@ def test_argument_errors(gpu_ctx):
         eng.constrained_forward_dynamics(z(desc.nq), z(desc.nv), z(desc.nv), z(13 * desc.nv), z(13))
     with pytest.raises(ValueError):
         eng.constrained_forward_dynamics(z(desc.nq), z(desc.nv), z(desc.nv), z(6 * desc.nv + 1), z(6))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("e", [0.0, 0.5])
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18"])
+def test_impulse_parity(gpu_ctx, name, e, layout, prec):
+    """impulseDynamics (contact-dynamics.hxx:264-331) on foot contacts against the oracle."""
+    desc, model = gpu_model(gpu_ctx, name)
+    B = 555
+    q, v, _, _ = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 720)
+    joints, kind = FEET[name]
+    J = contact_jacobian(desc, q, joints, kind)
+    dtype = torch.float32 if prec == "f32" else torch.float64
+    if prec == "f32":
+        q, v, J = f32_round(q, v, J)
+    eng = pa.Engine(model, layout)
+    vp, imp = eng.impulse_dynamics(dev_rows(q, layout, dtype), dev_rows(v, layout, dtype),
+                                   dev_rows(j_rows(J), layout, dtype), r_coeff=e)
+    torch.cuda.synchronize()
+    vp, imp = host_rows(vp, layout), host_rows(imp, layout)
+    vr, ir = oracle.impulse_dynamics(desc, q, v, J, r_coeff=e)
+    ev, ei = rel_err_cols(vp, vr), rel_err_cols(imp, ir)
+    print(f"{name} e={e} {layout} {prec}: dq_after max {ev.max():.2e}; impulse max {ei.max():.2e}")
+    assert ev.max() < TOL[prec] and ei.max() < TOL[prec]

@@ -7,7 +7,10 @@ inv_damping, contact-dynamics.hxx:90-167) without the reference.
   * with damping: lambda = -(J M^-1 J^T + mu I)^-1 (J M^-1 (tau - nle) + gamma);
   * J = rows of the identity with gamma = -a* prescribes those accelerations exactly, and
     then rnea(q, v, ddq) - tau = J^T lambda (Newton-Euler balance by RNEA alone);
-  * planted mistakes (a flipped gamma sign, a dropped nle) fail the dense check.
+  * planted mistakes (a flipped gamma sign, a dropped nle) fail the dense check;
+and its impulseDynamics (contact-dynamics.hxx:264-331): the impact KKT system
+[[M, J^T], [J, 0]] [v+; -lambda] = [M v-; -e J v-] solved by numpy, J v+ = -e J v-, and the
+damped closed form.
 """
 import numpy as np
 import pytest
@@ -112,3 +115,35 @@ def test_planted_mistakes_are_caught():
     nle = oracle.rnea(d, q, v, np.zeros_like(v))
     a, lam = oracle.constrained_forward_dynamics(d, q, v, tau + nle, J, gamma)
     assert relmax(a, a_ref) > 1e-3
+
+
+# ---- impulseDynamics (contact-dynamics.hxx:264-331) ---------------------------------------
+@pytest.mark.parametrize("name", ["arm7", "quadruped18", "humanoid36"])
+@pytest.mark.parametrize("e", [0.0, 0.4])
+def test_impulse_matches_dense_kkt(name, e):
+    """[[M, J^T], [J, 0]] [v+; -lambda] = [M v-; -e J v-] solved by numpy; J v+ = -e J v-."""
+    d = mk(name)
+    nc = min(6, d.nv - 1)
+    q, v, _, J, _ = problem(d, 6, nc, 7)
+    vp, lam = oracle.impulse_dynamics(d, q, v, J, r_coeff=e)
+    Ms = oracle.crba(d, q)
+    for b in range(q.shape[1]):
+        Mb, Jb = Ms[b], J[b]
+        K = np.block([[Mb, Jb.T], [Jb, np.zeros((nc, nc))]])
+        sol = np.linalg.solve(K, np.concatenate([Mb @ v[:, b], -e * (Jb @ v[:, b])]))
+        assert relmax(vp[:, b], sol[:d.nv]) < 1e-9 and relmax(lam[:, b], -sol[d.nv:]) < 1e-9
+        assert np.abs(Jb @ vp[:, b] + e * (Jb @ v[:, b])).max() < 1e-9 * max(1.0, np.abs(Jb @ v[:, b]).max())
+
+
+def test_impulse_damping_matches_dense():
+    d = mk("humanoid36")
+    q, v, _, J, _ = problem(d, 4, 12, 8)
+    mu = 1e-3
+    vp, lam = oracle.impulse_dynamics(d, q, v, J, r_coeff=0.2, inv_damping=mu)
+    Ms = oracle.crba(d, q)
+    for b in range(q.shape[1]):
+        Mb, Jb = Ms[b], J[b]
+        G = Jb @ np.linalg.solve(Mb, Jb.T) + mu * np.eye(12)
+        lr = np.linalg.solve(G, -1.2 * (Jb @ v[:, b]))
+        assert relmax(lam[:, b], lr) < 1e-9
+        assert relmax(vp[:, b], v[:, b] + np.linalg.solve(Mb, Jb.T @ lr)) < 1e-9
