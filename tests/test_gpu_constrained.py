@@ -182,3 +182,27 @@ def test_impulse_parity(gpu_ctx, name, e, layout, prec):
     ev, ei = rel_err_cols(vp, vr), rel_err_cols(imp, ir)
     print(f"{name} e={e} {layout} {prec}: dq_after max {ev.max():.2e}; impulse max {ei.max():.2e}")
     assert ev.max() < TOL[prec] and ei.max() < TOL[prec]
+
+
+def test_impulse_host_pointers(gpu_ctx):
+    """rbd_impulse_dynamics through the raw C ABI on host arrays (RBD_HOST_PTRS), FP64,
+    Eigen column-major layout."""
+    import ctypes
+    desc, model = gpu_model(gpu_ctx, "quadruped18")
+    B = 97
+    q, v, _, _ = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 730)
+    J = contact_jacobian(desc, q, *FEET["quadruped18"])
+    nc, nv = J.shape[1], desc.nv
+    qc, vc = np.ascontiguousarray(q.T), np.ascontiguousarray(v.T)
+    Jc = np.ascontiguousarray(J.transpose(0, 2, 1).reshape(B, -1))  # per configuration column major
+    vp = np.zeros((B, nv))
+    imp = np.zeros((B, nc))
+    L = pa.lib()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = L.rbd_impulse_dynamics(gpu_ctx.handle, model.handle, ctypes.c_int64(B), pa.RBD_F64, p(qc),
+                                ctypes.c_int64(desc.nq), p(vc), ctypes.c_int64(nv), nc, p(Jc), ctypes.c_int64(nc * nv),
+                                ctypes.c_double(0.3), ctypes.c_double(0.0), p(vp), ctypes.c_int64(nv), p(imp),
+                                ctypes.c_int64(nc), ctypes.c_uint32(pa.RBD_HOST_PTRS))
+    assert rc == 0, L.rbd_last_error()
+    vr, ir = oracle.impulse_dynamics(desc, q, v, J, r_coeff=0.3)
+    assert rel_err_cols(vp.T, vr).max() < 1e-9 and rel_err_cols(imp.T, ir).max() < 1e-9

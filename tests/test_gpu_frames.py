@@ -74,3 +74,30 @@ def test_frames_arguments(gpu_ctx):
     o = oMf.cpu().numpy()
     assert np.allclose(o[:9].T, np.eye(3).ravel()) and np.allclose(o[9:].T, [1.0, 2.0, 3.0])
     assert float(J.abs().max()) == 0.0  # the universe supports no dof
+
+
+def test_frames_host_pointers(gpu_ctx):
+    """The raw C ABI with host arrays (RBD_HOST_PTRS), Eigen column-major layout with a
+    leading dimension larger than the rows: the gaps stay as the caller left them."""
+    import ctypes
+    desc, model = gpu_model(gpu_ctx, "humanoid36")
+    B, nf = 130, 3
+    q, _, _, _ = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 881)
+    par, pl = random_frames(desc, nf, 4)
+    ldo, ldj = 12 * nf + 5, 6 * desc.nv * nf + 3
+    oMf = np.full((B, ldo), 7.0)
+    J = np.full((B, ldj), 7.0)
+    qc = np.ascontiguousarray(q.T)
+    L = pa.lib()
+    rc = L.rbd_frames(gpu_ctx.handle, model.handle, ctypes.c_int64(B), pa.RBD_F64,
+                      qc.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(desc.nq), nf,
+                      np.ascontiguousarray(par, dtype=np.int32).ctypes.data_as(ctypes.c_void_p),
+                      np.ascontiguousarray(pl).ctypes.data_as(ctypes.c_void_p), 2,
+                      oMf.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(ldo),
+                      J.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(ldj), ctypes.c_uint32(pa.RBD_HOST_PTRS))
+    assert rc == 0, L.rbd_last_error()
+    ref_o, ref_J = oracle.frames(desc, q, par, pl, "local_world_aligned")
+    assert rel(oMf[:, :12 * nf].reshape(B, nf, 12), ref_o) < 1e-12
+    Jg = J[:, :6 * desc.nv * nf].reshape(B, nf, desc.nv, 6).transpose(0, 1, 3, 2)
+    assert rel(Jg, ref_J) < 1e-12
+    assert (oMf[:, 12 * nf:] == 7.0).all() and (J[:, 6 * desc.nv * nf:] == 7.0).all()
