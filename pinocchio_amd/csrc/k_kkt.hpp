@@ -29,7 +29,7 @@ template<class S> struct KktShape {
   static constexpr int T = sizeof(S) == 4 ? 16 : 8;
   static constexpr int RPW = 64 / T;  // right-hand sides per wave
   static constexpr int W = (kKktMaxConstraints + 1 + RPW - 1) / RPW;
-  static constexpr int kU = 16;       // staging loads in flight per thread
+  static constexpr int kU = 32;       // staging loads in flight per thread
 };
 
 // LDS bytes of the kernel: the factor (U entries then 1/D, FP64) and J (S) of a tile; after
