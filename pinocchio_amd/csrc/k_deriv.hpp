@@ -100,11 +100,10 @@ __device__ __forceinline__ void rneadTile(const S* __restrict__ mc_in, int64_t B
   constexpr int NJ = TP::NJ, NQ = T.nq, NV = T.nv;
   constexpr WaveSplit<TP, G> WS{};
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  if (W == 0) {
-    stageIn<NQ>(sq, gq, b0, nb);
-    stageIn<NV>(sv, gv, b0, nb);
-    stageIn<NV>(sa, ga, b0, nb);
-  }
+  // the three inputs are staged by different waves when there are several
+  if constexpr (W == 0) stageIn<NQ>(sq, gq, b0, nb);
+  if constexpr (W == 1 % G) stageIn<NV>(sv, gv, b0, nb);
+  if constexpr (W == 2 % G) stageIn<NV>(sa, ga, b0, nb);
   __syncthreads();
   const int L = laneId();
   const int64_t b = b0 + L;

@@ -165,11 +165,11 @@ __device__ __forceinline__ void abaTile(const S* __restrict__ mc_in, int64_t B, 
   S* st = lds + (NQ + NV) * kPitch;                    // tau in
   S* so = G > 1 ? lds + (NQ + 2 * NV) * kPitch : st;  // ddq out
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  if (W == 0) {
-    stageIn<NQ, S, kU>(sq, gq, b0, nb);
-    stageIn<NV, S, kU>(sv, gv, b0, nb);
-    stageIn<NV, S, kU>(st, gt, b0, nb);
-  }
+  // the three inputs are staged by different waves when there are several (more loads
+  // in flight per tile)
+  if constexpr (W == 0) stageIn<NQ, S, kU>(sq, gq, b0, nb);
+  if constexpr (W == 1 % G) stageIn<NV, S, kU>(sv, gv, b0, nb);
+  if constexpr (W == 2 % G) stageIn<NV, S, kU>(st, gt, b0, nb);
   __syncthreads();
   const int L = laneId();
   const Mot<S> a0 = mcGravityNeg(mc_in, mcGravity<NJ, NV>());
