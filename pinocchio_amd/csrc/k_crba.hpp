@@ -71,7 +71,13 @@ __global__ __launch_bounds__(64 * G) void k_crba(const S* __restrict__ mc_in, in
   const int64_t b0 = int64_t(blockIdx.x) * kTile;
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
   const int w = G > 1 ? (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
-  if (w == 0) stageIn<NQ>(sq, gq, b0, nb);
+  if constexpr (G == 1) {
+    stageIn<NQ>(sq, gq, b0, nb);
+  } else {  // two waves stage half of q each
+    constexpr int NQh = (NQ + 1) / 2;
+    if (w == 0) stageIn<NQh>(sq, gq, b0, nb);
+    else if (w == 1) stageIn<NQ - NQh>(sq + NQh * kPitch, CView<S>{gq.p + (int64_t)NQh * gq.rs, gq.rs, gq.bs}, b0, nb);
+  }
   __syncthreads();
   const int L = laneId();
   const int64_t b = b0 + L;

@@ -136,7 +136,8 @@ template<class TP, class S>
 __global__ __launch_bounds__(64) void k_rnea(const S* __restrict__ mc, int64_t B, CView<S> gq, CView<S> gv,
                                              CView<S> ga, View<S> gtau) {
   __shared__ S lds[dynLdsRows<TP>() * kPitch];
-  rneaTile<TP, S>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, ga, gtau, lds);
+  // a single wave stages the whole tile: 16 loads in flight per lane
+  rneaTile<TP, S, 2 * kStageU>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, ga, gtau, lds);
 }
 
 // ---------------------------------------------------------------------------------------
