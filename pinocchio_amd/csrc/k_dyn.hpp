@@ -346,7 +346,7 @@ template<class TP, class S>
 __global__ __launch_bounds__(64) void k_aba(const S* __restrict__ mc, int64_t B, CView<S> gq, CView<S> gv,
                                             CView<S> gt, View<S> gddq) {
   __shared__ S lds[dynLdsRows<TP>() * kPitch];
-  abaTile<TP, S>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, gt, gddq, lds);
+  abaTile<TP, S, 1, 0, 2 * kStageU>(mc, B, int64_t(blockIdx.x) * kTile, gq, gv, gt, gddq, lds);
 }
 
 // G waves per 64-configuration tile, split by subtrees (small batches)
