@@ -98,8 +98,15 @@ void launchCrba(hipStream_t st, const S* mc, int64_t B, CView<S> q, View<S> M, i
 template<class TP, class S> void launchChol(hipStream_t st, int64_t B, CView<S> M, View<double> U, View<double> D) {
   hipLaunchKernelGGL((k_chol<TP, S, double>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
 }
+// FP32 factor (ABA derivatives, computeMinverse): M staged into LDS first when the batch
+// leaves about one tile per CU or fewer (RBD_CHOL_STAGED=0|1 fixes the choice)
 template<class TP> void launchChol32(hipStream_t st, int64_t B, CView<float> M, View<float> U, View<float> D) {
-  hipLaunchKernelGGL((k_chol<TP, float, float>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
+  static const int forced = std::getenv("RBD_CHOL_STAGED") ? envInt("RBD_CHOL_STAGED") : -1;
+  const bool staged = forced >= 0 ? forced != 0 : (B + kTile - 1) / kTile <= 512;
+  if (staged && cholStagedLdsBytes<TP, float>() <= 160 * 1024)
+    hipLaunchKernelGGL((k_chol_staged<TP, float, float>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
+  else
+    hipLaunchKernelGGL((k_chol<TP, float, float>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
 }
 template<class TP, class S>
 void launchCholSolve(hipStream_t st, int64_t B, CView<double> U, CView<double> D, View<S> x) {

@@ -266,6 +266,100 @@ __global__ __launch_bounds__(64) void k_chol(int64_t B, CView<SIN> gM, View<SC> 
   });
 }
 
+// The same factorisation with M's pattern entries staged into LDS first, kStageU loads in
+// flight per lane, instead of lazily per column: for small batches (one tile per CU) the
+// per-column loads otherwise expose the memory latency once per column.  LDS holds the
+// strictly-upper pattern entries in packed order, then the diagonal; [entry][lane].
+template<class TP> struct CholEntries {
+  int row[nnzOf<TP>() + TP::T.nv > 0 ? nnzOf<TP>() + TP::T.nv : 1] = {};
+  int col[nnzOf<TP>() + TP::T.nv > 0 ? nnzOf<TP>() + TP::T.nv : 1] = {};
+  constexpr CholEntries() {
+    using P = CholPattern<TP>;
+    constexpr int NV = TP::T.nv;
+    for (int c = 0; c < NV; ++c) {
+      for (int r = 0; r < c; ++r)
+        if (P::anc(r, c)) {
+          row[P::packed(r, c)] = r;
+          col[P::packed(r, c)] = c;
+        }
+      row[nnzOf<TP>() + c] = c;
+      col[nnzOf<TP>() + c] = c;
+    }
+  }
+};
+template<class TP, class SIN> constexpr int cholStagedLdsBytes() {
+  return (nnzOf<TP>() + TP::T.nv) * kTile * (int)sizeof(SIN);
+}
+template<class TP, class SIN, class SC>
+__global__ __launch_bounds__(64) void k_chol_staged(int64_t B, CView<SIN> gM, View<SC> gU, View<SC> gD) {
+  constexpr auto& T = TP::T;
+  constexpr int NV = T.nv, NNZ = nnzOf<TP>(), NE = NNZ + NV;
+  using P = CholPattern<TP>;
+  constexpr CholEntries<TP> CE{};
+  __shared__ SIN sM[NE * kTile];
+  const int L = laneId();
+  const int64_t b0 = int64_t(blockIdx.x) * kTile;
+  const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
+  {
+    const int64_t bc = b0 + (L < nb ? L : nb - 1);
+    static_for<0, (NE + kStageU - 1) / kStageU>([&](auto I) {
+      SIN x[kStageU];
+      static_for<0, kStageU>([&](auto U) {
+        constexpr int k = int(I) * kStageU + int(U);
+        if constexpr (k < NE) x[U] = gM.p[(int64_t)(CE.row[k] + CE.col[k] * NV) * gM.rs + bc * gM.bs];
+      });
+      static_for<0, kStageU>([&](auto U) {
+        constexpr int k = int(I) * kStageU + int(U);
+        if constexpr (k < NE) sM[k * kTile + L] = x[U];
+      });
+    });
+  }
+  // every lane reads only its own LDS column: no barrier needed between the two phases
+  if (L >= nb) return;
+  const int64_t b = b0 + L;
+  SC A[NV][NV];
+  auto ldM = [&](auto R, auto C) {
+    constexpr int r = decltype(R)::value, c = decltype(C)::value;
+    constexpr int k = r == c ? NNZ + c : P::packed(r, c);
+    return SC(sM[k * kTile + L]);
+  };
+  static_for<0, NV>([&](auto JJ) {
+    constexpr int j = NV - 1 - JJ;
+    stepFence();
+    if constexpr (P::firstTouch(j, j) == j) A[j][j] = ldM(IC<j>{}, IC<j>{});
+    static_for<0, NV>([&](auto RR) {
+      constexpr int r = RR;
+      if constexpr (P::anc(r, j) && P::firstTouch(r, j) == j) A[r][j] = ldM(IC<r>{}, IC<j>{});
+    });
+    const SC Dj = A[j][j];
+    const SC Dinv = SC(1) / Dj;
+    stOut(&gD.p[(int64_t)j * gD.rs + b * gD.bs], Dj);
+    SC Uc[NV];
+    static_for<0, NV>([&](auto RR) {
+      constexpr int r = RR;
+      if constexpr (P::anc(r, j)) {
+        Uc[r] = A[r][j] * Dinv;
+        constexpr int pk = P::packed(r, j);
+        stOut(&gU.p[(int64_t)pk * gU.rs + b * gU.bs], Uc[r]);
+      }
+    });
+    static_for<0, NV>([&](auto R2) {
+      constexpr int r2 = R2;
+      if constexpr (P::anc(r2, j)) {
+        if constexpr (P::firstTouch(r2, r2) == j) A[r2][r2] = ldM(IC<r2>{}, IC<r2>{});
+        A[r2][r2] -= Uc[r2] * A[r2][j];
+        static_for<0, NV>([&](auto R1) {
+          constexpr int r1 = R1;
+          if constexpr (P::anc(r1, r2)) {
+            if constexpr (P::firstTouch(r1, r2) == j) A[r1][r2] = ldM(IC<r1>{}, IC<r2>{});
+            A[r1][r2] -= Uc[r1] * A[r2][j];
+          }
+        });
+      }
+    });
+  });
+}
+
 // ---------------------------------------------------------------------------------------
 // x <- M^-1 x with a packed FP64 factor (cholesky::solve = Uiv, D^-1, Utiv,
 // cholesky.hxx:193-317, 466-520); arithmetic in FP64, x in S.
