@@ -44,6 +44,39 @@ def rel_mats(x, ref):
     return float((d / np.maximum(np.abs(ref).reshape(len(ref), -1).max(1), 1e-30)).max())
 
 
+def err_cols(x, ref):
+    """per-configuration inf-norm relative errors of (n, B) arrays"""
+    return np.abs(x - ref).max(0) / np.maximum(np.abs(ref).max(0), 1e-30)
+
+
+def err_mats(x, ref):
+    """per-configuration errors of (B, n, n) arrays"""
+    d = np.abs(x - ref).reshape(len(ref), -1).max(1)
+    return d / np.maximum(np.abs(ref).reshape(len(ref), -1).max(1), 1e-30)
+
+
+def stats(e):
+    """max, median and 99.9th percentile of per-configuration errors, over the whole batch"""
+    return {"max": float(e.max()), "p50": float(np.median(e)), "p99.9": float(np.quantile(e, 0.999)),
+            "configs": int(e.size)}
+
+
+def host_info():
+    """CPU model, sockets, cores, SMT of the host (lscpu fields), for the CPU baseline"""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)", "NUMA node(s)"):
+                info[k] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 class Timer:
     """HIP-event timing on the engine's stream (torch events on an external stream)."""
 
@@ -426,30 +459,70 @@ def main():
     others["mixed_rnea_aba_f32"] = {"instances": n_mix, "groups": dict(mix_spec), "configs_per_s": n_mix / t,
                                     "ms": t * 1e3, "chosen": tune.get("chosen"),
                                     "candidates_ms": {c["variant"]: c["ms"] for c in tune.get("candidates", [])}}
+    # run-time-topology kernels (the tree read as data; what every tree without prebuilt
+    # kernels runs on) on the same humanoid and inputs, against the prebuilt kernels
+    ert = pa.Engine(pa.Model(ctx, desc, runtime_topology=True), "batch")
+    Mrt = torch.empty_like(Mbuf)
+    rt = {}
+    for name, f_pre, f_rt, bb in (
+            ("rnea", lambda: eng.rnea(Q, V, A, async_=True), lambda: ert.rnea(Q, V, A, async_=True), B),
+            ("aba", lambda: eng.aba(Q, V, T, out=ddq, async_=True), lambda: ert.aba(Q, V, T, async_=True), B),
+            ("crba", lambda: eng.crba(Q, out=Mbuf, async_=True), lambda: ert.crba(Q, out=Mrt, async_=True), B),
+            ("rnea_derivatives", lambda: eng.rnea_derivatives(Qd, Vd, Ad, async_=True),
+             lambda: ert.rnea_derivatives(Qd, Vd, Ad, async_=True), Bd),
+            ("aba_derivatives", lambda: eng.aba_derivatives(Qd, Vd, Td, async_=True),
+             lambda: ert.aba_derivatives(Qd, Vd, Td, async_=True), Bd)):
+        tp, tr = time_call(f_pre, reps=5), time_call(f_rt, reps=5)
+        rt[name] = {"batch": bb, "prebuilt_configs_per_s": bb / tp, "runtime_topology_configs_per_s": bb / tr,
+                    "runtime_over_prebuilt": tp / tr}
+    others["runtime_topology_humanoid36_f32"] = rt
+    # FP64 ABA and ABA derivatives (run-time-topology kernels; the reference's precision)
+    Q64, V64, T64 = (torch.tensor(x, dtype=torch.float64, device=dev) for x in (q, v, tau))
+    t = time_call(lambda: eng.aba(Q64, V64, T64, async_=True), reps=5)
+    others["aba_humanoid36_f64"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3}
+    Qd64, Vd64, Td64 = (x[:, :Bd].contiguous() for x in (Q64, V64, T64))
+    t = time_call(lambda: eng.aba_derivatives(Qd64, Vd64, Td64, async_=True), reps=3)
+    others["aba_derivatives_humanoid36_f64"] = {"batch": Bd, "configs_per_s": Bd / t, "ms": t * 1e3}
+
     if not args.no_surrogate:
         others["surrogate_quadruped18"] = surrogate_bench(torch, pa, M, ctx, args)
 
     # ---------------- accuracy against the FP64 oracle ----------------
     errors = {}
     if not args.no_check:
-        n = 4096
-        qs, vs, as_, ts = (x[:, :n].astype(np.float64) for x in (q32, v32, a32, tau32))
-        errors["aba_ddq"] = rel_cols(ddq[:, :n].double().cpu().numpy(), oracle.aba(desc, qs, vs, ts))
-        Mh = pa.Engine.mats_to_numpy(Mbuf[:, :n], "batch", nv)
-        errors["crba_M"] = rel_mats(Mh, oracle.crba(desc, qs))
+        # the step's outputs over the whole 65536 batch and the derivatives over the whole
+        # 16384 batch: max, median, 99.9th percentile of the per-configuration error
+        qs, vs, as_, ts = (x.astype(np.float64) for x in (q32, v32, a32, tau32))
+        step()  # the step's outputs of the timed buffers
+        ctx.synchronize()
+        errors["aba_ddq"] = stats(err_cols(ddq.double().cpu().numpy(), oracle.aba(desc, qs, vs, ts)))
+        Mh = pa.Engine.mats_to_numpy(Mbuf, "batch", nv)
+        errors["crba_M"] = stats(err_mats(Mh, oracle.crba(desc, qs)))
         _, Dref, _, _ = oracle.cholesky(desc, Mh)
-        errors["cholesky_D_vs_fp64_of_same_M"] = float(np.abs(Dbuf[:, :n].cpu().numpy().T / Dref - 1).max())
-        tg = eng.rnea(Q[:, :n].contiguous(), V[:, :n].contiguous(), A[:, :n].contiguous())
-        errors["rnea_tau"] = rel_cols(tg.double().cpu().numpy(), oracle.rnea(desc, qs, vs, as_))
-        nd = 1024
-        o = eng.rnea_derivatives(*(x[:, :nd].contiguous() for x in (Q, V, A)))
-        r = oracle.rnea_derivatives(desc, qs[:, :nd], vs[:, :nd], as_[:, :nd])
+        errors["cholesky_D_vs_fp64_of_same_M"] = stats(np.abs(Dbuf.cpu().numpy().T / Dref - 1).max(1))
+        del Mh, Dref
+        errors["rnea_tau"] = stats(err_cols(eng.rnea(Q, V, A).double().cpu().numpy(), oracle.rnea(desc, qs, vs, as_)))
+        o = eng.rnea_derivatives(Qd, Vd, Ad)
+        r = oracle.rnea_derivatives(desc, qs[:, :Bd], vs[:, :Bd], as_[:, :Bd])
         for name, x, ref in zip(("dtau_dq", "dtau_dv"), o[:2], r[:2]):
-            errors[name] = rel_mats(pa.Engine.mats_to_numpy(x, "batch", nv), ref)
-        o = eng.aba_derivatives(*(x[:, :nd].contiguous() for x in (Q, V, T)))
-        r = oracle.aba_derivatives(desc, qs[:, :nd], vs[:, :nd], ts[:, :nd])
+            errors[name] = stats(err_mats(pa.Engine.mats_to_numpy(x, "batch", nv), ref))
+        del o, r
+        o = eng.aba_derivatives(Qd, Vd, Td)
+        r = oracle.aba_derivatives(desc, qs[:, :Bd], vs[:, :Bd], ts[:, :Bd])
         for name, x, ref in zip(("ddq_dq", "ddq_dv", "Minv"), o[:3], r[:3]):
-            errors[name] = rel_mats(pa.Engine.mats_to_numpy(x, "batch", nv), ref)
+            errors[name] = stats(err_mats(pa.Engine.mats_to_numpy(x, "batch", nv), ref))
+        del o, r
+        # FP64 ABA and derivatives against the oracle on the same (unrounded) inputs
+        nf = 4096
+        q64, v64, t64 = q[:, :nf], v[:, :nf], tau[:, :nf]
+        g64 = eng.aba(*(torch.tensor(x, dtype=torch.float64, device=dev) for x in (q64, v64, t64)))
+        errors["aba_ddq_f64"] = stats(err_cols(g64.cpu().numpy(), oracle.aba(desc, q64, v64, t64)))
+        o = eng.aba_derivatives(*(torch.tensor(x, dtype=torch.float64, device=dev) for x in (q64, v64, t64)))
+        r = oracle.aba_derivatives(desc, q64, v64, t64)
+        for name, x, ref in zip(("ddq_dq_f64", "ddq_dv_f64", "Minv_f64"), o[:3], r[:3]):
+            errors[name] = stats(err_mats(pa.Engine.mats_to_numpy(x, "batch", nv), ref))
+        del o, r
+        nd = 1024
         Jh = Jrows[:, :nd].double().cpu().numpy().reshape(nv, ncon, nd).transpose(2, 1, 0)
         ra, rl = oracle.constrained_forward_dynamics(desc, qs[:, :nd], vs[:, :nd], ts[:, :nd], Jh,
                                                      np.zeros((ncon, nd)))
@@ -471,7 +544,20 @@ def main():
         sec = oracle.time_batch(desc, "step", q[:, :nb], v[:, :nb], tau[:, :nb], nthreads=cores, reps=reps)
         cpu = {"value": nb / sec, "unit": "configs/s", "cores": cores, "kind": "port",
                "sample": f"{nb} humanoid36 configurations x {reps} repetitions (median), "
-                         f"ABA(WORLD)+CRBA+Cholesky in FP64, OpenMP static schedule over configurations"}
+                         f"ABA(WORLD)+CRBA+Cholesky in FP64, OpenMP static schedule over configurations",
+               "host": host_info(), "threads": cores,
+               "omp": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND", "unset"),
+                       "OMP_PLACES": os.environ.get("OMP_PLACES", "unset"), "schedule": "static"}}
+        # single-thread latency per call, against the reference's published single-thread
+        # figures (README.md:138-143: TALOS, the closest robot, ~4.2 / 4.9 / 8.4 us for
+        # RNEA / CRBA / ABA on an i7 @ 2.4 GHz)
+        n1 = 4096
+        lat = {}
+        for alg, x in (("rnea", a), ("crba", a), ("aba", tau)):
+            s1 = oracle.time_batch(desc, alg, q[:, :n1], v[:, :n1], x[:, :n1], nthreads=1, reps=5)
+            lat[alg + "_us"] = s1 / n1 * 1e6
+        lat["reference_readme_talos_us"] = {"rnea": 4.2, "crba": 4.9, "aba": 8.4}
+        cpu["single_thread_latency"] = lat
         cpu_alg = {}
         for alg, nbb, x in (("rnea", nb, a), ("aba", nb, tau), ("rnea_derivatives", Bd, a),
                             ("aba_derivatives", Bd, tau)):

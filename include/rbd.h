@@ -99,16 +99,31 @@ const char* rbd_version(void);
 
 /* Context: owns a HIP stream on `device` and the staging buffers.  Replaces the
  * ModelPool + OpenMP thread team of the reference (multibody/pool/model.hpp:19-165,
- * algorithm/parallel/omp.hpp:12-16); no global state is mutated. */
+ * algorithm/parallel/omp.hpp:12-16); no global state is mutated.  The stream is a
+ * blocking stream: its work is ordered with work on the legacy default stream (HIP's
+ * null stream, PyTorch's default stream); callers that produce inputs or consume outputs
+ * on other streams order them with events on rbd_ctx_stream(). */
 int rbd_ctx_create(int device, rbd_ctx** out);
 int rbd_ctx_destroy(rbd_ctx* ctx);
 /* The stream the context enqueues on (a hipStream_t). */
 void* rbd_ctx_stream(rbd_ctx* ctx);
 int rbd_ctx_synchronize(rbd_ctx* ctx);
 
-/* Model upload (ModelTpl::addJoint, multibody/model.hxx:61-170, flattened).  Fails with
- * RBD_EUNSUPPORTED when the engine has no kernels for the model's topology. */
+/* Model upload (ModelTpl::addJoint, multibody/model.hxx:61-170, flattened).  Any tree of
+ * revolute X / Y / Z joints with free-flyer joints at the roots is accepted: trees the
+ * library has prebuilt kernels for (compiled for their exact topology) run on those, every
+ * other tree on the run-time-topology kernels, which read the tree as data.  Fails with
+ * RBD_EUNSUPPORTED for other joint types or a free flyer below another joint. */
 int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* desc, rbd_model** out);
+/* As rbd_model_create; with RBD_MODEL_RUNTIME_TOPOLOGY the model runs on the
+ * run-time-topology kernels even when its tree is prebuilt (for comparisons).  No
+ * reference counterpart. */
+#define RBD_MODEL_RUNTIME_TOPOLOGY 0x1u
+int rbd_model_create_ex(rbd_ctx* ctx, const rbd_model_desc* desc, uint32_t model_flags, rbd_model** out);
+/* *prebuilt = 1 when the model's FP32 sweeps run on kernels compiled for its topology, 0
+ * when on the run-time-topology kernels.  (FP64 ABA and FP64 derivatives always use the
+ * run-time-topology kernels.)  No reference counterpart. */
+int rbd_model_kernels(const rbd_model* model, int32_t* prebuilt);
 int rbd_model_destroy(rbd_model* model);
 /* nq, nv, njoints and whether the model is DFS-compact (crba.hxx:573-595). */
 int rbd_model_dims(const rbd_model* model, int32_t* nq, int32_t* nv, int32_t* njoints, int32_t* compact);
@@ -148,7 +163,7 @@ int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, in
                          void* dtau_dv, void* dtau_da, int64_t ldout, uint32_t flags);
 
 /* computeABADerivatives: ddq_dq, ddq_dv, ddq_dtau (= M^{-1}) per configuration, and
- * optionally ddq (may be NULL).  FP32 only. */
+ * optionally ddq (may be NULL).  FP32 or FP64 (every step in the given precision). */
 int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
                         int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq,
                         void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags);

@@ -80,19 +80,30 @@ class Context:
             self.handle = None
 
 
-class Model:
-    """A model uploaded to a context (rbd_model)."""
+RBD_MODEL_RUNTIME_TOPOLOGY = 0x1
 
-    def __init__(self, ctx: Context, desc: ModelDesc):
+
+class Model:
+    """A model uploaded to a context (rbd_model).
+
+    Any tree of revolute X/Y/Z joints with free-flyer roots is accepted.  Trees the
+    library has prebuilt kernels for run on those (``prebuilt`` is True); every other tree,
+    or any tree with ``runtime_topology=True``, runs on the run-time-topology kernels."""
+
+    def __init__(self, ctx: Context, desc: ModelDesc, runtime_topology: bool = False):
         self.ctx = ctx
         self.desc = desc
         self._cdesc = desc.cstruct()
         h = ctypes.c_void_p()
-        _check(lib().rbd_model_create(ctx.handle, ctypes.byref(self._cdesc), ctypes.byref(h)))
+        flags = RBD_MODEL_RUNTIME_TOPOLOGY if runtime_topology else 0
+        _check(lib().rbd_model_create_ex(ctx.handle, ctypes.byref(self._cdesc), ctypes.c_uint32(flags),
+                                         ctypes.byref(h)))
         self.handle = h
-        nq, nv, nj, compact = (ctypes.c_int32() for _ in range(4))
+        nq, nv, nj, compact, pre = (ctypes.c_int32() for _ in range(5))
         _check(lib().rbd_model_dims(h, ctypes.byref(nq), ctypes.byref(nv), ctypes.byref(nj), ctypes.byref(compact)))
+        _check(lib().rbd_model_kernels(h, ctypes.byref(pre)))
         self.nq, self.nv, self.njoints, self.compact = nq.value, nv.value, nj.value, bool(compact.value)
+        self.prebuilt = bool(pre.value)
 
     def cholesky_pattern(self):
         n = ctypes.c_int32()
@@ -454,21 +465,26 @@ class MixedBatch:
 class ModelPool:
     """Counterpart of pinocchio.ModelPool (multibody/pool/model.hpp:19-165): holds the
     model once on the GPU instead of one copy per OpenMP thread; `size` is kept for the
-    reference's argument checks."""
+    reference's argument checks.  `model` is a ModelDesc or a pinocchio Model (the
+    reference's Python ``pinocchio.Model``, or any object with its attributes; see
+    ``models.from_pinocchio``)."""
 
-    def __init__(self, model_desc: ModelDesc, size: int = 1, device: int = 0):
+    def __init__(self, model, size: int = 1, device: int = 0):
         if size < 1:
             raise ValueError("The pool size must be positive")
+        desc = model if isinstance(model, ModelDesc) else models.from_pinocchio(model)
         self.ctx = Context(device)
-        self.model = Model(self.ctx, model_desc)
+        self.model = Model(self.ctx, desc)
         self._size = size
 
     def size(self) -> int:
         return self._size
 
 
-def _host_call(fn_name, num_threads, pool, q, v, x, dtype):
-    """Shared argument checks of algorithm/parallel/{rnea,aba}.hpp:53-69 and the call."""
+def _host_call(fn_name, num_threads, pool, q, v, x, dtype, out=None):
+    """Shared argument checks of algorithm/parallel/{rnea,aba}.hpp:53-69 and the call.
+    With `out` (an nv x B float64 array) the result is written into it, as the
+    reference's output-argument overloads do, and None is returned."""
     m = pool.model
     if pool.size() == 0:
         raise ValueError("The pool should have at least one element")
@@ -486,25 +502,40 @@ def _host_call(fn_name, num_threads, pool, q, v, x, dtype):
     B = q.shape[1]
     if v.shape[1] != B or x.shape[1] != B:
         raise ValueError("The input matrices do not have the same number of columns")
-    out = np.zeros((m.nv, B), dtype=dtype, order="F")
+    if out is not None:
+        if not isinstance(out, np.ndarray) or out.ndim != 2 or out.shape[0] != m.nv:
+            raise ValueError("The output matrix has wrong number of rows")
+        if out.shape[1] != B:
+            raise ValueError("The output matrix does not have the same number of columns")
+        if not out.flags.writeable:
+            raise ValueError("The output matrix is not writeable")
+    res = np.zeros((m.nv, B), dtype=dtype, order="F")
     code = RBD_F64 if dtype == np.float64 else RBD_F32
-    if B == 0:
-        return out.astype(np.float64)
-    f = getattr(lib(), fn_name)
-    _check(f(pool.ctx.handle, m.handle, _i64(B), code, q.ctypes.data_as(ctypes.c_void_p), _i64(m.nq),
-             v.ctypes.data_as(ctypes.c_void_p), _i64(m.nv), x.ctypes.data_as(ctypes.c_void_p), _i64(m.nv),
-             out.ctypes.data_as(ctypes.c_void_p), _i64(m.nv), ctypes.c_uint32(RBD_HOST_PTRS)))
-    return out.astype(np.float64)
+    if B > 0:
+        f = getattr(lib(), fn_name)
+        _check(f(pool.ctx.handle, m.handle, _i64(B), code, q.ctypes.data_as(ctypes.c_void_p), _i64(m.nq),
+                 v.ctypes.data_as(ctypes.c_void_p), _i64(m.nv), x.ctypes.data_as(ctypes.c_void_p), _i64(m.nv),
+                 res.ctypes.data_as(ctypes.c_void_p), _i64(m.nv), ctypes.c_uint32(RBD_HOST_PTRS)))
+    if out is not None:
+        out[...] = res
+        return None
+    return res.astype(np.float64)
 
 
-def rneaInParallel(num_threads, pool, q, v, a, dtype=np.float64):
-    """tau = rnea(q[:, i], v[:, i], a[:, i]) for each column (algorithm/parallel/rnea.hpp:39-83)."""
-    return _host_call("rbd_rnea", num_threads, pool, q, v, a, dtype)
+def rneaInParallel(num_threads, pool, q, v, a, tau=None, dtype=np.float64):
+    """tau = rnea(q[:, i], v[:, i], a[:, i]) for each column (algorithm/parallel/rnea.hpp:39-83).
+
+    Both overloads of the reference's binding (bindings/python/algorithm/parallel/
+    rnea.cpp:16-24, 54-55): without `tau` the result is returned; with `tau` (an nv x B
+    array) it is written there and None is returned.  FP64 by default, like the
+    reference; dtype=np.float32 computes in FP32."""
+    return _host_call("rbd_rnea", num_threads, pool, q, v, a, dtype, tau)
 
 
-def abaInParallel(num_threads, pool, q, v, tau, dtype=np.float32):
+def abaInParallel(num_threads, pool, q, v, tau, a=None, dtype=np.float64):
     """ddq = aba(q[:, i], v[:, i], tau[:, i]) for each column (algorithm/parallel/aba.hpp:40-84).
 
-    The engine's forward dynamics computes in FP32 (inputs are rounded, the result is
-    returned as float64 like the reference's Eigen::MatrixXd)."""
-    return _host_call("rbd_aba", num_threads, pool, q, v, tau, dtype)
+    Both overloads of the reference's binding (bindings/python/algorithm/parallel/
+    aba.cpp:16-24, 54-55): with `a` (an nv x B array) the result is written there and None
+    is returned.  FP64 by default, like the reference; dtype=np.float32 computes in FP32."""
+    return _host_call("rbd_aba", num_threads, pool, q, v, tau, dtype, a)

@@ -1,8 +1,6 @@
 // instantiate.hpp -- per-topology kernel instantiation and launcher tables.
 #pragma once
 
-#include <cstdlib>
-
 #include "ops.hpp"
 #include "k_dyn.hpp"
 #include "k_crba.hpp"
@@ -16,12 +14,8 @@ namespace rbd {
 inline unsigned gridFor(int64_t B) { return (unsigned)((B + kTile - 1) / kTile); }
 
 template<class TP, class S>
-void launchRnea(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> out) {
+void launchRnea(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> out) {
   hipLaunchKernelGGL((k_rnea<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, out);
-}
-inline int envInt(const char* name) {
-  const char* e = std::getenv(name);
-  return e ? std::atoi(e) : 0;
 }
 // Below about four tiles per CU the sweeps are latency-bound with one wave per tile.
 inline int wavesForBatch(int64_t B) {
@@ -55,13 +49,11 @@ inline int residentWaves(const void* kernel, int threads) {
   }
   return blocks * (threads / 64);
 }
-// RBD_ABA_WAVES=1|2|4 fixes the waves per tile.  Default: chosen from the batch size, and
-// four waves at every size when the four-wave kernel keeps at least twice as many waves
+// Waves per tile: chosen from the batch size, and four waves at every size when the four-wave kernel keeps at least twice as many waves
 // resident per CU as the one-wave kernel (humanoid: 389 -> 243 registers per lane, one ->
 // two waves per SIMD; ABA at 65,536 configurations 67.7 -> 61.7 us).
 template<class TP, class S>
-void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
-  static const int forced = envInt("RBD_ABA_WAVES");
+void launchAba(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
   bool denser = false;
   if constexpr (abaSplitPays<TP, 4>()) {
     static const bool d = [] {
@@ -70,7 +62,7 @@ void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, C
     }();
     denser = d;
   }
-  const int waves = forced > 0 ? forced : denser ? 4 : wavesForBatch(B);
+  const int waves = denser ? 4 : wavesForBatch(B);
   if constexpr (abaSplitPays<TP, 4>()) {
     if (waves >= 4) {
       hipLaunchKernelGGL((k_aba_waves<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, tau, out);
@@ -86,36 +78,34 @@ void launchAba(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, C
   hipLaunchKernelGGL((k_aba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, tau, out);
 }
 // CRBA: two waves per tile (output emission split) keep two waves resident per SIMD at
-// large batches; RBD_CRBA_WAVES=1|2 fixes the choice
+// large batches
 template<class TP, class S>
-void launchCrba(hipStream_t st, const S* mc, int64_t B, CView<S> q, View<S> M, int upper) {
-  static const int forced = envInt("RBD_CRBA_WAVES");
-  if (forced == 1 || TP::NJ <= 3)
+void launchCrba(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, View<S> M, int upper) {
+  if (TP::NJ <= 3)
     hipLaunchKernelGGL((k_crba<TP, S, 1>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, M, upper);
   else
     hipLaunchKernelGGL((k_crba<TP, S, 2>), dim3(gridFor(B)), dim3(kTile * 2), 0, st, mc, B, q, M, upper);
 }
-template<class TP, class S> void launchChol(hipStream_t st, int64_t B, CView<S> M, View<double> U, View<double> D) {
+template<class TP, class S> void launchChol(hipStream_t st, GM, int64_t B, CView<S> M, View<double> U, View<double> D) {
   hipLaunchKernelGGL((k_chol<TP, S, double>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
 }
 // FP32 factor (ABA derivatives, computeMinverse): M staged into LDS first when the batch
-// leaves about one tile per CU or fewer (RBD_CHOL_STAGED=0|1 fixes the choice)
-template<class TP> void launchChol32(hipStream_t st, int64_t B, CView<float> M, View<float> U, View<float> D) {
-  static const int forced = std::getenv("RBD_CHOL_STAGED") ? envInt("RBD_CHOL_STAGED") : -1;
-  const bool staged = forced >= 0 ? forced != 0 : (B + kTile - 1) / kTile <= 512;
+// leaves about two tiles per CU or fewer
+template<class TP> void launchChol32(hipStream_t st, GM, int64_t B, CView<float> M, View<float> U, View<float> D) {
+  const bool staged = (B + kTile - 1) / kTile <= 512;
   if (staged && cholStagedLdsBytes<TP, float>() <= 160 * 1024)
     hipLaunchKernelGGL((k_chol_staged<TP, float, float>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
   else
     hipLaunchKernelGGL((k_chol<TP, float, float>), dim3(gridFor(B)), dim3(kTile), 0, st, B, M, U, D);
 }
 template<class TP, class S>
-void launchCholSolve(hipStream_t st, int64_t B, CView<double> U, CView<double> D, View<S> x) {
+void launchCholSolve(hipStream_t st, GM, int64_t B, CView<double> U, CView<double> D, View<S> x) {
   hipLaunchKernelGGL((k_chol64_solve<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, B, U, D, x);
 }
 // RNEA derivatives: with few tiles, 4 waves per tile, each running a share of the subtrees
 // (WaveSplit), shorten the critical path
 template<class TP, class S>
-void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
+void launchRneaD(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
                  View<S> M, int mask, int waves) {
   if (waves <= 0) waves = wavesForBatch(B);
   if constexpr (rneadSplitPays<TP, 4>()) {
@@ -128,7 +118,7 @@ void launchRneaD(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v,
   hipLaunchKernelGGL((k_rnead<TP, S, 1>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, dq, dv, M, mask);
 }
 template<class TP, class S>
-void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv,
+void launchAbaDSolve(hipStream_t st, GM, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv,
                      int jobBegin) {
   hipLaunchKernelGGL((k_abad_solve<TP, S>), dim3(gridFor(B)), dim3(kTile * kSolveWaves), 0, st, B, U, D, Xq, Xv,
                      Minv, jobBegin);
@@ -136,13 +126,13 @@ void launchAbaDSolve(hipStream_t st, int64_t B, CView<S> U, CView<S> D, View<S> 
 
 
 template<class TP, class S>
-void launchKin(hipStream_t st, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> oM, View<S> vo,
+void launchKin(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> oM, View<S> vo,
                View<S> ao, View<S> J) {
   hipLaunchKernelGGL((k_kin<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, a, oM, vo, ao, J);
 }
 
 template<class TP, class S>
-void launchKkt(hipStream_t st, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J, CView<S> tau,
+void launchKkt(hipStream_t st, GM, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J, CView<S> tau,
                CView<S> nle, CView<S> gamma, double damping, int impulse, double rcoef, View<S> ddq, View<S> lam) {
   using K = KktShape<S>;
   hipLaunchKernelGGL((k_kkt<TP, S>), dim3((unsigned)((B + K::T - 1) / K::T)), dim3(64 * K::W), 0, st, B, U, D, nc, J,

@@ -554,7 +554,7 @@ int rbd_mlp_forward_dynamics(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q
     hipLaunchKernelGGL(k_outputs, dim3(blocksFor(B * nv)), dim3(256), 0, st, nv, m->kout, B, m->out.p, m->ymean(),
                        m->ystd(), ddq, ldd);
     float* tc = m->tauChk.get((size_t)nv * B);
-    md->ops->rnea_f32(st, md->mc32, B, CView<float>{q, ldq, 1}, CView<float>{v, ldv, 1}, CView<float>{ddq, ldd, 1},
+    md->ops->rnea_f32(st, md->gen.get(), md->mc32, B, CView<float>{q, ldq, 1}, CView<float>{v, ldv, 1}, CView<float>{ddq, ldd, 1},
                       View<float>{tc, B, 1});
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, m->count.p, (int64_t)1, 0u);
     float* cq = m->cq.get((size_t)nq * B);
@@ -568,7 +568,7 @@ int rbd_mlp_forward_dynamics(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q
     HIP_CHECK(hipStreamSynchronize(st));
     if (cnt > 0) {
       float* cdd = m->cdd.get((size_t)nv * B);
-      md->ops->aba_f32(st, md->mc32, (int64_t)cnt, CView<float>{cq, B, 1}, CView<float>{cv, B, 1},
+      md->ops->aba_f32(st, md->gen.get(), md->mc32, (int64_t)cnt, CView<float>{cq, B, 1}, CView<float>{cv, B, 1},
                        CView<float>{ct, B, 1}, View<float>{cdd, B, 1});
       hipLaunchKernelGGL(k_scatter, dim3(blocksFor((int64_t)cnt * nv)), dim3(256), 0, st, nv, (int64_t)cnt, idx, cdd,
                          B, ddq, ldd);

@@ -188,7 +188,7 @@ int vec3Op(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* x0, int64_t 
     const S* d2 = (const S*)st.in(x2, v2.extent * sizeof(S));
     S* dout = (S*)st.out(out, vo.extent * sizeof(S), ldo != nv || (flags & RBD_LAYOUT_BATCH));
     const S* mc = std::is_same<S, float>::value ? (const S*)m->mc32 : (const S*)m->mc64;
-    fn(ctx->stream, mc, B, CView<S>{d0, v0.rs, v0.bs}, CView<S>{d1, v1.rs, v1.bs}, CView<S>{d2, v2.rs, v2.bs},
+    fn(ctx->stream, m->gen.get(), mc, B, CView<S>{d0, v0.rs, v0.bs}, CView<S>{d1, v1.rs, v1.bs}, CView<S>{d2, v2.rs, v2.bs},
        View<S>{dout, vo.rs, vo.bs});
     finishCall(ctx, flags, st);
   });
@@ -211,7 +211,10 @@ int rbd_ctx_create(int device, rbd_ctx** out) {
     c->device = device;
     try {
       HIP_CHECK(hipSetDevice(device));
-      HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      // a blocking stream: work on it is ordered with the legacy default stream, which is
+      // where PyTorch (and most callers) enqueue the producers of the inputs and the
+      // consumers of the outputs
+      HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamDefault));
     } catch (...) {
       delete c;
       throw;
@@ -243,7 +246,41 @@ int rbd_ctx_synchronize(rbd_ctx* ctx) {
   });
 }
 
-int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* d, rbd_model** out) {
+}  // extern "C"
+
+namespace {
+// entries of `t` that are not set take the run-time-topology launcher
+void fillMissing(TopoOps& t, const TopoOps& g) {
+#define RBD_FILL_OP(F) \
+  if (!t.F) t.F = g.F;
+  RBD_FILL_OP(rnea_f32) RBD_FILL_OP(rnea_f64) RBD_FILL_OP(aba_f32) RBD_FILL_OP(aba_f64)
+  RBD_FILL_OP(crba_f32) RBD_FILL_OP(crba_f64) RBD_FILL_OP(chol_f32) RBD_FILL_OP(chol_f64)
+  RBD_FILL_OP(cholsolve_f32) RBD_FILL_OP(cholsolve_f64) RBD_FILL_OP(rnead_f32) RBD_FILL_OP(rnead_f64)
+  RBD_FILL_OP(chol32_f32) RBD_FILL_OP(abad_solve_f32) RBD_FILL_OP(abad_solve_f64) RBD_FILL_OP(kin_f32)
+  RBD_FILL_OP(kin_f64) RBD_FILL_OP(kkt_f32) RBD_FILL_OP(kkt_f64)
+#undef RBD_FILL_OP
+}
+
+// DFS pre-order with children in increasing index order is the index order
+// (crba.hxx:573-595)
+bool isCompact(const std::vector<int>& parents) {
+  const int nj = (int)parents.size();
+  std::vector<std::vector<int>> kids(nj);
+  for (int j = 1; j < nj; ++j) kids[parents[j]].push_back(j);
+  std::vector<int> stack, order;
+  for (int k = (int)kids[0].size() - 1; k >= 0; --k) stack.push_back(kids[0][k]);
+  while (!stack.empty()) {
+    const int j = stack.back();
+    stack.pop_back();
+    order.push_back(j);
+    for (int k = (int)kids[j].size() - 1; k >= 0; --k) stack.push_back(kids[j][k]);
+  }
+  for (int k = 0; k < (int)order.size(); ++k)
+    if (order[k] != k + 1) return false;
+  return true;
+}
+
+int createModel(rbd_ctx* ctx, const rbd_model_desc* d, uint32_t mflags, rbd_model** out) {
   return guarded([&] {
     RBD_CHECK_ARG(ctx && d && out, "null argument");
     RBD_CHECK_ARG(d->njoints >= 2, "model needs at least one moving joint");
@@ -273,18 +310,19 @@ int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* d, rbd_model** out) {
       const double* Y = d->inertia + 10 * j;
       RBD_CHECK_ARG(std::isfinite(Y[0]) && Y[0] >= 0.0, "body mass must be finite and non-negative");
     }
-    int n = 0;
-    const TopoOps* const* all = registry(&n);
-    for (int k = 0; k < n && !m->ops; ++k) {
-      const TopoOps* o = all[k];
-      if (o->njoints != nj) continue;
-      if (std::memcmp(o->parent, m->parents.data(), sizeof(int) * nj) != 0) continue;
-      bool same = true;
-      for (int j = 1; j < nj; ++j) same = same && o->type[j] == m->jtype[j];
-      if (same) m->ops = o;
+    if (!(mflags & RBD_MODEL_RUNTIME_TOPOLOGY)) {
+      int n = 0;
+      const TopoOps* const* all = registry(&n);
+      for (int k = 0; k < n && !m->prebuilt; ++k) {
+        const TopoOps* o = all[k];
+        if (o->njoints != nj) continue;
+        if (std::memcmp(o->parent, m->parents.data(), sizeof(int) * nj) != 0) continue;
+        bool same = true;
+        for (int j = 1; j < nj; ++j) same = same && o->type[j] == m->jtype[j];
+        if (same) m->prebuilt = o;
+      }
     }
-    if (!m->ops) throw Error(RBD_EUNSUPPORTED, "no compiled kernels for this model topology");
-    m->compact = m->ops->compact;
+    m->compact = isCompact(m->parents) ? 1 : 0;
     m->gravity.assign(6, 0.0);
     const double g0[6] = {0, 0, -9.81, 0, 0, 0};
     for (int k = 0; k < 6; ++k) m->gravity[k] = d->gravity ? d->gravity[k] : g0[k];
@@ -309,11 +347,40 @@ int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* d, rbd_model** out) {
     const std::vector<double> mc = buildConstants(d, m->nv);
     std::vector<float> mcf(mc.begin(), mc.end());
     setDevice(ctx);
+    m->gen.reset(buildGenericModel(nj, m->parents.data(), m->jtype.data()));
+    m->tab = m->prebuilt ? *m->prebuilt : TopoOps{};
+    fillMissing(m->tab, *genericOps());
+    m->tab.name = m->prebuilt ? m->prebuilt->name : "generic";
+    m->tab.njoints = nj;
+    m->tab.parent = m->parents.data();
+    m->tab.type = m->jtype.data();
+    m->tab.nq = m->nq;
+    m->tab.nv = m->nv;
+    m->tab.compact = m->compact;
+    m->tab.nnzU = (int)m->patRows.size();
+    m->ops = &m->tab;
     HIP_CHECK(hipMalloc(&m->mc64, mc.size() * sizeof(double)));
     HIP_CHECK(hipMalloc(&m->mc32, mcf.size() * sizeof(float)));
     HIP_CHECK(hipMemcpy(m->mc64, mc.data(), mc.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(m->mc32, mcf.data(), mcf.size() * sizeof(float), hipMemcpyHostToDevice));
     *out = m.release();
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbd_model_create(rbd_ctx* ctx, const rbd_model_desc* d, rbd_model** out) { return createModel(ctx, d, 0, out); }
+
+int rbd_model_create_ex(rbd_ctx* ctx, const rbd_model_desc* d, uint32_t model_flags, rbd_model** out) {
+  return createModel(ctx, d, model_flags, out);
+}
+
+int rbd_model_kernels(const rbd_model* m, int32_t* prebuilt) {
+  return guarded([&] {
+    RBD_CHECK_ARG(m && prebuilt, "null argument");
+    *prebuilt = m->prebuilt ? 1 : 0;
   });
 }
 
@@ -377,10 +444,10 @@ int rbd_crba(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void*
     void* dM = st.out(M, vM.extent * es, true);
     const int upper = (flags & RBD_UPPER_ONLY) ? 1 : 0;
     if (f32)
-      m->ops->crba_f32(ctx->stream, m->mc32, B, CView<float>{(const float*)dq, vq.rs, vq.bs},
+      m->ops->crba_f32(ctx->stream, m->gen.get(), m->mc32, B, CView<float>{(const float*)dq, vq.rs, vq.bs},
                        View<float>{(float*)dM, vM.rs, vM.bs}, upper);
     else
-      m->ops->crba_f64(ctx->stream, m->mc64, B, CView<double>{(const double*)dq, vq.rs, vq.bs},
+      m->ops->crba_f64(ctx->stream, m->gen.get(), m->mc64, B, CView<double>{(const double*)dq, vq.rs, vq.bs},
                        View<double>{(double*)dM, vM.rs, vM.bs}, upper);
     finishCall(ctx, flags, st);
   });
@@ -421,7 +488,7 @@ void kinCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
   const S* mc;
   if constexpr (f32) mc = m->mc32;
   else mc = m->mc64;
-  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, mc, B, CView<S>{dq, vq.rs, vq.bs}, CView<S>{dv, vv.rs, vv.bs},
+  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, m->gen.get(), mc, B, CView<S>{dq, vq.rs, vq.bs}, CView<S>{dv, vv.rs, vv.bs},
                                         CView<S>{da, va.rs, va.bs}, View<S>{dM, vM.rs, vM.bs},
                                         View<S>{dvo, vva.rs, vva.bs}, View<S>{dao, vva.rs, vva.bs},
                                         View<S>{dJ, vJ.rs, vJ.bs});
@@ -475,10 +542,10 @@ int rbd_cholesky_decompose(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtyp
     double* dU = (double*)st.out(U, vU.extent * 8, true);
     double* dD = (double*)st.out(D, vD.extent * 8, true);
     if (f32)
-      m->ops->chol_f32(ctx->stream, B, CView<float>{(const float*)dM, vM.rs, vM.bs}, View<double>{dU, vU.rs, vU.bs},
+      m->ops->chol_f32(ctx->stream, m->gen.get(), B, CView<float>{(const float*)dM, vM.rs, vM.bs}, View<double>{dU, vU.rs, vU.bs},
                        View<double>{dD, vD.rs, vD.bs});
     else
-      m->ops->chol_f64(ctx->stream, B, CView<double>{(const double*)dM, vM.rs, vM.bs},
+      m->ops->chol_f64(ctx->stream, m->gen.get(), B, CView<double>{(const double*)dM, vM.rs, vM.bs},
                        View<double>{dU, vU.rs, vU.bs}, View<double>{dD, vD.rs, vD.bs});
     finishCall(ctx, flags, st);
   });
@@ -503,32 +570,117 @@ int rbd_cholesky_solve(rbd_ctx* ctx, const rbd_model* m, int64_t B, const double
     const double* dD = (const double*)st.in(D, vD.extent * 8);
     void* dx = st.out(x, vx.extent * es, true);
     if (f32)
-      m->ops->cholsolve_f32(ctx->stream, B, CView<double>{dU, vU.rs, vU.bs}, CView<double>{dD, vD.rs, vD.bs},
+      m->ops->cholsolve_f32(ctx->stream, m->gen.get(), B, CView<double>{dU, vU.rs, vU.bs}, CView<double>{dD, vD.rs, vD.bs},
                             View<float>{(float*)dx, vx.rs, vx.bs});
     else
-      m->ops->cholsolve_f64(ctx->stream, B, CView<double>{dU, vU.rs, vU.bs}, CView<double>{dD, vD.rs, vD.bs},
+      m->ops->cholsolve_f64(ctx->stream, m->gen.get(), B, CView<double>{dU, vU.rs, vU.bs}, CView<double>{dD, vD.rs, vD.bs},
                             View<double>{(double*)dx, vx.rs, vx.bs});
     finishCall(ctx, flags, st);
   });
 }
 
+}  // extern "C"
+
 namespace {
-// RBD_RNEAD_WAVES=1|2|4 fixes the waves per tile of the RNEA-derivative sweep (default:
-// chosen from the batch size)
-int rneadWavesOverride() {
-  static const int w = [] {
-    const char* e = std::getenv("RBD_RNEAD_WAVES");
-    return e ? std::atoi(e) : 0;
-  }();
-  return w;
-}
 void checkDerivativeModel(const rbd_model* m) {
   if (!m->compact) throw Error(RBD_EUNSUPPORTED, "derivatives: the model is not DFS-compact");
   if (m->gravity[3] != 0.0 || m->gravity[4] != 0.0 || m->gravity[5] != 0.0)
     throw Error(RBD_EINVAL, "The gravity must be a pure force vector, no angular part");
-  if (!m->ops->rnead_f32) throw Error(RBD_EUNSUPPORTED, "derivatives: no kernel for this topology");
+}
+
+template<class S> const S* mcOf(const rbd_model* m) {
+  if constexpr (std::is_same_v<S, float>) return m->mc32;
+  else return m->mc64;
+}
+template<class S> LaunchRneaD<S> rneadOf(const rbd_model* m) {
+  if constexpr (std::is_same_v<S, float>) return m->ops->rnead_f32;
+  else return m->ops->rnead_f64;
+}
+
+template<class S>
+void rneaDerivCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
+                   const void* a, int64_t lda, void* dq, void* dv, void* da, int64_t ldout, uint32_t flags) {
+  const int nv = m->nv;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v");
+  auto va = argView<S>(nv, B, lda, flags, "a");
+  auto vo = argView<S>(nv * nv, B, ldout, flags, "outputs");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq_ = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv_ = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* da_ = (const S*)st.in(a, va.extent * sizeof(S));
+  S* o1 = (S*)st.out(dq, vo.extent * sizeof(S), true);
+  S* o2 = (S*)st.out(dv, vo.extent * sizeof(S), true);
+  S* o3 = (S*)st.out(da, vo.extent * sizeof(S), true);
+  const int mask = (dq ? 1 : 0) | (dv ? 2 : 0) | (da ? 4 : 0);
+  rneadOf<S>(m)(ctx->stream, m->gen.get(), mcOf<S>(m), B, CView<S>{dq_, vq.rs, vq.bs}, CView<S>{dv_, vv.rs, vv.bs},
+                CView<S>{da_, va.rs, va.bs}, View<S>{o1, vo.rs, vo.bs}, View<S>{o2, vo.rs, vo.bs},
+                View<S>{o3, vo.rs, vo.bs}, mask, 0);
+  finishCall(ctx, flags, st);
+}
+
+// ABA (ddq), RNEA derivatives at a = ddq (dtau_dq, dtau_dv, M into the outputs), the tree
+// factor of M, then Minv and -Minv dtau in place.  FP32: the prebuilt kernels where the
+// topology has them; FP64: the run-time-topology kernels.
+template<class S>
+void abaDerivCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
+                  const void* tau, int64_t ldtau, void* ddq_dq, void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq,
+                  int64_t ldddq, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const int nv = m->nv, nnz = m->ops->nnzU;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v");
+  auto vt = argView<S>(nv, B, ldtau, flags, "tau");
+  auto vo = argView<S>(nv * nv, B, ldout, flags, "outputs");
+  ArgView<S> va{};
+  if (ddq) va = argView<S>(nv, B, ldddq, flags, "ddq");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq_ = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv_ = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* dt_ = (const S*)st.in(tau, vt.extent * sizeof(S));
+  S* Xq = (S*)st.out(ddq_dq, vo.extent * sizeof(S), true);
+  S* Xv = (S*)st.out(ddq_dv, vo.extent * sizeof(S), true);
+  S* Mi = (S*)st.out(ddq_dtau, vo.extent * sizeof(S), true);
+  // workspace: U and D of the factor, then ddq (batch-fastest nv x B) unless the caller wants it
+  const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
+  S* wU = (S*)ctx->workspace((nU + nD + (size_t)nv * B) * sizeof(S));
+  S* wD = wU + nU;
+  S* wA = wD + nD;
+  View<S> vddq{wA, B, 1};
+  if (ddq) {
+    S* dd = (S*)st.out(ddq, va.extent * sizeof(S), true);
+    vddq = View<S>{dd, va.rs, va.bs};
+  }
+  const TopoOps& o = *m->ops;
+  const CView<S> cq{dq_, vq.rs, vq.bs}, cv{dv_, vv.rs, vv.bs};
+  // 1. ddq = aba(q, v, tau)
+  if constexpr (f32) o.aba_f32(ctx->stream, m->gen.get(), m->mc32, B, cq, cv, CView<S>{dt_, vt.rs, vt.bs}, vddq);
+  else o.aba_f64(ctx->stream, m->gen.get(), m->mc64, B, cq, cv, CView<S>{dt_, vt.rs, vt.bs}, vddq);
+  // 2. dtau_dq, dtau_dv, M at a = ddq, into the output buffers
+  rneadOf<S>(m)(ctx->stream, m->gen.get(), mcOf<S>(m), B, cq, cv, CView<S>{vddq.p, vddq.rs, vddq.bs},
+                View<S>{Xq, vo.rs, vo.bs}, View<S>{Xv, vo.rs, vo.bs}, View<S>{Mi, vo.rs, vo.bs}, 7, 0);
+  // 3. tree factor of M, then Minv and -Minv dtau in place
+  if constexpr (f32) {
+    o.chol32_f32(ctx->stream, m->gen.get(), B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1},
+                 View<float>{wD, B, 1});
+    o.abad_solve_f32(ctx->stream, m->gen.get(), B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1},
+                     View<float>{Xq, vo.rs, vo.bs}, View<float>{Xv, vo.rs, vo.bs}, View<float>{Mi, vo.rs, vo.bs}, 0);
+  } else {
+    o.chol_f64(ctx->stream, m->gen.get(), B, CView<double>{Mi, vo.rs, vo.bs}, View<double>{wU, B, 1},
+               View<double>{wD, B, 1});
+    o.abad_solve_f64(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1},
+                     View<double>{Xq, vo.rs, vo.bs}, View<double>{Xv, vo.rs, vo.bs}, View<double>{Mi, vo.rs, vo.bs},
+                     0);
+  }
+  finishCall(ctx, flags, st);
 }
 }  // namespace
+
+extern "C" {
 
 int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
                          const void* v, int64_t ldv, const void* a, int64_t lda, void* dq, void* dv, void* da,
@@ -536,27 +688,9 @@ int rbd_rnea_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
   return guarded([&] {
     checkCommon(ctx, m, B, dtype);
     RBD_CHECK_ARG(B == 0 || (q && v && a), "null input argument");
-    if (dtype != RBD_F32) throw Error(RBD_EUNSUPPORTED, "rnea_derivatives: FP32 only");
     checkDerivativeModel(m);
-    const int nv = m->nv;
-    auto vq = argView<float>(m->nq, B, ldq, flags, "q");
-    auto vv = argView<float>(nv, B, ldv, flags, "v");
-    auto va = argView<float>(nv, B, lda, flags, "a");
-    auto vo = argView<float>(nv * nv, B, ldout, flags, "outputs");
-    if (B == 0) return;
-    setDevice(ctx);
-    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
-    const float* dq_ = (const float*)st.in(q, vq.extent * 4);
-    const float* dv_ = (const float*)st.in(v, vv.extent * 4);
-    const float* da_ = (const float*)st.in(a, va.extent * 4);
-    float* o1 = (float*)st.out(dq, vo.extent * 4, true);
-    float* o2 = (float*)st.out(dv, vo.extent * 4, true);
-    float* o3 = (float*)st.out(da, vo.extent * 4, true);
-    const int mask = (dq ? 1 : 0) | (dv ? 2 : 0) | (da ? 4 : 0);
-    m->ops->rnead_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
-                      CView<float>{da_, va.rs, va.bs}, View<float>{o1, vo.rs, vo.bs}, View<float>{o2, vo.rs, vo.bs},
-                      View<float>{o3, vo.rs, vo.bs}, mask, rneadWavesOverride());
-    finishCall(ctx, flags, st);
+    if (dtype == RBD_F32) rneaDerivCall<float>(ctx, m, B, q, ldq, v, ldv, a, lda, dq, dv, da, ldout, flags);
+    else rneaDerivCall<double>(ctx, m, B, q, ldq, v, ldv, a, lda, dq, dv, da, ldout, flags);
   });
 }
 
@@ -566,48 +700,11 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, 
   return guarded([&] {
     checkCommon(ctx, m, B, dtype);
     RBD_CHECK_ARG(B == 0 || (q && v && tau && ddq_dq && ddq_dv && ddq_dtau), "null array argument");
-    if (dtype != RBD_F32) throw Error(RBD_EUNSUPPORTED, "aba_derivatives: FP32 only");
     checkDerivativeModel(m);
-    const int nv = m->nv, nnz = m->ops->nnzU;
-    auto vq = argView<float>(m->nq, B, ldq, flags, "q");
-    auto vv = argView<float>(nv, B, ldv, flags, "v");
-    auto vt = argView<float>(nv, B, ldtau, flags, "tau");
-    auto vo = argView<float>(nv * nv, B, ldout, flags, "outputs");
-    ArgView<float> va{};
-    if (ddq) va = argView<float>(nv, B, ldddq, flags, "ddq");
-    if (B == 0) return;
-    setDevice(ctx);
-    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
-    const float* dq_ = (const float*)st.in(q, vq.extent * 4);
-    const float* dv_ = (const float*)st.in(v, vv.extent * 4);
-    const float* dt_ = (const float*)st.in(tau, vt.extent * 4);
-    float* Xq = (float*)st.out(ddq_dq, vo.extent * 4, true);
-    float* Xv = (float*)st.out(ddq_dv, vo.extent * 4, true);
-    float* Mi = (float*)st.out(ddq_dtau, vo.extent * 4, true);
-    // workspace: ddq (batch-fastest nv x B) unless the caller wants it, then U and D
-    const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
-    float* wsp = (float*)ctx->workspace((nU + nD + (size_t)nv * B) * 4);
-    float* wU = wsp;
-    float* wD = wU + nU;
-    float* wA = wD + nD;
-    View<float> vddq{wA, B, 1};
-    if (ddq) {
-      float* dd = (float*)st.out(ddq, va.extent * 4, true);
-      vddq = View<float>{dd, va.rs, va.bs};
-    }
-    // 1. ddq = aba(q, v, tau)
-    m->ops->aba_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
-                    CView<float>{dt_, vt.rs, vt.bs}, vddq);
-    // 2. dtau_dq, dtau_dv, M at a = ddq, into the output buffers
-    m->ops->rnead_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, CView<float>{dv_, vv.rs, vv.bs},
-                      CView<float>{vddq.p, vddq.rs, vddq.bs}, View<float>{Xq, vo.rs, vo.bs},
-                      View<float>{Xv, vo.rs, vo.bs}, View<float>{Mi, vo.rs, vo.bs}, 7, rneadWavesOverride());
-    // 3. tree factor of M, then Minv and -Minv dtau in place
-    m->ops->chol32_f32(ctx->stream, B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1}, View<float>{wD, B, 1});
-    m->ops->abad_solve_f32(ctx->stream, B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1},
-                           View<float>{Xq, vo.rs, vo.bs}, View<float>{Xv, vo.rs, vo.bs},
-                           View<float>{Mi, vo.rs, vo.bs}, 0);
-    finishCall(ctx, flags, st);
+    if (dtype == RBD_F32)
+      abaDerivCall<float>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ddq_dq, ddq_dv, ddq_dtau, ldout, ddq, ldddq, flags);
+    else
+      abaDerivCall<double>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ddq_dq, ddq_dv, ddq_dtau, ldout, ddq, ldddq, flags);
   });
 }
 
@@ -633,9 +730,9 @@ int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
       float* wU = (float*)ctx->workspace((nU + nD) * 4);
       float* wD = wU + nU;
       const View<float> vM{Mi, vo.rs, vo.bs};
-      m->ops->crba_f32(ctx->stream, m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, vM, 0);
-      m->ops->chol32_f32(ctx->stream, B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1}, View<float>{wD, B, 1});
-      m->ops->abad_solve_f32(ctx->stream, B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1}, vM, vM, vM, 2 * nv);
+      m->ops->crba_f32(ctx->stream, m->gen.get(), m->mc32, B, CView<float>{dq_, vq.rs, vq.bs}, vM, 0);
+      m->ops->chol32_f32(ctx->stream, m->gen.get(), B, CView<float>{Mi, vo.rs, vo.bs}, View<float>{wU, B, 1}, View<float>{wD, B, 1});
+      m->ops->abad_solve_f32(ctx->stream, m->gen.get(), B, CView<float>{wU, B, 1}, CView<float>{wD, B, 1}, vM, vM, vM, 2 * nv);
       finishCall(ctx, flags, st);
       return;
     }
@@ -650,12 +747,12 @@ int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
     const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
     double* wU = (double*)ctx->workspace((nU + nD) * 8);
     double* wD = wU + nU;
-    m->ops->crba_f64(ctx->stream, m->mc64, B, CView<double>{dq_, vq.rs, vq.bs}, View<double>{Mi, vo.rs, vo.bs}, 0);
-    m->ops->chol_f64(ctx->stream, B, CView<double>{Mi, vo.rs, vo.bs}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    m->ops->crba_f64(ctx->stream, m->gen.get(), m->mc64, B, CView<double>{dq_, vq.rs, vq.bs}, View<double>{Mi, vo.rs, vo.bs}, 0);
+    m->ops->chol_f64(ctx->stream, m->gen.get(), B, CView<double>{Mi, vo.rs, vo.bs}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
     hipLaunchKernelGGL(k_identity, dim3((unsigned)((nv * nv * B + 255) / 256)), dim3(256), 0, ctx->stream, nv, B, Mi,
                        vo.rs, vo.bs);
     for (int c = 0; c < nv; ++c)
-      m->ops->cholsolve_f64(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1},
+      m->ops->cholsolve_f64(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1},
                             View<double>{Mi + (int64_t)c * nv * vo.rs, vo.rs, vo.bs});
     finishCall(ctx, flags, st);
   });
@@ -710,7 +807,7 @@ void framesCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int6
   const S* mc;
   if constexpr (f32) mc = m->mc32;
   else mc = m->mc64;
-  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, mc, B, CView<S>{dq, vq.rs, vq.bs}, CView<S>{}, CView<S>{},
+  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, m->gen.get(), mc, B, CView<S>{dq, vq.rs, vq.bs}, CView<S>{}, CView<S>{},
                                         View<S>{wO, B, 1}, View<S>{}, View<S>{}, View<S>{wJ, B, 1});
   hipLaunchKernelGGL(k_frames<S>, dim3((unsigned)((B + 255) / 256), (unsigned)nf), dim3(256), 0, ctx->stream, B, nv,
                      dpar, dpl, dsup, CView<S>{wO, B, 1}, CView<S>{wJ, B, 1}, rf, View<S>{dM, vM.rs, vM.bs},
@@ -761,18 +858,18 @@ void cfdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
   const TopoOps& o = *m->ops;
   const CView<S> cq{dq, vq.rs, vq.bs}, cv{dv, vv.rs, vv.bs};
   if constexpr (f32) {
-    o.crba_f32(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
-    o.chol_f32(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    o.crba_f32(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f32(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
   } else {
-    o.crba_f64(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
-    o.chol_f64(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    o.crba_f64(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f64(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
   }
   // nle = rnea(q, v, 0): the acceleration view repeats one zero (both strides 0)
   hipLaunchKernelGGL(k_fill_zero<S>, dim3(1), dim3(64), 0, ctx->stream, wZ, (int64_t)1);
-  if constexpr (f32) o.rnea_f32(ctx->stream, mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
-  else o.rnea_f64(ctx->stream, mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
+  if constexpr (f32) o.rnea_f32(ctx->stream, m->gen.get(), mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
+  else o.rnea_f64(ctx->stream, m->gen.get(), mc, B, cq, cv, CView<S>{wZ, 0, 0}, View<S>{wN, B, 1});
   const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
-  kkt(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
+  kkt(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
       CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{dg, vg.rs, vg.bs}, damping, 0, 0.0,
       View<S>{da, va.rs, va.bs}, View<S>{dl, vl.rs, vl.bs});
   finishCall(ctx, flags, st);
@@ -808,14 +905,14 @@ void impulseCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int
   const TopoOps& o = *m->ops;
   const CView<S> cq{dq, vq.rs, vq.bs};
   if constexpr (f32) {
-    o.crba_f32(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
-    o.chol_f32(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    o.crba_f32(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f32(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
   } else {
-    o.crba_f64(ctx->stream, mc, B, cq, View<S>{wM, B, 1}, 0);
-    o.chol_f64(ctx->stream, B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    o.crba_f64(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f64(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
   }
   const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
-  kkt(ctx->stream, B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
+  kkt(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
       CView<S>{dv, vv.rs, vv.bs}, CView<S>{}, CView<S>{}, damping, 1, rcoef, View<S>{da, va.rs, va.bs},
       View<S>{di, vi.rs, vi.bs});
   finishCall(ctx, flags, st);
@@ -1005,9 +1102,9 @@ void enqueue(rbd_ctx* ctx, MultiPlan& p, const rbd_batch_group* g, int ng) {
     const rbd_model* m = g[k].model;
     const int64_t n = g[k].count;
     if (n == 0) continue;
-    m->ops->rnea_f32(ctx->stream, m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
+    m->ops->rnea_f32(ctx->stream, m->gen.get(), m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
                      CView<float>{g[k].a, n, 1}, View<float>{g[k].tau_out, n, 1});
-    m->ops->aba_f32(ctx->stream, m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
+    m->ops->aba_f32(ctx->stream, m->gen.get(), m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
                     CView<float>{g[k].tau, n, 1}, View<float>{g[k].ddq_out, n, 1});
   }
 }
@@ -1127,7 +1224,7 @@ int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t ngroups, const rbd_batch_group* gro
       for (const void* out : {(const void*)g.tau_out, (const void*)g.ddq_out})
         for (const void* in : {(const void*)g.q, (const void*)g.v, (const void*)g.a, (const void*)g.tau})
           RBD_CHECK_ARG(g.count == 0 || out != in, "group outputs must not alias its inputs");
-      need.push_back(g.model->ops);
+      need.push_back(g.model->prebuilt);
     }
     setDevice(ctx);
     std::lock_guard<std::mutex> lock(g_multiMutex);
@@ -1156,12 +1253,16 @@ int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t ngroups, const rbd_batch_group* gro
           const rbd_batch_group& g = groups[k];
           int topo = 0;
           for (int t = 0; t < p->mix->ntopo; ++t)
-            if (p->mix->topo[t] == g.model->ops) topo = t;
+            if (p->mix->topo[t] == g.model->prebuilt) topo = t;
           hg[k] = MultiGroup{g.model->mc32, g.count, g.q, g.v, g.a, g.tau, g.tau_out, g.ddq_out, topo, 0};
         }
         HIP_CHECK(hipMalloc(&p->dGroups, sizeof(MultiGroup) * ngroups));
         HIP_CHECK(hipMemcpy(p->dGroups, hg.data(), sizeof(MultiGroup) * ngroups, hipMemcpyHostToDevice));
       }
+      // run-time-topology kernels size their lane memory now: nothing may allocate
+      // inside the graph capture below
+      for (int k = 0; k < ngroups; ++k)
+        if (!groups[k].model->prebuilt) genericReserveDyn(groups[k].model->gen.get(), groups[k].count);
       autotune(ctx, *p, groups, ngroups, !(flags & RBD_NO_AUTOTUNE));
       if (!(flags & RBD_NO_GRAPH)) {
         HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

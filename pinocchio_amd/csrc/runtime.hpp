@@ -4,11 +4,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/rbd.h"
+#include "generic.hpp"
 #include "ops.hpp"
 
 namespace rbd_rt {
@@ -82,7 +84,13 @@ struct rbd_ctx {
 struct rbd_model {
   rbd_ctx* ctx = nullptr;
   uint64_t serial = 0;  // unique per created model: mixed-batch plans key on it
+  // kernels: the prebuilt topology's launchers (null when the tree is not prebuilt or the
+  // run-time-topology kernels were requested), completed by the run-time-topology
+  // launchers in `tab`; `ops` points to `tab`
+  const rbd::TopoOps* prebuilt = nullptr;
+  rbd::TopoOps tab{};
   const rbd::TopoOps* ops = nullptr;
+  std::unique_ptr<rbd::GenericModel> gen;
   int njoints = 0, nq = 0, nv = 0, compact = 0;
   std::vector<int> parents, jtype, iq, iv;
   std::vector<double> gravity;

@@ -273,6 +273,89 @@ def free_body(seed: int = 7) -> ModelDesc:
     return b.build()
 
 
+def random_tree(n_revolute: int, seed: int, free_flyer: bool = True, compact: bool = True,
+                name: str = None) -> ModelDesc:
+    """Random tree of `n_revolute` revolute joints (axes uniform over X/Y/Z) on an optional
+    free-flyer root, with the BASELINE parameter distributions.  The parent of each new
+    joint is uniform over the joints added before it.  compact=True renumbers the joints
+    in depth-first pre-order (children in increasing order), as addJoint callers do;
+    compact=False keeps the insertion order, which is not DFS-ordered in general."""
+    b = _Builder(name or f"tree{n_revolute}_{seed & 0xffff:x}", seed)
+    parents, types = [0], [JT_UNIVERSE]
+    if free_flyer:
+        parents.append(0)
+        types.append(JT_FREEFLYER)
+    first = len(parents)
+    for i in range(n_revolute):
+        j = first + i
+        lo = 1 if free_flyer else 0
+        if j == lo:
+            p = 0
+        else:
+            p = lo + int(b.rng.uniform(0.0, 1.0, (1,))[0] * (j - lo))
+            p = min(p, j - 1)
+        parents.append(p)
+        types.append(AXIS["XYZ"[min(2, int(b.rng.uniform(0.0, 3.0, (1,))[0]))]])
+    nj = len(parents)
+    order = list(range(nj))
+    if compact:
+        kids = [[] for _ in range(nj)]
+        for j in range(1, nj):
+            kids[parents[j]].append(j)
+        order, stack = [0], list(reversed(kids[0]))
+        while stack:
+            j = stack.pop()
+            order.append(j)
+            stack.extend(reversed(kids[j]))
+    new_index = {old: k for k, old in enumerate(order)}
+    for old in order[1:]:
+        if types[old] == JT_FREEFLYER:
+            b.free_flyer()
+        else:
+            b.add(new_index[parents[old]], types[old])
+    return b.build()
+
+
+def serial_chain(n: int, seed: int, axes: str = "XYZ") -> ModelDesc:
+    """n revolute joints in one chain, axes cycling through `axes` (a deep tree)."""
+    b = _Builder(f"chain{n}", seed)
+    b.chain(0, (axes * n)[:n])
+    return b.build()
+
+
+def from_pinocchio(model) -> ModelDesc:
+    """Flatten a pinocchio Model (the Python binding's pinocchio.Model, or any object with
+    the same attributes: njoints, parents, joints[i].shortname(), jointPlacements[i]
+    .rotation / .translation, inertias[i].mass / .lever / .inertia, armature, gravity
+    .linear / .angular) into a ModelDesc.  Joint types FreeFlyer (at a root) and
+    RX / RY / RZ only, as the engine supports."""
+    kinds = {"JointModelFreeFlyer": JT_FREEFLYER, "JointModelRX": JT_RX, "JointModelRY": JT_RY,
+             "JointModelRZ": JT_RZ}
+    nj = int(model.njoints)
+    parents = np.array([int(p) for p in list(model.parents)[:nj]], dtype=np.int32)
+    jtype = np.zeros(nj, dtype=np.int32)
+    placement = np.zeros((nj, 12))
+    inertia = np.zeros((nj, 10))
+    placement[0, :9] = np.eye(3).ravel()
+    for j in range(1, nj):
+        kind = model.joints[j].shortname()
+        if kind not in kinds:
+            raise NotImplementedError(f"joint {j}: {kind} is not supported (FreeFlyer, RX, RY, RZ)")
+        jtype[j] = kinds[kind]
+        M = model.jointPlacements[j]
+        placement[j, :9] = np.asarray(M.rotation, dtype=np.float64).reshape(3, 3).ravel()
+        placement[j, 9:] = np.asarray(M.translation, dtype=np.float64).reshape(3)
+        Y = model.inertias[j]
+        Ic = np.asarray(Y.inertia, dtype=np.float64).reshape(3, 3)
+        inertia[j] = [float(Y.mass), *np.asarray(Y.lever, dtype=np.float64).reshape(3), Ic[0, 0], Ic[0, 1], Ic[1, 1],
+                      Ic[0, 2], Ic[1, 2], Ic[2, 2]]
+    g = model.gravity
+    gravity = np.concatenate([np.asarray(g.linear, dtype=np.float64).reshape(3),
+                              np.asarray(g.angular, dtype=np.float64).reshape(3)])
+    return ModelDesc(getattr(model, "name", "") or "model", parents, jtype, placement, inertia,
+                     np.asarray(model.armature, dtype=np.float64).reshape(-1), gravity)
+
+
 SAMPLE_MODELS = {
     "manipulator6": manipulator6,
     "arm7": arm7,

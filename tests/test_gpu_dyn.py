@@ -137,6 +137,16 @@ def test_dropin_host_api(gpu_ctx):
     qf, vf, tf = f32_round(q, v, tau)
     ddq_g = pa.abaInParallel(2, pool, qf, vf, tf, dtype=np.float32)
     assert rel_err_cols(ddq_g, oracle.aba(desc, qf, vf, tf)).max() < 1e-4
+    # FP64 by default, like the reference (batch == serial to 1e-12, parallel-aba.cpp:54)
+    ddq_64 = pa.abaInParallel(2, pool, q, v, tau)
+    assert rel_err_cols(ddq_64, oracle.aba(desc, q, v, tau)).max() < 1e-12
+    # output-argument overloads (bindings/python/algorithm/parallel/{rnea,aba}.cpp:54-55)
+    out = np.zeros((desc.nv, 300), order="F")
+    assert pa.rneaInParallel(4, pool, q, v, a, out) is None
+    assert np.array_equal(out, tau_g)
+    out2 = np.zeros((desc.nv, 300))
+    assert pa.abaInParallel(4, pool, q, v, tau, out2) is None
+    assert np.array_equal(out2, ddq_64)
     # argument errors of algorithm/parallel/{rnea,aba}.hpp:53-69
     with pytest.raises(ValueError):
         pa.rneaInParallel(5, pool, q, v, a)  # more threads than the pool size
@@ -144,8 +154,8 @@ def test_dropin_host_api(gpu_ctx):
         pa.rneaInParallel(1, pool, q[:-1], v, a)  # wrong q rows
     with pytest.raises(ValueError):
         pa.abaInParallel(1, pool, q, v, tau[:, :-1])  # column mismatch (also checked for tau)
-    with pytest.raises(NotImplementedError):
-        pa.abaInParallel(1, pool, q, v, tau, dtype=np.float64)  # FP64 ABA is not compiled
+    with pytest.raises(ValueError):
+        pa.rneaInParallel(1, pool, q, v, a, np.zeros((desc.nv, 299)))  # output column mismatch
 
 
 def test_bad_arguments_c_abi(gpu_ctx):

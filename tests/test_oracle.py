@@ -25,6 +25,7 @@ import pytest
 
 import oracle
 from pinocchio_amd import models as M
+from helpers import two_link_arm, two_link_closed_form
 
 ALL = ["manipulator6", "arm7", "quadruped18", "humanoid36"]
 
@@ -80,6 +81,23 @@ def test_pendulum_closed_form():
     np.testing.assert_allclose(tau, expect, rtol=1e-13, atol=1e-13)
     ddq = oracle.aba(m, q, v, expect)
     np.testing.assert_allclose(ddq, a, rtol=1e-12, atol=1e-12)
+
+
+def test_two_link_rotated_offset_closed_form():
+    """Composition order liMi = placement * joint(q), the joint-axis sign and the gravity
+    direction through a rotated placement, pinned by the planar two-link equations of
+    motion and the forward kinematics of the second mass."""
+    d = two_link_arm()
+    q = np.array([[0.3, -1.2, 2.5, 0.0], [0.9, 0.4, -2.2, 0.0]])
+    v = np.array([[0.7, -0.2, 1.1, 0.0], [-0.5, 1.3, 0.2, 0.0]])
+    a = np.array([[-0.4, 2.0, 0.5, 0.0], [1.5, -0.7, 0.3, 0.0]])
+    tau_ref, pos_ref = two_link_closed_form(q, v, a)
+    np.testing.assert_allclose(oracle.rnea(d, q, v, a), tau_ref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(oracle.aba(d, q, v, tau_ref), a, rtol=1e-10, atol=1e-10)
+    oM, _, _ = oracle.forward_kinematics(d, q)
+    # mass 2 sits at lever (l2c, 0, 0) of joint 2's frame
+    pos = np.stack([oM[b, 1, 9:] + oM[b, 1, :9].reshape(3, 3) @ np.array([0.25, 0, 0]) for b in range(4)], axis=1)
+    np.testing.assert_allclose(pos, pos_ref, rtol=1e-12, atol=1e-12)
 
 
 def test_free_body_gravity_closed_form():
