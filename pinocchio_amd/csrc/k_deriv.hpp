@@ -14,6 +14,10 @@
 // and the entries inside one joint's block use dFdq_c before the J_c x* of_c term, as the
 // reference does.
 //
+// As in CRBA (k_crba.hpp), the world frame's origin is put at the floating base: the
+// outputs are joint-space quantities, and composite inertias about a nearby origin keep
+// more FP32 bits (humanoid, 16384 configurations: ddq_dq max error 1.5e-4 -> 3.6e-5).
+//
 // computeABADerivatives (aba-derivatives.hxx:380-453) returns ddq_dq = -Minv dtau_dq,
 // ddq_dv = -Minv dtau_dv, ddq_dtau = Minv with dtau evaluated at a = ddq (the identity
 // its own Bwd2 pass implements).  The engine runs ABA (k_aba), this RNEA-derivative
@@ -150,6 +154,7 @@ __device__ __forceinline__ void rneadTile(const S* __restrict__ mc_in, int64_t B
 #pragma unroll
         for (int k = 0; k < 7; ++k) q7[k] = sq[(iq + k) * kPitch + L];
         oM[i] = freeflyerX(mc, i, q7, false);
+        oM[i].p = {S(0), S(0), S(0)};  // frame origin at the floating base
         Mot<S> vJ = zeroMot<S>(), aJ = zeroMot<S>();
 #pragma unroll
         for (int k = 0; k < 6; ++k) {

@@ -19,6 +19,9 @@
 // wave's access to one slot is one contiguous 256- (FP32) or 512-byte (FP64) row: either
 // LDS (when the model's state fits) or a per-workgroup region of a device buffer.
 //
+// CRBA and the RNEA derivatives put the origin of their world frame at each floating base,
+// as the prebuilt kernels do (k_crba.hpp): their outputs are joint-space quantities.
+//
 // Algorithms and formulations are those of the prebuilt kernels: local-frame RNEA
 // (rnea.hxx:17-161) and ABA with the bias re-associated down the tree (aba.hxx:79-293),
 // world-frame CRBA (crba.hxx:20-99, 498-548) and RNEA derivatives
@@ -326,7 +329,8 @@ __global__ __launch_bounds__(64) void g_crba(GProg g, const S* __restrict__ mc, 
       const int s0 = jr[GJ_LEVEL] * kGCrbaLvl, sp = s0 - kGCrbaLvl;
       const int sd = dbase + jr[GJ_DOFLEVEL] * kGCrbaDof;
       if ((ev & 1) == EV_PRE) {
-        const Xform<S> X = gJointX(mc, i, ty, gq, jr[GJ_IQ], b);
+        Xform<S> X = gJointX(mc, i, ty, gq, jr[GJ_IQ], b);
+        if (ty == JT_FF) X.p = {S(0), S(0), S(0)};  // frame origin at the floating base
         const Xform<S> oM = p > 0 ? compose(m.template get<Xform<S>>(sp), X) : X;
         m.put(s0, oM);
         m.put(s0 + 12, worldInertia(mc, i, oM));
@@ -466,6 +470,7 @@ __global__ __launch_bounds__(64) void g_rnead(GProg g, const S* __restrict__ mc,
         Mot<S> ov, oa;
         if (ty == JT_FF) {
           oM = X;
+          oM.p = {S(0), S(0), S(0)};  // frame origin at the floating base
           Mot<S> vJ = zeroMot<S>(), aJ = zeroMot<S>();
           for (int k = 0; k < 6; ++k) {
             const Mot<S> J = gWorldAxis(JT_FF, oM, k);
