@@ -168,6 +168,20 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int
                         int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq,
                         void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq, uint32_t flags);
 
+/* rbd_aba_derivatives without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight control or
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * rbd_aba_derivatives; FP32, device pointers, models with prebuilt kernels.  The factor of
+ * M and ddq go to a file-scope device array of 64 MiB that each GPU holds until the
+ * process ends (its address is taken once per device when a context is created), so the
+ * batch is limited to 64 MiB / (4 (nnz_upper + 2 nv)) configurations (44,979 for the
+ * 36-DoF humanoid; larger batches fail with RBD_EINVAL).  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (several contexts or threads) race. */
+int rbd_aba_derivatives_noalloc(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                                int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau,
+                                void* ddq_dq, void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq,
+                                int64_t ldddq, uint32_t flags);
+
 /* forwardKinematics (kinematics.hxx:121-145, 209-238): per configuration the world
  * placement oMi of joints 1 .. njoints-1 (12 entries per joint: R row major, then the
  * translation; element index 12 (i - 1) + k), and, when v (and a) are given, each joint's

@@ -46,6 +46,32 @@ namespace rbd_rt {
 thread_local std::string g_err;
 }  // namespace rbd_rt
 
+// Scratch of the allocation-free entry point (rbd_aba_derivatives_noalloc): the tree
+// factor of M and ddq of one call.  Every GPU the library runs on holds one copy until the
+// process ends.  Host code never uses this name as an address: kernels get the per-device
+// address taken at context creation (scratchPtr below).
+constexpr size_t kScratchFloats = size_t(16) << 20;  // 64 MiB
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+float* g_scratchPtr[kMaxDevices] = {};
+// Records the device address of `scratch` on `device` (the current device) once.
+void initScratch(int device) {
+  if (device >= kMaxDevices) return;  // the allocation-free entry point then reports an error
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (g_scratchPtr[device]) return;
+  void* p = nullptr;
+  HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratchPtr[device] = static_cast<float*>(p);
+}
+float* scratchPtr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  return device < kMaxDevices ? g_scratchPtr[device] : nullptr;
+}
+}  // namespace
+
 // batched identity matrices: element (r + c nv, b) at p[(r + c nv) rs + b bs]
 __global__ void k_identity(int nv, int64_t B, double* p, int64_t rs, int64_t bs) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -215,6 +241,7 @@ int rbd_ctx_create(int device, rbd_ctx** out) {
       // where PyTorch (and most callers) enqueue the producers of the inputs and the
       // consumers of the outputs
       HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamDefault));
+      initScratch(device);
     } catch (...) {
       delete c;
       throw;
@@ -624,10 +651,12 @@ void rneaDerivCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, i
 // ABA (ddq), RNEA derivatives at a = ddq (dtau_dq, dtau_dv, M into the outputs), the tree
 // factor of M, then Minv and -Minv dtau in place.  FP32: the prebuilt kernels where the
 // topology has them; FP64: the run-time-topology kernels.
+// fixedWs: a workspace of fixedWsElems scalars to use instead of the context's (which
+// grows by allocation); the call then allocates nothing
 template<class S>
 void abaDerivCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
                   const void* tau, int64_t ldtau, void* ddq_dq, void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq,
-                  int64_t ldddq, uint32_t flags) {
+                  int64_t ldddq, uint32_t flags, S* fixedWs = nullptr, size_t fixedWsElems = 0) {
   constexpr bool f32 = std::is_same_v<S, float>;
   const int nv = m->nv, nnz = m->ops->nnzU;
   auto vq = argView<S>(m->nq, B, ldq, flags, "q");
@@ -647,7 +676,13 @@ void abaDerivCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, in
   S* Mi = (S*)st.out(ddq_dtau, vo.extent * sizeof(S), true);
   // workspace: U and D of the factor, then ddq (batch-fastest nv x B) unless the caller wants it
   const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
-  S* wU = (S*)ctx->workspace((nU + nD + (size_t)nv * B) * sizeof(S));
+  const size_t need = nU + nD + (size_t)nv * B;
+  if (fixedWs && need > fixedWsElems)
+    throw Error(RBD_EINVAL, "aba_derivatives_noalloc: the batch needs " + std::to_string(need * sizeof(S)) +
+                                " bytes of scratch, more than the " + std::to_string(fixedWsElems * sizeof(S)) +
+                                " available (at most " + std::to_string(fixedWsElems / (nnz + 2 * nv)) +
+                                " configurations of this model)");
+  S* wU = fixedWs ? fixedWs : (S*)ctx->workspace(need * sizeof(S));
   S* wD = wU + nU;
   S* wA = wD + nD;
   View<S> vddq{wA, B, 1};
@@ -705,6 +740,29 @@ int rbd_aba_derivatives(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, 
       abaDerivCall<float>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ddq_dq, ddq_dv, ddq_dtau, ldout, ddq, ldddq, flags);
     else
       abaDerivCall<double>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ddq_dq, ddq_dv, ddq_dtau, ldout, ddq, ldddq, flags);
+  });
+}
+
+int rbd_aba_derivatives_noalloc(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                                const void* v, int64_t ldv, const void* tau, int64_t ldtau, void* ddq_dq,
+                                void* ddq_dv, void* ddq_dtau, int64_t ldout, void* ddq, int64_t ldddq,
+                                uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && v && tau && ddq_dq && ddq_dv && ddq_dtau), "null array argument");
+    checkDerivativeModel(m);
+    if (dtype != RBD_F32) throw Error(RBD_EUNSUPPORTED, "aba_derivatives_noalloc: FP32 only");
+    if (flags & RBD_HOST_PTRS)
+      throw Error(RBD_EINVAL, "aba_derivatives_noalloc: device pointers only (host staging allocates)");
+    // the prebuilt kernels keep their state on chip; the run-time-topology ones would need
+    // lane memory
+    const TopoOps* pre = m->prebuilt;
+    if (!pre || !pre->aba_f32 || !pre->rnead_f32 || !pre->chol32_f32 || !pre->abad_solve_f32)
+      throw Error(RBD_EUNSUPPORTED, "aba_derivatives_noalloc: needs a model with prebuilt kernels");
+    float* ws = scratchPtr(ctx->device);
+    if (!ws) throw Error(RBD_EDEVICE, "aba_derivatives_noalloc: no scratch address for this device");
+    abaDerivCall<float>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ddq_dq, ddq_dv, ddq_dtau, ldout, ddq, ldddq, flags, ws,
+                        kScratchFloats);
   });
 }
 
