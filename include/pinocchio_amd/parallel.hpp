@@ -126,9 +126,9 @@ inline void rneaInParallel(size_t num_threads, ModelPool& pool, const CMat& q, c
   detail::vecOp(&rbd_rnea, pool, fp32, q, v, a, tau);
 }
 
-// a.col(i) = aba(q.col(i), v.col(i), tau.col(i)); the engine computes in FP32.
+// a.col(i) = aba(q.col(i), v.col(i), tau.col(i)) for every column (FP64 by default).
 inline void abaInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v, const CMat& tau,
-                          const MMat& a) {
+                          const MMat& a, bool fp32 = false) {
   detail::checkThreads(num_threads, pool);
   detail::checkRows(q, pool.nq(), "The configuration matrix");
   detail::checkRows(v, pool.nv(), "The velocity matrix");
@@ -137,7 +137,7 @@ inline void abaInParallel(size_t num_threads, ModelPool& pool, const CMat& q, co
   detail::checkCols(v, q.cols, "The velocity matrix");
   detail::checkCols(tau, q.cols, "The torque matrix");
   detail::checkCols(a, q.cols, "The output matrix");
-  detail::vecOp(&rbd_aba, pool, true, q, v, tau, a);
+  detail::vecOp(&rbd_aba, pool, fp32, q, v, tau, a);
 }
 
 // M_i = crba(q.col(i)) for every column, written as nv x nv blocks side by side in M
@@ -162,6 +162,36 @@ inline std::vector<float> toF32(const CMat& m) {
 inline void fromF32(const std::vector<float>& f, const MMat& m) {
   for (int64_t c = 0; c < m.cols; ++c)
     for (int64_t r = 0; r < m.rows; ++r) m.data[r + c * m.ld] = f[(size_t)(r + c * m.rows)];
+}
+// Runs a derivative op (rbd_rnea_derivatives / rbd_aba_derivatives) whose three outputs
+// are nv x (nv B) blocks side by side.  FP64: straight into the caller's matrices when
+// they are contiguous (ld = nv), else through packed copies; FP32: inputs rounded, outputs
+// widened.
+template<class Fn>
+void derivOp(Fn fn, const ModelPool& pool, bool fp32, const CMat& x0, const CMat& x1, const CMat& x2, const MMat& o1,
+             const MMat& o2, const MMat& o3) {
+  const int64_t nv = pool.nv(), nn = nv * nv, B = x0.cols;
+  if (!fp32) {
+    if (o1.ld == nv && o2.ld == nv && o3.ld == nv) {
+      check(fn(pool.ctx(), pool.model(), B, RBD_F64, x0.data, x0.ld, x1.data, x1.ld, x2.data, x2.ld, o1.data, o2.data,
+               o3.data, nn, RBD_HOST_PTRS));
+      return;
+    }
+    std::vector<double> d1((size_t)(nn * B)), d2(d1.size()), d3(d1.size());
+    check(fn(pool.ctx(), pool.model(), B, RBD_F64, x0.data, x0.ld, x1.data, x1.ld, x2.data, x2.ld, d1.data(),
+             d2.data(), d3.data(), nn, RBD_HOST_PTRS));
+    for (auto [d, m] : {std::pair{&d1, &o1}, std::pair{&d2, &o2}, std::pair{&d3, &o3}})
+      for (int64_t c = 0; c < m->cols; ++c)
+        for (int64_t r = 0; r < nv; ++r) m->data[r + c * m->ld] = (*d)[(size_t)(r + c * nv)];
+    return;
+  }
+  std::vector<float> f0 = toF32(x0), f1 = toF32(x1), f2 = toF32(x2);
+  std::vector<float> e1((size_t)(nn * B)), e2(e1.size()), e3(e1.size());
+  check(fn(pool.ctx(), pool.model(), B, RBD_F32, f0.data(), x0.rows, f1.data(), x1.rows, f2.data(), x2.rows,
+           e1.data(), e2.data(), e3.data(), nn, RBD_HOST_PTRS));
+  fromF32(e1, o1);
+  fromF32(e2, o2);
+  fromF32(e3, o3);
 }
 }  // namespace detail
 
@@ -213,10 +243,11 @@ inline void forwardDynamicsInParallel(size_t num_threads, ModelPool& pool, const
                                          RBD_HOST_PTRS));
 }
 
-// computeRNEADerivatives per column: three nv x (nv * B) outputs (blocks side by side).
+// computeRNEADerivatives per column: three nv x (nv * B) outputs (blocks side by side),
+// FP64 by default.
 inline void computeRNEADerivativesInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
                                              const CMat& a, const MMat& dtau_dq, const MMat& dtau_dv,
-                                             const MMat& dtau_da) {
+                                             const MMat& dtau_da, bool fp32 = false) {
   detail::checkThreads(num_threads, pool);
   detail::checkRows(q, pool.nq(), "The configuration matrix");
   detail::checkRows(v, pool.nv(), "The velocity matrix");
@@ -227,20 +258,14 @@ inline void computeRNEADerivativesInParallel(size_t num_threads, ModelPool& pool
     detail::checkRows(*o, pool.nv(), "An output matrix");
     detail::checkCols(*o, q.cols * pool.nv(), "An output matrix");
   }
-  const int64_t nn = (int64_t)pool.nv() * pool.nv();
-  std::vector<float> fq = detail::toF32(q), fv = detail::toF32(v), fa = detail::toF32(a);
-  std::vector<float> o1((size_t)(nn * q.cols)), o2(o1.size()), o3(o1.size());
-  check(rbd_rnea_derivatives(pool.ctx(), pool.model(), q.cols, RBD_F32, fq.data(), q.rows, fv.data(), v.rows,
-                             fa.data(), a.rows, o1.data(), o2.data(), o3.data(), nn, RBD_HOST_PTRS));
-  detail::fromF32(o1, dtau_dq);
-  detail::fromF32(o2, dtau_dv);
-  detail::fromF32(o3, dtau_da);
+  detail::derivOp(&rbd_rnea_derivatives, pool, fp32, q, v, a, dtau_dq, dtau_dv, dtau_da);
 }
 
-// computeABADerivatives per column: ddq_dq, ddq_dv, ddq_dtau (= Minv) as nv x (nv * B).
+// computeABADerivatives per column: ddq_dq, ddq_dv, ddq_dtau (= Minv) as nv x (nv * B),
+// FP64 by default.
 inline void computeABADerivativesInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
                                             const CMat& tau, const MMat& ddq_dq, const MMat& ddq_dv,
-                                            const MMat& ddq_dtau) {
+                                            const MMat& ddq_dtau, bool fp32 = false) {
   detail::checkThreads(num_threads, pool);
   detail::checkRows(q, pool.nq(), "The configuration matrix");
   detail::checkRows(v, pool.nv(), "The velocity matrix");
@@ -251,14 +276,11 @@ inline void computeABADerivativesInParallel(size_t num_threads, ModelPool& pool,
     detail::checkRows(*o, pool.nv(), "An output matrix");
     detail::checkCols(*o, q.cols * pool.nv(), "An output matrix");
   }
-  const int64_t nn = (int64_t)pool.nv() * pool.nv();
-  std::vector<float> fq = detail::toF32(q), fv = detail::toF32(v), ft = detail::toF32(tau);
-  std::vector<float> o1((size_t)(nn * q.cols)), o2(o1.size()), o3(o1.size());
-  check(rbd_aba_derivatives(pool.ctx(), pool.model(), q.cols, RBD_F32, fq.data(), q.rows, fv.data(), v.rows,
-                            ft.data(), tau.rows, o1.data(), o2.data(), o3.data(), nn, nullptr, 0, RBD_HOST_PTRS));
-  detail::fromF32(o1, ddq_dq);
-  detail::fromF32(o2, ddq_dv);
-  detail::fromF32(o3, ddq_dtau);
+  auto fn = [](rbd_ctx* c, const rbd_model* m, int64_t B, int dt, const void* x0, int64_t l0, const void* x1,
+               int64_t l1, const void* x2, int64_t l2, void* o1, void* o2, void* o3, int64_t lo, uint32_t fl) {
+    return rbd_aba_derivatives(c, m, B, dt, x0, l0, x1, l1, x2, l2, o1, o2, o3, lo, nullptr, 0, fl);
+  };
+  detail::derivOp(fn, pool, fp32, q, v, tau, ddq_dq, ddq_dv, ddq_dtau);
 }
 
 }  // namespace pinocchio_amd

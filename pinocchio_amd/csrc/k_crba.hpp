@@ -68,16 +68,15 @@ template<class S> __device__ __forceinline__ void stMat(View<S> g, int64_t b, in
 // G > 1: G waves share the tile; all run the forward pass and the composite-inertia
 // accumulation (cheap), and joint i's entries are computed and stored by wave
 // EmitGroups<TP, G>::grp[i] only, so the store-heavy emission is spread over more waves.
-template<class TP, class S, int G = 1>
-__global__ __launch_bounds__(64 * G) void k_crba(const S* __restrict__ mc_in, int64_t B, CView<S> gq, View<S> gM,
-                                                 int upperOnly) {
+// One 64-configuration tile starting at b0, run by G waves (w: this wave); `sq` holds
+// NQ * kPitch scalars of LDS.
+template<class TP, class S, int G>
+__device__ __forceinline__ void crbaTile(const S* __restrict__ mc_in, int64_t B, int64_t b0, CView<S> gq, View<S> gM,
+                                         int upperOnly, S* __restrict__ sq, int w) {
   constexpr auto& T = TP::T;
   constexpr int NJ = TP::NJ, NQ = T.nq, NV = T.nv;
   constexpr EmitGroups<TP, G> EG{};
-  __shared__ S sq[NQ * kPitch];
-  const int64_t b0 = int64_t(blockIdx.x) * kTile;
   const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
-  const int w = G > 1 ? (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
   if constexpr (G == 1) {
     stageIn<NQ>(sq, gq, b0, nb);
   } else {  // two waves stage half of q each
@@ -171,6 +170,14 @@ __global__ __launch_bounds__(64 * G) void k_crba(const S* __restrict__ mc_in, in
       });
     });
   }
+}
+
+template<class TP, class S, int G = 1>
+__global__ __launch_bounds__(64 * G) void k_crba(const S* __restrict__ mc_in, int64_t B, CView<S> gq, View<S> gM,
+                                                 int upperOnly) {
+  __shared__ S sq[TP::T.nq * kPitch];
+  const int w = G > 1 ? (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
+  crbaTile<TP, S, G>(mc_in, B, int64_t(blockIdx.x) * kTile, gq, gM, upperOnly, sq, w);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -781,7 +781,9 @@ int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
       setDevice(ctx);
       Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
       const float* dq_ = (const float*)st.in(q, vq.extent * 4);
-      float* Mi = (float*)st.out(Minv, vo.extent * 4, false);
+      // host outputs with gaps (padded leading dimension, batch-fastest layout) are
+      // preloaded so the copy back leaves the gaps as the caller had them
+      float* Mi = (float*)st.out(Minv, vo.extent * 4, (flags & RBD_LAYOUT_BATCH) || ldM != (int64_t)nv * nv);
       // M into the output buffer, its FP32 tree factor into the workspace, then the unit
       // right-hand sides of the solve overwrite M with its inverse
       const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
@@ -801,7 +803,7 @@ int rbd_compute_minverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
     setDevice(ctx);
     Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
     const double* dq_ = (const double*)st.in(q, vq.extent * 8);
-    double* Mi = (double*)st.out(Minv, vo.extent * 8, false);
+    double* Mi = (double*)st.out(Minv, vo.extent * 8, (flags & RBD_LAYOUT_BATCH) || ldM != (int64_t)nv * nv);
     const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
     double* wU = (double*)ctx->workspace((nU + nD) * 8);
     double* wD = wU + nU;

@@ -68,6 +68,8 @@ int main(int argc, char** argv) {
   pinocchio_amd::crbaInParallel(threads, pool, Q, MMat{M.data(), nv, nv * B, nv});
   writeRaw(dir + "/out_tau.f64", tau_out);
   writeRaw(dir + "/out_ddq.f64", ddq);
+  pinocchio_amd::abaInParallel(threads, pool, Q, V, T, MMat{ddq.data(), nv, B, nv}, /*fp32=*/true);
+  writeRaw(dir + "/out_ddq32.f64", ddq);
   writeRaw(dir + "/out_M.f64", M);
   if (pool.nv() == nv) {
     std::vector<double> d1((size_t)nv * nv * B), d2(d1.size()), d3(d1.size());
@@ -79,6 +81,20 @@ int main(int argc, char** argv) {
     writeRaw(dir + "/out_ddq_dq.f64", d1);
     writeRaw(dir + "/out_ddq_dv.f64", d2);
     writeRaw(dir + "/out_minv.f64", d3);
+    pinocchio_amd::computeRNEADerivativesInParallel(threads, pool, Q, V, A, o1, o2, o3, /*fp32=*/true);
+    writeRaw(dir + "/out_dtau_dq32.f64", d1);
+    pinocchio_amd::computeABADerivativesInParallel(threads, pool, Q, V, T, o1, o2, o3, /*fp32=*/true);
+    writeRaw(dir + "/out_ddq_dq32.f64", d1);
+    writeRaw(dir + "/out_minv32.f64", d3);
+    // outputs with a leading dimension above nv (staged FP64 path)
+    std::vector<double> w1((size_t)(nv + 3) * nv * B), w2(w1.size()), w3(w1.size());
+    pinocchio_amd::computeABADerivativesInParallel(threads, pool, Q, V, T, MMat{w1.data(), nv, nv * B, nv + 3},
+                                                   MMat{w2.data(), nv, nv * B, nv + 3},
+                                                   MMat{w3.data(), nv, nv * B, nv + 3});
+    std::vector<double> packed((size_t)nv * nv * B);
+    for (long c = 0; c < nv * B; ++c)
+      for (int r = 0; r < nv; ++r) packed[(size_t)(r + c * nv)] = w2[(size_t)(r + c * (nv + 3))];
+    writeRaw(dir + "/out_ddq_dv_ld.f64", packed);
     pinocchio_amd::computeMinverseInParallel(threads, pool, Q, o1);
     writeRaw(dir + "/out_minv64.f64", d1);
     // constrained forwardDynamics: J is nc x (nv B), the configurations' blocks side by side

@@ -279,3 +279,48 @@ def test_largest_batches_use_64bit_offsets(gpu_ctx):
     dd = eng.aba(Q, V, T)[:, torch.tensor(cols, device="cuda")].double().cpu().numpy()
     from helpers import rel_err_cols
     assert rel_err_cols(dd, oracle.aba(desc, q[:, cols], v[:, cols], tau[:, cols])).max() < 1e-4
+
+
+def test_compute_minverse_host_pointers_padded(gpu_ctx):
+    """rbd_compute_minverse through the raw C ABI on host arrays (RBD_HOST_PTRS), Eigen
+    column-major layout with a leading dimension larger than nv * nv: the gaps stay as the
+    caller left them (FP32 and FP64)."""
+    import ctypes
+    desc, model = gpu_model(gpu_ctx, "quadruped18")
+    nv = desc.nv
+    B = 97
+    q, v, _, tau = inputs(desc, B, k=71)
+    L = pa.lib()
+    for code, dt, bar in ((pa.RBD_F32, np.float32, 1e-4), (pa.RBD_F64, np.float64, 1e-10)):
+        qc = np.ascontiguousarray(q.T.astype(dt))
+        ld = nv * nv + 7
+        out = np.full((B, ld), 7.0, dtype=dt)
+        rc = L.rbd_compute_minverse(gpu_ctx.handle, model.handle, ctypes.c_int64(B), code,
+                                    qc.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(desc.nq),
+                                    out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(ld),
+                                    ctypes.c_uint32(pa.RBD_HOST_PTRS))
+        assert rc == 0, L.rbd_last_error()
+        qq = qc.T.astype(np.float64)
+        _, _, rmi, _ = oracle.aba_derivatives(desc, qq, v, tau)
+        Mi = out[:, :nv * nv].astype(np.float64).reshape(B, nv, nv).transpose(0, 2, 1)
+        assert rel_err_mats(Mi, rmi).max() < bar
+        assert (out[:, nv * nv:] == 7.0).all()
+
+
+def test_aba_derivatives_unstaged_factor_path(gpu_ctx):
+    """Above 32768 configurations the FP32 tree factor of the ABA-derivative pipeline reads
+    M lazily instead of staging it in LDS (k_chol instead of k_chol_staged): 40000
+    quadruped configurations, every 7th checked against the oracle."""
+    desc, model = gpu_model(gpu_ctx, "quadruped18")
+    eng = pa.Engine(model, "batch")
+    B = 40000
+    q, v, _, tau = f32_round(*inputs(desc, B, k=72))
+    Xq, Xv, Mi, ddq = eng.aba_derivatives(to_dev(q, "batch"), to_dev(v, "batch"), to_dev(tau, "batch"))
+    idx = np.arange(0, B, 7)
+    rq, rv, rmi, _ = oracle.aba_derivatives(desc, q[:, idx], v[:, idx], tau[:, idx])
+    for n, o, r in (("ddq_dq", Xq, rq), ("ddq_dv", Xv, rv), ("Minv", Mi, rmi)):
+        x = pa.Engine.mats_to_numpy(o[:, idx].contiguous(), "batch", desc.nv)
+        e = rel_err_mats(x, r)
+        print(f"quadruped18 B={B}: {n} max {e.max():.3e}")
+        assert e.max() < 1e-4, n
+

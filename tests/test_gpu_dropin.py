@@ -3,10 +3,11 @@ tests/cpp/dropin_check builds a ModelPool from a flattened model, runs rneaInPar
 abaInParallel, crbaInParallel, computeRNEADerivativesInParallel,
 computeABADerivativesInParallel, computeMinverseInParallel and forwardDynamicsInParallel
 (constrained) on column-major host matrices, and checks the reference's argument errors.
-Outputs against the FP64 oracle: FP64 RNEA and CRBA 1e-12, FP32 ABA 1e-4, FP32 RNEA
-derivatives 1e-5, FP32 ABA derivatives 1e-4, FP64 Minv 1e-10, FP64 constrained ddq and
-lambda 1e-8 with a random J (per configuration, inf-norm relative; oracle on the
-FP32-rounded inputs for the FP32 paths).
+Outputs against the FP64 oracle (the calls default to FP64, like the reference): RNEA,
+ABA and CRBA 1e-12, RNEA and ABA derivatives and Minv 1e-10 (also with outputs whose
+leading dimension exceeds nv), constrained ddq and lambda 1e-8 with a random J; with the
+FP32 option: ABA 1e-4, RNEA derivatives 1e-5, ABA derivatives 1e-4 (per configuration,
+inf-norm relative; oracle on the FP32-rounded inputs for the FP32 paths).
 """
 import os
 import subprocess
@@ -60,17 +61,24 @@ def test_cpp_dropin(tmp_path, name):
     tau_g = rd("tau", B)
     assert rel_err_cols(tau_g, oracle.rnea(desc, q, v, a)).max() < 1e-12
     qf, vf, af, tf = f32_round(q, v, a, tau)
-    assert rel_err_cols(rd("ddq", B), oracle.aba(desc, qf, vf, tf)).max() < 1e-4
+    assert rel_err_cols(rd("ddq", B), oracle.aba(desc, q, v, tau)).max() < 1e-12
+    assert rel_err_cols(rd("ddq32", B), oracle.aba(desc, qf, vf, tf)).max() < 1e-4
     Mg = blocks(np.fromfile(os.path.join(d, "out_M.f64")), nv, B)
     assert rel_err_mats(Mg, oracle.crba(desc, q)).max() < 1e-12
-    rdq, rdv, _, _ = oracle.rnea_derivatives(desc, qf, vf, af)
+
+    def mats(n):
+        return blocks(np.fromfile(os.path.join(d, f"out_{n}.f64")), nv, B)
+
+    rdq, rdv, _, _ = oracle.rnea_derivatives(desc, q, v, a)
     for n, ref in (("dtau_dq", rdq), ("dtau_dv", rdv)):
-        assert rel_err_mats(blocks(np.fromfile(os.path.join(d, f"out_{n}.f64")), nv, B), ref).max() < 1e-5
-    rq, rv, rmi, _ = oracle.aba_derivatives(desc, qf, vf, tf)
-    for n, ref in (("ddq_dq", rq), ("ddq_dv", rv), ("minv", rmi)):
-        assert rel_err_mats(blocks(np.fromfile(os.path.join(d, f"out_{n}.f64")), nv, B), ref).max() < 1e-4
-    _, _, rmi64, _ = oracle.aba_derivatives(desc, q, v, tau)
-    assert rel_err_mats(blocks(np.fromfile(os.path.join(d, "out_minv64.f64")), nv, B), rmi64).max() < 1e-10
+        assert rel_err_mats(mats(n), ref).max() < 1e-10, n
+    rq, rv, rmi, _ = oracle.aba_derivatives(desc, q, v, tau)
+    for n, ref in (("ddq_dq", rq), ("ddq_dv", rv), ("minv", rmi), ("minv64", rmi), ("ddq_dv_ld", rv)):
+        assert rel_err_mats(mats(n), ref).max() < 1e-10, n
+    assert rel_err_mats(mats("dtau_dq32"), oracle.rnea_derivatives(desc, qf, vf, af)[0]).max() < 1e-5
+    rq32, _, rmi32, _ = oracle.aba_derivatives(desc, qf, vf, tf)
+    assert rel_err_mats(mats("ddq_dq32"), rq32).max() < 1e-4
+    assert rel_err_mats(mats("minv32"), rmi32).max() < 1e-4
     ca, cl = oracle.constrained_forward_dynamics(desc, q, v, tau, J, gamma)
     cddq = np.fromfile(os.path.join(d, "out_cddq.f64")).reshape(B, nv).T
     clam = np.fromfile(os.path.join(d, "out_lambda.f64")).reshape(B, nc).T

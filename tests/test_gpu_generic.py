@@ -6,15 +6,20 @@ The reference accepts any tree its caller builds with ModelTpl::addJoint
 (algorithm/parallel/rnea.hpp:39-83).  Here: random trees that are not in the prebuilt
 list -- a compact free-flyer tree, a non-compact one (insertion order is not DFS order),
 a fixed-base tree with several revolute roots, a 40-link chain (its per-lane state does
-not fit LDS, so the device-memory variants run) -- and the 36-DoF humanoid forced onto the
-run-time-topology kernels, compared with its prebuilt kernels.
+not fit LDS, so the device-memory variants run), a 20-DoF floating torso read from MJCF
+(bodies with several hinges, flattened into chains through massless links) -- and the
+36-DoF humanoid forced onto the run-time-topology kernels, compared with its prebuilt
+kernels.
 
 Bars (per configuration, inf-norm relative, oracle in FP64 on the rounded inputs):
   FP32: RNEA, CRBA, RNEA derivatives 1e-5; ABA 1e-4; ABA derivatives and Minv 1e-4, or,
-        where M^-1 is worse conditioned (the 40-link chain, cond(M) ~ 3e4), four times the
-        error of the reference algorithm run in FP32 (the oracle in float) or four times
+        where M^-1 is worse conditioned (the 40-link chain and the MJCF torso, whose
+        light hands give cond(M) ~ 3e4), four times the error of the reference algorithm
+        run in FP32 (the oracle in float) or, for the ABA derivatives and Minv, four times
         the error of the exact inverse of the FP32 M the pipeline factors, if larger
-  FP64: RNEA, ABA, CRBA 1e-12; RNEA / ABA derivatives, Minv 1e-10
+  FP64: RNEA, ABA, CRBA 1e-12 (ABA: or the FP32 floor above scaled by 2^-29, the ratio
+        of the two unit roundoffs: 3.4e-12 on the MJCF torso); RNEA / ABA derivatives,
+        Minv 1e-10
 """
 import numpy as np
 import pytest
@@ -22,6 +27,7 @@ import torch
 
 import oracle
 import pinocchio_amd as pa
+from pinocchio_amd import mjcf
 from pinocchio_amd import models as M
 from helpers import f32_round, inputs, rel_err_cols, rel_err_mats, two_link_arm, two_link_closed_form
 
@@ -32,8 +38,30 @@ TREES = {
     "tree_noncompact30": lambda: M.random_tree(30, M.MODEL_SEED_BASE + 102, free_flyer=True, compact=False),
     "tree_fixed15": lambda: M.random_tree(15, M.MODEL_SEED_BASE + 103, free_flyer=False, compact=True),
     "chain40": lambda: M.serial_chain(40, M.MODEL_SEED_BASE + 104),
+    "mjcf_torso20": lambda: mjcf.parse(TORSO_MJCF, root="freeflyer"),
 }
-COMPACT_TREES = ["tree_ff24", "tree_fixed15", "chain40"]
+COMPACT_TREES = ["tree_ff24", "tree_fixed15", "chain40", "mjcf_torso20"]
+
+# a floating torso with two arms and a head; shoulders, wrists and neck hold several
+# hinges in one body (the reference's composite joints), flattened by the MJCF reader
+_ARM = """<body name="{s}shoulder" pos="0 {y} 0.4" euler="10 0 {yaw}">
+  <joint type="hinge" axis="1 0 0" pos="0 0 0.02" armature="0.05"/><joint type="hinge" axis="0 1 0" pos="0.01 0 0"/>
+  <joint type="hinge" axis="0 0 1"/>
+  <inertial mass="2.1" pos="0 0 -0.15" diaginertia="0.02 0.021 0.004"/>
+  <body name="{s}elbow" pos="0 0 -0.3"><joint type="hinge" axis="0 1 0" armature="0.02"/>
+    <inertial mass="1.3" pos="0.01 0 -0.12" quat="0.98 0.1 0.05 0" diaginertia="0.01 0.012 0.002"/>
+    <body name="{s}wrist" pos="0 0 -0.25" quat="0.7071 0 0 0.7071">
+      <joint type="hinge" axis="0 0 1" pos="0 0 -0.01"/><joint type="hinge" axis="1 0 0" pos="0 0.005 0"/>
+      <inertial mass="0.5" pos="0 0 -0.05" fullinertia="0.001 0.0012 0.0006 1e-5 -2e-5 3e-5"/>
+      <body name="{s}palm" pos="0 0 -0.08"><inertial mass="0.2" pos="0 0 -0.02" diaginertia="2e-4 2e-4 1e-4"/></body>
+    </body></body></body>"""
+TORSO_MJCF = ('<mujoco model="torso20"><compiler angle="degree"/><worldbody><body name="torso" pos="0 0 1">'
+              '<inertial mass="12" pos="0 0 0.2" diaginertia="0.4 0.35 0.2"/>'
+              + _ARM.format(s="l", y=0.2, yaw=15) + _ARM.format(s="r", y=-0.2, yaw=-15)
+              + '<body name="head" pos="0 0 0.55"><joint type="hinge" axis="0 0 1"/>'
+              '<joint type="hinge" axis="0 1 0" pos="0 0 0.03"/>'
+              '<inertial mass="3" pos="0 0 0.1" diaginertia="0.02 0.02 0.015"/></body>'
+              '</body></worldbody></mujoco>')
 
 _models = {}
 
@@ -83,12 +111,19 @@ def test_rnea_aba(gpu_ctx, name, layout):
     eng = pa.Engine(model, layout)
     B = 777
     q, v, a, tau = inputs(desc, B, k=40)
+    qf, vf, af, tf = f32_round(q, v, a, tau)
+    r_f = oracle.aba(desc, qf, vf, tf)
+    amp = float(rel_err_cols(oracle.aba(desc, qf, vf, tf, scalar="f32"), r_f).max())  # FP32 error of the algorithm
     for dtype, bar_r, bar_a in ((torch.float32, 1e-5, 1e-4), (torch.float64, 1e-12, 1e-12)):
-        qq, vv, aa, tt = f32_round(q, v, a, tau) if dtype == torch.float32 else (q, v, a, tau)
+        qq, vv, aa, tt = (qf, vf, af, tf) if dtype == torch.float32 else (q, v, a, tau)
         Q, V, A, T = (to_dev(x, layout, dtype) for x in (qq, vv, aa, tt))
         e_r = rel_err_cols(to_host(eng.rnea(Q, V, A), layout), oracle.rnea(desc, qq, vv, aa))
-        e_a = rel_err_cols(to_host(eng.aba(Q, V, T), layout), oracle.aba(desc, qq, vv, tt))
-        print(f"{name} {layout} {dtype}: rnea {e_r.max():.3e} aba {e_a.max():.3e}")
+        r_a = r_f if dtype == torch.float32 else oracle.aba(desc, qq, vv, tt)
+        e_a = rel_err_cols(to_host(eng.aba(Q, V, T), layout), r_a)
+        # conditioning floor: four times the algorithm's own FP32 error, scaled to the
+        # unit roundoff of the precision under test
+        bar_a = max(bar_a, 4.0 * amp * (1.0 if dtype == torch.float32 else 2.0 ** -29))
+        print(f"{name} {layout} {dtype}: rnea {e_r.max():.3e} aba {e_a.max():.3e} (bar {bar_a:.1e})")
         assert e_r.max() < bar_r
         assert e_a.max() < bar_a
 
@@ -166,7 +201,8 @@ def test_derivatives(gpu_ctx, name, layout):
             bar = bars[1] if o32 is None else max(fp32_bar(bars[1], rel_err_mats(o32[k], r)), 4.0 * floor)
             print(f"{name} {layout} {dtype}: {n} {e.max():.3e} (bar {bar:.1e})")
             assert e.max() < bar, n
-        assert rel_err_cols(to_host(ddq, layout), rddq).max() < bars[1]
+        bar = bars[1] if o32 is None else fp32_bar(bars[1], rel_err_cols(o32[3], rddq))
+        assert rel_err_cols(to_host(ddq, layout), rddq).max() < bar
 
 
 @pytest.mark.parametrize("name", list(TREES) + ["humanoid36_rt"])

@@ -118,3 +118,30 @@ def test_mix_argument_errors(gpu_ctx, mix_models):
         pa.MixedBatch(gpu_ctx, [(model, q, v, v, v[:, :5])])
     with pytest.raises(ValueError):
         pa.MixedBatch(gpu_ctx, [(model, q.double(), v, v, v)])
+
+
+def test_benchmark_mix_at_full_size(gpu_ctx, mix_models):
+    """The benchmark's exact mix -- arm7 x 40000, quadruped18 x 40000, humanoid36 x 15000,
+    random60 x 5000, seeded as in the benchmark -- through the autotuned, graph-replayed
+    plan the tuner picks at that size; per group the first and the last 2048 instances
+    (so each group's last, partial chunk) against the oracle."""
+    spec = [("arm7", 40000), ("quadruped18", 40000), ("humanoid36", 15000), ("random60", 5000)]
+    host, groups = [], []
+    for j, (name, n) in enumerate(spec):
+        desc, model = mix_models[name]
+        q, v, a, tau = f32_round(*M.sample_inputs(desc, n, M.INPUT_SEED_BASE + 1 + j))
+        host.append((desc, q, v, a, tau))
+        groups.append((model, dev(q), dev(v), dev(a), dev(tau)))
+    mb = pa.MixedBatch(gpu_ctx, groups)
+    mb.run()  # tunes and captures
+    outs = mb.run()  # graph replay of the chosen plan
+    torch.cuda.synchronize()
+    print("benchmark mix:", mb.tuning().get("chosen"))
+    for (desc, q, v, a, tau), (tau_g, ddq_g) in zip(host, outs):
+        n = q.shape[1]
+        idx = np.r_[0:2048, n - 2048:n]
+        sel = lambda x: np.ascontiguousarray(x[:, idx])
+        e_r = rel_err_cols(tau_g.double().cpu().numpy()[:, idx], oracle.rnea(desc, sel(q), sel(v), sel(a))).max()
+        e_a = rel_err_cols(ddq_g.double().cpu().numpy()[:, idx], oracle.aba(desc, sel(q), sel(v), sel(tau))).max()
+        print(f"  {desc.name} x {n}: rnea {e_r:.3e} aba {e_a:.3e}")
+        assert e_r < 1e-5 and e_a < 1e-4, desc.name

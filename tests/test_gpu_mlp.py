@@ -160,8 +160,10 @@ def res_of(sur, desc, Q, V, T):
 
 
 def test_training_gain(quad):
-    """A short training run (65536 GPU-ABA samples, 4 epochs) against the untrained network:
-    the held-out loss must drop severalfold and the median RNEA residual with it."""
+    """A short training run (65536 GPU-ABA samples, 8 epochs) against the untrained network:
+    the held-out loss must drop severalfold and the median RNEA residual at least by half
+    (measured on MI355X: loss 0.997 -> 0.082, median residual 4.12 -> 1.63, i.e. 0.40;
+    4 epochs stop at 0.51)."""
     desc, model = quad
     data = S.make_dataset(model, 65536, M.INPUT_SEED_BASE + 500)
     sur = S.Surrogate(model, seed=5)
@@ -171,9 +173,52 @@ def test_training_gain(quad):
     D = pa.Engine(model, "batch").aba(Q, V, T)
     l0, _ = sur.gradients(Q, V, T, D)
     r0 = np.median(res_of(sur, desc, Q, V, T))
-    losses, secs = S.train(sur, data, epochs=4, batch=1024, lr=2e-3)
+    losses, secs = S.train(sur, data, epochs=8, batch=1024, lr=2e-3)
     l1, _ = sur.gradients(Q, V, T, D)
     r1 = np.median(res_of(sur, desc, Q, V, T))
     print(f"held-out loss {l0:.4f} -> {l1:.4f}, median residual {r0:.4f} -> {r1:.4f}, epochs {losses}, {secs:.2f} s")
     assert l1 < 0.3 * l0, (l0, l1)
-    assert r1 < 0.7 * r0, (r0, r1)
+    assert r1 < 0.5 * r0, (r0, r1)
+
+
+def test_gated_path_against_plain_aba(quad):
+    """What the surrogate path delivers against plain GPU ABA on the same 65536
+    configurations at the 1e-3 RNEA-residual gate: the accepted fraction (consistent with
+    the per-configuration residuals) and the throughput ratio, measured with HIP events.
+    At this gate the trained network's predictions are rejected (median residual ~0.6
+    after the full training run), so every configuration pays for the network, the
+    residual RNEA and ABA: the gated path is slower than ABA alone.  It cannot reach 3x by
+    construction: the network costs ~1.14 MFLOP per configuration against ~15 kFLOP for
+    the quadruped's ABA."""
+    desc, model = quad
+    data = S.make_dataset(model, 65536, M.INPUT_SEED_BASE + 510)
+    sur = S.Surrogate(model, seed=6)
+    sur.fit_normalization(*data)
+    S.train(sur, data, epochs=4, batch=1024, lr=2e-3)
+    n = 65536
+    Q, V, T = S.sample_inputs_device(model, n, M.INPUT_SEED_BASE + 520, tau_sigma=20.0)
+    eng = pa.Engine(model, "batch")
+    out = torch.empty((desc.nv, n), device="cuda")
+
+    def timed(f, reps=5):
+        f()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            f()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+
+    t_aba = timed(lambda: eng.aba(Q, V, T, out=out))
+    t_gate = timed(lambda: sur.forward_dynamics(Q, V, T, tol=1e-3, out=out))
+    _, nfb, acc, res = sur.forward_dynamics(Q, V, T, tol=1e-3, out=out, want_mask=True)
+    r = res.cpu().numpy()
+    accepted = 1.0 - nfb / n
+    print(f"tol 1e-3: accepted {accepted:.4f}, median residual {np.median(r):.3f}, "
+          f"ABA {t_aba:.3f} ms, gated {t_gate:.3f} ms, speedup {t_aba / t_gate:.3f}")
+    assert accepted == pytest.approx(float((r <= 1e-3).mean()), abs=1.0 / n)
+    assert int(acc.sum().item()) == n - nfb
+    assert t_aba / t_gate < 1.0
+
