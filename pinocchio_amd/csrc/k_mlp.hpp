@@ -139,6 +139,139 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
   }
 }
 
+// ---- forward product: both operands k-contiguous ---------------------------------------
+// C(m, n) = sum_k A[m * lda + k] B[n * ldb + k] (activations x weights of a linear layer).
+// A 256 x 128 tile per workgroup; four waves in a 2 x 2 grid, each 128 x 64 as 4 x 2
+// blocks of v_mfma_f32_32x32x16_bf16 (128 accumulators per lane), so every operand value a
+// wave reads from LDS feeds 64 or 128 products.  Stages of 64 k: while the matrix cores
+// work on stage s from LDS, the loads of stage s + 1 are in flight into registers (32
+// products per wave to cover their latency), parked into LDS after the stage's barrier.
+// Launch: (M / 256) * (N / 128) workgroups on a one-dimensional grid; K a multiple of 64.
+constexpr int kNtBM = 256, kNtBN = 128, kNtBK = 64, kNtLdk = kNtBK + 8;
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template<int EPI>
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_nt(GemmArgs g) {
+  constexpr int BM = kNtBM, BN = kNtBN, WM = BM / 2, WN = BN / 2, TM = WM / 32;
+  __shared__ __attribute__((aligned(16))) bf16 sA[BM * kNtLdk];
+  __shared__ __attribute__((aligned(16))) bf16 sB[BN * kNtLdk];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  // One-dimensional grid over the tiles, column tile fastest.  Consecutive workgroups go
+  // to different XCDs (round robin over 8), each with its own L2: remap so that every XCD
+  // owns a contiguous run of tiles, where the column tiles of one row tile run side by
+  // side and read their A rows from HBM once.
+  const int ncol = g.N / BN, ntiles = gridDim.x;
+  int t = blockIdx.x;
+  if (ntiles % 8 == 0) t = (t % 8) * (ntiles / 8) + t / 8;
+  const int m0 = (t / ncol) * BM, n0 = (t % ncol) * BN;
+  const int nst = g.K / kNtBK;
+  // thread tid stages the 16-byte chunks of rows tid / 8 + 32 i, k offset (tid % 8) * 8
+  const bf16* gA = g.A + (int64_t)(m0 + (tid >> 3)) * g.lda + (tid & 7) * 8;
+  const bf16* gB = g.B + (int64_t)(n0 + (tid >> 3)) * g.ldb + (tid & 7) * 8;
+  const int64_t stA = 32 * g.lda, stB = 32 * g.ldb;
+  const int so = (tid >> 3) * kNtLdk + (tid & 7) * 8, ss = 32 * kNtLdk;
+  uint4 ra0, ra1, ra2, ra3, ra4, ra5, ra6, ra7, rb0, rb1, rb2, rb3;
+#define RBD_NT_FETCH(k0)                                                                                      \
+  ra0 = *reinterpret_cast<const uint4*>(gA + (k0));                                                           \
+  ra1 = *reinterpret_cast<const uint4*>(gA + stA + (k0));                                                     \
+  ra2 = *reinterpret_cast<const uint4*>(gA + 2 * stA + (k0));                                                 \
+  ra3 = *reinterpret_cast<const uint4*>(gA + 3 * stA + (k0));                                                 \
+  ra4 = *reinterpret_cast<const uint4*>(gA + 4 * stA + (k0));                                                 \
+  ra5 = *reinterpret_cast<const uint4*>(gA + 5 * stA + (k0));                                                 \
+  ra6 = *reinterpret_cast<const uint4*>(gA + 6 * stA + (k0));                                                 \
+  ra7 = *reinterpret_cast<const uint4*>(gA + 7 * stA + (k0));                                                 \
+  rb0 = *reinterpret_cast<const uint4*>(gB + (k0));                                                           \
+  rb1 = *reinterpret_cast<const uint4*>(gB + stB + (k0));                                                     \
+  rb2 = *reinterpret_cast<const uint4*>(gB + 2 * stB + (k0));                                                 \
+  rb3 = *reinterpret_cast<const uint4*>(gB + 3 * stB + (k0));
+#define RBD_NT_PARK                                                                                           \
+  *reinterpret_cast<uint4*>(&sA[so]) = ra0;                                                                   \
+  *reinterpret_cast<uint4*>(&sA[so + ss]) = ra1;                                                              \
+  *reinterpret_cast<uint4*>(&sA[so + 2 * ss]) = ra2;                                                          \
+  *reinterpret_cast<uint4*>(&sA[so + 3 * ss]) = ra3;                                                          \
+  *reinterpret_cast<uint4*>(&sA[so + 4 * ss]) = ra4;                                                          \
+  *reinterpret_cast<uint4*>(&sA[so + 5 * ss]) = ra5;                                                          \
+  *reinterpret_cast<uint4*>(&sA[so + 6 * ss]) = ra6;                                                          \
+  *reinterpret_cast<uint4*>(&sA[so + 7 * ss]) = ra7;                                                          \
+  *reinterpret_cast<uint4*>(&sB[so]) = rb0;                                                                   \
+  *reinterpret_cast<uint4*>(&sB[so + ss]) = rb1;                                                              \
+  *reinterpret_cast<uint4*>(&sB[so + 2 * ss]) = rb2;                                                          \
+  *reinterpret_cast<uint4*>(&sB[so + 3 * ss]) = rb3;
+  f32x16 acc[TM][2];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[i][0][e] = acc[i][1][e] = 0.f;
+
+  // lane: row (or column) lane % 32 of a 32-block, k half (lane / 32) * 8 of a 16-k step
+  const int lr = lane & 31, lk = (lane >> 5) * 8;
+  RBD_NT_FETCH(0)
+  for (int s = 0; s < nst; ++s) {
+    RBD_NT_PARK
+    __syncthreads();
+    if (s + 1 < nst) {
+      RBD_NT_FETCH((s + 1) * kNtBK)
+    }
+#pragma unroll
+    for (int kk = 0; kk < kNtBK; kk += 16) {
+      const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(&sB[(wn * WN + lr) * kNtLdk + kk + lk]);
+      const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(&sB[(wn * WN + 32 + lr) * kNtLdk + kk + lk]);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[(wm * WM + i * 32 + lr) * kNtLdk + kk + lk]);
+        acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[i][0], 0, 0, 0);
+        acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[i][1], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#undef RBD_NT_FETCH
+#undef RBD_NT_PARK
+
+  // lane holds C(row = 8 (e / 4) + 4 (lane / 32) + e % 4, col = lane % 32) of each 32 x 32 block
+  const float bias0 = g.bias[n0 + wn * WN + lr], bias1 = g.bias[n0 + wn * WN + 32 + lr];
+  if constexpr (EPI == EPI_SILU) {
+    // bf16 activations leave through LDS: each wave turns one 32 x 64 block row at a time
+    // into 16-byte row segments (the LDS tiles are free after the last barrier)
+    constexpr int P = WN + 8;  // staging pitch in bf16
+    bf16* stg = sA + wave * 32 * P;
+    const bool keepZ = g.Zout != nullptr;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int r = (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+          const float z = acc[i][j][e] + (j ? bias1 : bias0);
+          if (keepZ) g.Zout[(int64_t)(m0 + wm * WM + i * 32 + r) * g.ldz + n0 + wn * WN + j * 32 + lr] = z;
+          stg[r * P + j * 32 + lr] = (bf16)(z * sigmoidf(z));
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {  // 32 rows x 8 segments of 8 bf16
+        const int idx = c * 64 + lane, r = idx >> 3, sg = (idx & 7) * 8;
+        *reinterpret_cast<uint4*>(g.Cb + (int64_t)(m0 + wm * WM + i * 32 + r) * g.ldc + n0 + wn * WN + sg) =
+            *reinterpret_cast<const uint4*>(&stg[r * P + sg]);
+      }
+      __syncthreads();
+    }
+  } else {
+    static_assert(EPI == EPI_LINEAR, "forward epilogues only");
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int row = m0 + wm * WM + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+          g.Cf[(int64_t)row * g.ldc + n0 + wn * WN + j * 32 + lr] = acc[i][j][e] + (j ? bias1 : bias0);
+        }
+  }
+}
+
 // ---- features ------------------------------------------------------------------------
 // Input vector of one configuration (feature order):
 //   sin q_r, cos q_r for every revolute joint r       2 * nrev
@@ -171,8 +304,8 @@ __global__ __launch_bounds__(64) void k_features(FeatMap fm, int64_t B, const fl
                                                  int64_t ldv, const float* tau, int64_t ldt, const float* mean,
                                                  const float* invstd, bf16* X, float* Xraw) {
   // row pitch ldx + 2: an odd number of 4-byte words, so the per-thread row writes of a
-  // wave fall in distinct LDS banks
-  __shared__ __attribute__((aligned(16))) bf16 tile[64 * (kMaxLdx + 2)];
+  // wave fall in distinct LDS banks (dynamic LDS of 64 (ldx + 2) bf16)
+  extern __shared__ __attribute__((aligned(16))) bf16 tile[];
   const int t = threadIdx.x;
   const int64_t b0 = (int64_t)blockIdx.x * 64, b = b0 + t;
   const bool live = b < B;

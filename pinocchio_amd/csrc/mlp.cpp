@@ -21,7 +21,8 @@ using namespace rbd::mlp;
 
 namespace {
 
-constexpr int kBM = 128;     // row tile of every product (batch padding unit)
+constexpr int kBM = 128;     // row tile of the products other than the forward layers
+constexpr int kPadRows = 256; // batch padding unit: the forward layers' row tile (kNtBM)
 constexpr int kMaxSplits = 32;
 
 int64_t roundUp(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
@@ -106,6 +107,13 @@ template<int BM, int BN, bool AK, bool BK, int EPI> void gemm(hipStream_t st, co
   hipLaunchKernelGGL((k_gemm<BM, BN, AK, BK, EPI>), dim3(g.M / BM, g.N / BN, splits), dim3(kGemmThreads), 0, st, g);
 }
 
+// forward product over 256 x 128 tiles (hidden layers)
+template<int EPI> void gemmNt(hipStream_t st, const GemmArgs& g) {
+  if (g.M % kNtBM || g.N % kNtBN || g.K % kNtBK || g.kChunk != g.K)
+    throw Error(RBD_EINVAL, "internal: product shape does not fit its tiles");
+  hipLaunchKernelGGL((k_gemm_nt<EPI>), dim3((g.M / kNtBM) * (g.N / kNtBN)), dim3(kGemmThreads), 0, st, g);
+}
+
 GemmArgs gargs(const bf16* A, int64_t lda, const bf16* B, int64_t ldb, int M, int N, int K) {
   GemmArgs g{};
   g.A = A;
@@ -142,8 +150,8 @@ void checkVec(const void* p, int64_t ld, int64_t B, const char* what) {
 void features(rbd_mlp* m, hipStream_t st, int64_t B, int64_t Bp, const float* q, int64_t ldq, const float* v,
               int64_t ldv, const float* tau, int64_t ldt, float* Xraw) {
   if (B > 0)
-    hipLaunchKernelGGL(k_features, dim3((unsigned)((B + 63) / 64)), dim3(64), 0, st, m->fm, B, q, ldq, v, ldv, tau,
-                       ldt, m->xmean(), m->xinv(), m->X.p, Xraw);
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((B + 63) / 64)), dim3(64), 64 * (m->fm.ldx + 2) * sizeof(bf16), st,
+                       m->fm, B, q, ldq, v, ldv, tau, ldt, m->xmean(), m->xinv(), m->X.p, Xraw);
   if (Bp > B) hipLaunchKernelGGL(k_zero_rows, dim3(blocksFor((Bp - B) * m->kin)), dim3(256), 0, st, m->X.p,
                                  (int64_t)m->kin, B, Bp);
 }
@@ -161,7 +169,7 @@ void forward(rbd_mlp* m, hipStream_t st, int64_t Bp, bool keepZ) {
       g.ldc = m->H;
       g.Zout = keepZ ? m->ZL(l, Bp) : nullptr;
       g.ldz = m->H;
-      gemm<kBM, 128, true, true, EPI_SILU>(st, g);
+      gemmNt<EPI_SILU>(st, g);
       in = g.Cb;
       ldin = m->H;
     } else {
@@ -286,12 +294,12 @@ int rbd_mlp_create(rbd_ctx* ctx, const rbd_model* model, int32_t hidden, int32_t
       }
     }
     fm.nfeat = 2 * fm.nrev + (fm.hasFF ? 13 : 0) + 2 * model->nv;
-    RBD_CHECK_ARG(fm.nfeat <= kMaxFeat && (int)roundUp(fm.nfeat, kStage) <= kMaxLdx, "too many input features");
+    RBD_CHECK_ARG(fm.nfeat <= kMaxFeat && (int)roundUp(fm.nfeat, kNtBK) <= kMaxLdx, "too many input features");
     m->nin = fm.nfeat;
     m->nout = model->nv;
     m->H = hidden;
     m->L = nhidden;
-    m->kin = (int)roundUp(m->nin, kStage);  // padded widths: whole LDS stages
+    m->kin = (int)roundUp(m->nin, kNtBK);  // padded widths: whole LDS stages of the products
     m->kout = (int)roundUp(m->nout, kStage);
     RBD_CHECK_ARG(m->nout <= kMaxFeat, "too many outputs");
     fm.ldx = m->kin;
@@ -428,7 +436,7 @@ int rbd_mlp_features(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q, int64_
     RBD_CHECK_ARG(B == 0 || Xraw, "null output");
     if (B == 0) return;
     setDev(ctx);
-    const int64_t Bp = roundUp(B, kBM);
+    const int64_t Bp = roundUp(B, kPadRows);
     ensure(m, Bp, false);
     features(m, ctx->stream, B, Bp, q, ldq, v, ldv, tau, ldt, Xraw);
     HIP_CHECK(hipGetLastError());
@@ -447,7 +455,7 @@ int rbd_mlp_predict(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q, int64_t
     checkVec(ddq, ldd, B, "ddq");
     if (B == 0) return;
     setDev(ctx);
-    const int64_t Bp = roundUp(B, kBM);
+    const int64_t Bp = roundUp(B, kPadRows);
     ensure(m, Bp, false);
     features(m, ctx->stream, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
     forward(m, ctx->stream, Bp, false);
@@ -472,7 +480,7 @@ int rbd_mlp_train_step(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q, int6
     RBD_CHECK_ARG(lr > 0.f, "learning rate must be positive");
     setDev(ctx);
     hipStream_t st = ctx->stream;
-    const int64_t Bp = roundUp(B, kBM);
+    const int64_t Bp = roundUp(B, kPadRows);
     ensure(m, Bp, true);
     features(m, st, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
     forward(m, st, Bp, true);
@@ -503,7 +511,7 @@ int rbd_mlp_gradients(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q, int64
     RBD_CHECK_ARG(grad && n == m->nparams, "gradient buffer of nparams floats expected");
     setDev(ctx);
     hipStream_t st = ctx->stream;
-    const int64_t Bp = roundUp(B, kBM);
+    const int64_t Bp = roundUp(B, kPadRows);
     ensure(m, Bp, true);
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, st, m->lossAcc.p, (int64_t)1, 0.f);
     features(m, st, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
@@ -547,7 +555,7 @@ int rbd_mlp_forward_dynamics(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q
     hipStream_t st = ctx->stream;
     const rbd_model* md = m->model;
     const int nq = md->nq, nv = md->nv;
-    const int64_t Bp = roundUp(B, kBM);
+    const int64_t Bp = roundUp(B, kPadRows);
     ensure(m, Bp, false);
     features(m, st, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
     forward(m, st, Bp, false);
