@@ -418,6 +418,28 @@ def main():
         "batch": B, "constraints": ncon, "configs_per_s": B / t, "ms": t * 1e3,
         "note": "CRBA, RNEA(a=0) in FP32; FP64 tree factor, KKT solve and LLT; J = both feet's 6D Jacobians"}
 
+    # constraintDynamics: both feet held in 6-D on the world frames where they are at a
+    # reference posture q0, LOCAL / LOCAL_WORLD_ALIGNED, Baumgarte gains (12 rows), the
+    # reference's default settings (mu 0, one iteration); configurations scattered around q0
+    qc = M.perturb_configurations(desc, q32[:, 0], B, 0.25, M.INPUT_SEED_BASE + 900).astype(np.float32)
+    Qc = torch.tensor(qc, device=dev).contiguous()
+    oM0 = eng.forward_kinematics(Q[:, :1].contiguous())[0].double().cpu().numpy()[:, 0]
+    foot = np.concatenate([np.eye(3).ravel(), [0.03, -0.01, -0.08]])
+
+    def foot_world(j):
+        R, p = oM0[12 * (j - 1):12 * (j - 1) + 9].reshape(3, 3), oM0[12 * (j - 1) + 9:12 * j]
+        return np.concatenate([R.ravel(), p + R @ foot[9:]])
+
+    rig = [pa.RigidConstraintModel(pa.CONTACT_6D, 7, foot, 0, foot_world(7), pa.LOCAL, [10] * 6, [2] * 6),
+           pa.RigidConstraintModel(pa.CONTACT_6D, 13, foot, 0, foot_world(13), pa.LOCAL_WORLD_ALIGNED, [10] * 6,
+                                   [2] * 6)]
+    kddq = torch.empty_like(cddq)
+    klam = torch.empty_like(clam)
+    t = time_call(lambda: eng.constraint_dynamics(Qc, V, T, rig, out=kddq, lambda_out=klam, async_=True), reps=10)
+    others["constraint_dynamics_humanoid36_f32"] = {
+        "batch": B, "constraints": "two 6D feet, Baumgarte Kp 10 Kd 2", "configs_per_s": B / t, "ms": t * 1e3,
+        "note": "kinematics, CRBA, RNEA(a=0) in FP32; constraint rows, FP64 tree factor and KKT in FP64"}
+
     idq = torch.empty_like(cddq)
     iimp = torch.empty_like(clam)
     t = time_call(lambda: eng.impulse_dynamics(Q, V, Jrows, r_coeff=0.0, out=idq, impulse_out=iimp, async_=True),
@@ -528,6 +550,11 @@ def main():
                                                      np.zeros((ncon, nd)))
         errors["constrained_ddq"] = rel_cols(cddq[:, :nd].double().cpu().numpy(), ra)
         errors["constrained_lambda"] = rel_cols(clam[:, :nd].double().cpu().numpy(), rl)
+        rcon = [dict(type=c.type, joint1=c.joint1_id, joint2=c.joint2_id, reference_frame=c.reference_frame,
+                     placement1=c.joint1_placement, placement2=c.joint2_placement, Kp=c.Kp, Kd=c.Kd) for c in rig]
+        ka, kl, _ = oracle.constraint_dynamics(desc, qc[:, :nd].astype(np.float64), vs[:, :nd], ts[:, :nd], rcon)
+        errors["constraint_dynamics_ddq"] = rel_cols(kddq[:, :nd].double().cpu().numpy(), ka)
+        errors["constraint_dynamics_lambda"] = rel_cols(klam[:, :nd].double().cpu().numpy(), kl)
         rm = oracle.rnea(man, qm[:, :1024], vm[:, :1024], am[:, :1024])
         errors["rnea_manipulator6_f64"] = rel_cols(em.rnea(Qm[:, :1024].contiguous(), Vm[:, :1024].contiguous(),
                                                            Am[:, :1024].contiguous()).cpu().numpy(), rm)

@@ -243,6 +243,31 @@ inline void forwardDynamicsInParallel(size_t num_threads, ModelPool& pool, const
                                          RBD_HOST_PTRS));
 }
 
+// constraintDynamics(model, data, q, v, tau, contact_models, contact_datas, settings) per
+// column (constrained-dynamics.hxx:193-500): the constraints as rbd_rigid_constraint, the
+// same for every column; lambda has the constraints' total dimension in rows (at most 12);
+// iterations (optional, B entries) receives settings.iter per column.  FP64 throughout.
+inline void constraintDynamicsInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,
+                                         const CMat& tau, const std::vector<rbd_rigid_constraint>& contacts,
+                                         const rbd_proximal_settings& settings, const MMat& ddq, const MMat& lambda,
+                                         int32_t* iterations = nullptr) {
+  detail::checkThreads(num_threads, pool);
+  detail::checkRows(q, pool.nq(), "The configuration matrix");
+  detail::checkRows(v, pool.nv(), "The velocity matrix");
+  detail::checkRows(tau, pool.nv(), "The joint torque matrix");
+  detail::checkRows(ddq, pool.nv(), "The output matrix");
+  detail::checkCols(v, q.cols, "The velocity matrix");
+  detail::checkCols(tau, q.cols, "The joint torque matrix");
+  detail::checkCols(ddq, q.cols, "The output matrix");
+  int nrows = 0;
+  for (const auto& c : contacts) nrows += c.type == RBD_CONTACT_6D ? 6 : 3;
+  detail::checkRows(lambda, nrows, "The multiplier matrix");
+  detail::checkCols(lambda, q.cols, "The multiplier matrix");
+  check(rbd_constraint_dynamics(pool.ctx(), pool.model(), q.cols, RBD_F64, q.data, q.ld, v.data, v.ld, tau.data,
+                                tau.ld, (int)contacts.size(), contacts.data(), &settings, ddq.data, ddq.ld,
+                                lambda.data, lambda.ld, iterations, RBD_HOST_PTRS));
+}
+
 // computeRNEADerivatives per column: three nv x (nv * B) outputs (blocks side by side),
 // FP64 by default.
 inline void computeRNEADerivativesInParallel(size_t num_threads, ModelPool& pool, const CMat& q, const CMat& v,

@@ -20,6 +20,7 @@
  *   rbd_joint_jacobians   computeJointJacobians     algorithm/jacobian.hxx:23-62
  *   rbd_frames            updateFramePlacements,    algorithm/frames.hxx:15-35,
  *                         getFrameJacobian          algorithm/frames.hxx:156-175
+ *   rbd_constraint_dynamics constraintDynamics      algorithm/constrained-dynamics.hxx:193-500
  *   rbd_impulse_dynamics  impulseDynamics(q, v_before, J, r_coeff, inv_damping)
  *                                                   algorithm/contact-dynamics.hxx:264-331
  *   rbd_constrained_forward_dynamics
@@ -230,6 +231,46 @@ int rbd_constrained_forward_dynamics(rbd_ctx* ctx, const rbd_model* model, int64
                                      const void* J, int64_t ldJ, const void* gamma, int64_t ldgamma,
                                      double inv_damping, void* ddq, int64_t ldddq, void* lambda, int64_t ldlambda,
                                      uint32_t flags);
+
+/* Rigid constraints of constraintDynamics: RigidConstraintModel (contact-info.hpp:95-420)
+ * with the RigidConstraintData field constraintDynamics reads (contact_acceleration_desired,
+ * zero by default).  A CONTACT_3D constraint holds the point of joint1's frame on the point
+ * of joint2's frame (3 rows), CONTACT_6D the two frames together (6 rows); joint 0 is the
+ * universe (a contact with the environment).  Placements: R row major, then p. */
+enum { RBD_CONTACT_3D = 0, RBD_CONTACT_6D = 1 };
+typedef struct rbd_rigid_constraint {
+  int32_t type;             /* RBD_CONTACT_3D or RBD_CONTACT_6D */
+  int32_t joint1_id, joint2_id;
+  int32_t reference_frame;  /* RBD_LOCAL or RBD_LOCAL_WORLD_ALIGNED (RBD_WORLD is refused, like the reference) */
+  double joint1_placement[12], joint2_placement[12];
+  double Kp[6], Kd[6];      /* Baumgarte corrector gains (BaumgarteCorrectorParameters); first 3 for 3D */
+  double desired_acceleration[6]; /* contact_acceleration_desired: linear, angular */
+} rbd_rigid_constraint;
+
+/* ProximalSettingsTpl (proximal.hpp:20-60); NULL settings = the reference's defaults
+ * (accuracies 1e-12, mu 0, max_iter 1). */
+typedef struct rbd_proximal_settings {
+  double absolute_accuracy, relative_accuracy, mu;
+  int32_t max_iter;
+} rbd_proximal_settings;
+
+/* constraintDynamics(model, data, q, v, tau, contact_models, contact_datas, settings)
+ * (constrained-dynamics.hpp:90-147, constrained-dynamics.hxx:193-500) per configuration:
+ * the constraint Jacobian and drift of every constraint (Baumgarte-corrected), then
+ * [[-mu I, J], [J^T, M]] [-lambda; ddq] = [rhs + mu lambda_prox; tau - nle] solved with
+ * max_iter proximal iterations and the reference's stopping test.  Outputs ddq (nv),
+ * lambda (data.lambda_c: the constraints' rows in order, at most 12 in all) and, if
+ * `iterations` is not NULL, the iteration count of each configuration (settings.iter;
+ * host or device memory like the other arrays).  The constraint table and settings are
+ * host memory.  FP32: kinematics, CRBA and RNEA in FP32, the constraint rows, the factor
+ * and the solve in FP64; FP64: all in double.  Prebuilt topologies only.  A CONTACT_6D
+ * placement error is log6 of the relative placement, taken away from rotations by pi
+ * (within cbrt(eps) of pi, where the reference switches to its own singular branch, the
+ * value is not specified). */
+int rbd_constraint_dynamics(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                            int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, int ncontacts,
+                            const rbd_rigid_constraint* contacts, const rbd_proximal_settings* settings, void* ddq,
+                            int64_t ldddq, void* lambda, int64_t ldlambda, int32_t* iterations, uint32_t flags);
 
 /* impulseDynamics (contact-dynamics.hpp:241-300, contact-dynamics.hxx:264-331): the velocity
  * after an impact, dq_after = v_before + M^-1 J^T impulse with

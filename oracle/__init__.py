@@ -298,3 +298,57 @@ def time_batch(model, alg, q, v, x, nthreads=0, reps=5):
     _check(lib().orc_time(ctypes.byref(d), TIME_ALGS[alg], ctypes.c_int64(B), _p(q), _p(v), _p(x), int(nthreads),
                           int(reps), ctypes.byref(sec)))
     return sec.value
+
+
+CONTACT_3D, CONTACT_6D = 0, 1
+LOCAL, LOCAL_WORLD_ALIGNED = 1, 2
+
+
+class _Constraint(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("joint1", ctypes.c_int32), ("joint2", ctypes.c_int32),
+                ("rf", ctypes.c_int32), ("placement1", ctypes.c_double * 12), ("placement2", ctypes.c_double * 12),
+                ("Kp", ctypes.c_double * 6), ("Kd", ctypes.c_double * 6), ("desired", ctypes.c_double * 6)]
+
+
+def constraint_table(constraints):
+    """ctypes array of rigid constraints from dicts with keys type, joint1, joint2,
+    reference_frame, placement1, placement2 (12 numbers: R row major, p), Kp, Kd, desired
+    (3 or 6 numbers; missing ones are zero)."""
+    arr = (_Constraint * len(constraints))()
+    for k, c in enumerate(constraints):
+        a = arr[k]
+        a.type = int(c["type"])
+        a.joint1 = int(c.get("joint1", 0))
+        a.joint2 = int(c.get("joint2", 0))
+        a.rf = int(c.get("reference_frame", LOCAL))
+        eye = [1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]
+        for name in ("placement1", "placement2"):
+            vals = np.asarray(c.get(name, eye), dtype=np.float64).ravel()
+            assert vals.size == 12
+            getattr(a, name)[:] = list(vals)
+        for name in ("Kp", "Kd", "desired"):
+            vals = np.zeros(6)
+            x = np.asarray(c.get(name, []), dtype=np.float64).ravel()
+            vals[:x.size] = x
+            getattr(a, name)[:] = list(vals)
+    return arr
+
+
+def constraint_dynamics(model, q, v, tau, constraints, mu=0.0, max_iter=1, absolute_accuracy=1e-12,
+                        relative_accuracy=1e-12, scalar="f64", nthreads=0):
+    """constraintDynamics(model, data, q, v, tau, contact_models, contact_datas, settings) per
+    column (constrained-dynamics.hxx:193-500).  constraints: list of dicts (constraint_table).
+    Returns (ddq (nv, B), lambda (nrows, B), iterations (B,))."""
+    q, v, tau = _cols(q, model.nq), _cols(v, model.nv), _cols(tau, model.nv)
+    B, nv = q.shape[1], model.nv
+    arr = constraint_table(constraints)
+    nrows = sum(6 if c["type"] == CONTACT_6D else 3 for c in constraints)
+    ddq = np.zeros((nv, B), order="F")
+    lam = np.zeros((nrows, B), order="F")
+    its = np.zeros(B, dtype=np.int32)
+    d = model.cstruct()
+    _check(lib().orc_constraint_dynamics(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(v), _p(tau), len(constraints),
+                                         arr, ctypes.c_double(absolute_accuracy), ctypes.c_double(relative_accuracy),
+                                         ctypes.c_double(mu), int(max_iter), _p(ddq), _p(lam),
+                                         its.ctypes.data_as(ctypes.c_void_p), SCALARS[scalar], nthreads))
+    return np.ascontiguousarray(ddq), np.ascontiguousarray(lam), its

@@ -592,3 +592,85 @@ extern "C" int orc_time(const ModelDesc* d, int alg, int64_t B, const double* q,
     return 1;
   }
 }
+
+// ---- constraintDynamics with rigid constraints --------------------------------------------
+// One rigid constraint as the C API receives it (the same layout as the engine's
+// rbd_rigid_constraint): type (0: CONTACT_3D, 1: CONTACT_6D), joint1, joint2, reference
+// frame (1: LOCAL, 2: LOCAL_WORLD_ALIGNED), the two joint placements (R row major, p),
+// the corrector gains and the desired contact acceleration.
+struct OrcConstraint {
+  int32_t type, joint1, joint2, rf;
+  double placement1[12], placement2[12];
+  double Kp[6], Kd[6], desired[6];
+};
+
+namespace {
+template<class S>
+int cdT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau, int ncon,
+        const OrcConstraint* oc, double absAcc, double relAcc, double mu, int maxIter, double* ddq, double* lambda,
+        int32_t* iters, int nt) {
+  std::vector<RigidConstraint<S>> cons(ncon);
+  int nrows = 0;
+  for (int c = 0; c < ncon; ++c) {
+    RigidConstraint<S>& r = cons[c];
+    r.type = oc[c].type;
+    r.joint1 = oc[c].joint1;
+    r.joint2 = oc[c].joint2;
+    r.rf = oc[c].rf;
+    if (r.type != CONTACT_3D && r.type != CONTACT_6D) throw std::invalid_argument("bad constraint type");
+    for (int k = 0; k < 9; ++k) {
+      r.placement1.R.m[k] = S(oc[c].placement1[k]);
+      r.placement2.R.m[k] = S(oc[c].placement2[k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      r.placement1.p[k] = S(oc[c].placement1[9 + k]);
+      r.placement2.p[k] = S(oc[c].placement2[9 + k]);
+    }
+    for (int k = 0; k < 6; ++k) {
+      r.Kp[k] = S(oc[c].Kp[k]);
+      r.Kd[k] = S(oc[c].Kd[k]);
+      r.desired[k] = S(oc[c].desired[k]);
+    }
+    nrows += r.size();
+  }
+  const ProximalSettings<S> ps{S(absAcc), S(relAcc), S(mu), maxIter};
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    for (const auto& c : cons)
+      if (c.joint1 < 0 || c.joint1 >= m.njoints || c.joint2 < 0 || c.joint2 >= m.njoints)
+        throw std::invalid_argument("constraint joint out of range");
+    static thread_local Buf<S> bq, bv, bt;
+    static thread_local std::vector<S> a, l;
+    a.resize(m.nv);
+    l.resize(nrows);
+    int it = 0;
+    constraintDynamics(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * m.nv, m.nv),
+                       bt.load(tau + b * m.nv, m.nv), ncon, cons.data(), ps, a.data(), l.data(), &it);
+    store(a.data(), ddq + b * m.nv, m.nv);
+    store(l.data(), lambda + b * nrows, nrows);
+    if (iters) iters[b] = it;
+  });
+}
+}  // namespace
+
+extern "C" int orc_constraint_dynamics(const ModelDesc* d, int64_t B, const double* q, const double* v,
+                                       const double* tau, int ncon, const OrcConstraint* cons, double absolute_accuracy,
+                                       double relative_accuracy, double mu, int max_iter, double* ddq, double* lambda,
+                                       int32_t* iterations, int scalar, int nthreads) {
+  try {
+    if (ncon < 1 || !cons) throw std::invalid_argument("at least one constraint");
+    if (max_iter < 1) throw std::invalid_argument("max_iter must be positive");
+#define CALL(S) \
+  cdT<S>(d, B, q, v, tau, ncon, cons, absolute_accuracy, relative_accuracy, mu, max_iter, ddq, lambda, iterations, nthreads)
+    switch (scalar) {
+      case 0: return CALL(double);
+      case 1: return CALL(float);
+      case 2: return CALL(long double);
+    }
+#undef CALL
+    g_err = "bad scalar";
+    return 1;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}

@@ -22,6 +22,8 @@
 //   integrate (Lie group)    multibody/liegroup/special-euclidean.hpp:659-698,
 //                            spatial/explog-quaternion.hpp:92-147
 //   batch wrappers           algorithm/parallel/rnea.hpp:39-83, algorithm/parallel/aba.hpp:40-84
+//   constraint dynamics      algorithm/constrained-dynamics.hxx:71-500, contact-info.hpp:426-700,
+//                            spatial/log.hxx:58-230
 //
 // Only the joint types the engine supports are restated: FreeFlyer and Revolute{X,Y,Z}.
 // Conventions follow the reference: 6-vectors are [linear; angular]; Inertia = (m, c, I_c)
@@ -1075,6 +1077,276 @@ inline void impulseDynamics(const Model<S>& model, Data<S>& data, const S* q, co
   }
   choleskySolve(model, data, dq_after);
   for (int k = 0; k < nv; ++k) dq_after[k] += v_before[k];
+}
+
+// ----------------------------------------------------------------------------------------
+// constraintDynamics with rigid constraints (constrained-dynamics.hxx:193-500), restricted
+// to CONTACT_3D / CONTACT_6D constraints expressed in LOCAL or LOCAL_WORLD_ALIGNED:
+//   RigidConstraintModel::calc / jacobian (contact-info.hpp:426-700): oMc1 = oMi[j1] M1,
+//   oMc2 = oMi[j2] M2, c1Mc2 = oMc1^-1 oMc2, and the constraint Jacobian from the WORLD joint
+//   Jacobian columns, signed by which joint's support holds the column;
+//   the contact drifts from ov, oa of the forward step at a = 0 (constrained-dynamics.hxx:
+//   71-135: oa = oa_parent + ov_parent x ov, the joints here having c = 0), placement and
+//   velocity errors, Baumgarte correction -Kp e - Kd de (log6 for 6D placements);
+//   the KKT [[-mu I, J], [J^T, M]] [x; ddq] = [rhs + mu lambda_prox; tau - nle] solved
+//   max_iter times (proximal iterations) with the reference's residuals:
+//     absolute = |(J M^-1 J^T) x - rhs|_inf,  relative = |x + lambda_prox|_inf,
+//   stopping when either is within its accuracy; lambda = -x, lambda_prox = -x.
+// The reference factors the whole KKT matrix (contact Cholesky, contact-cholesky.hxx:
+// 184-324); here the same system is solved through its Schur complement
+// (J M^-1 J^T + mu I, LLT) and the tree factor of M -- the same solution up to rounding.
+// ----------------------------------------------------------------------------------------
+enum { CONTACT_3D = 0, CONTACT_6D = 1 };
+template<class S> struct RigidConstraint {
+  int type, joint1, joint2, rf;  // rf: RF_LOCAL or RF_LOCAL_WORLD_ALIGNED
+  SE3<S> placement1, placement2;
+  S Kp[6], Kd[6];                // corrector gains (first 3 for CONTACT_3D)
+  S desired[6];                  // contact_acceleration_desired (linear, angular)
+  int size() const { return type == CONTACT_6D ? 6 : 3; }
+};
+template<class S> struct ProximalSettings {
+  S absolute_accuracy, relative_accuracy, mu;
+  int max_iter;
+};
+
+// unSkew (spatial/skew.hpp): axial vector of the antisymmetric part of R
+template<class S> inline V3<S> unSkew(const M3<S>& R) {
+  return v3(S(0.5) * (R(2, 1) - R(1, 2)), S(0.5) * (R(0, 2) - R(2, 0)), S(0.5) * (R(1, 0) - R(0, 1)));
+}
+// log3 (spatial/log.hxx:58-183) away from theta = pi (the constraints' placement errors are
+// small): theta from the trace (Taylor expansion near 0), w = theta / sin(theta) unSkew(R)
+template<class S> inline V3<S> log3(const M3<S>& R, S& theta) {
+  using std::acos; using std::sin; using std::sqrt; using std::cbrt;
+  const S eps = std::numeric_limits<S>::epsilon();
+  const S prec = cbrt(eps);  // TaylorSeriesExpansion<S>::precision<2>()
+  const S tr = R(0, 0) + R(1, 1) + R(2, 2);
+  const S cv = (tr - S(1)) / S(2);
+  if (!(tr > S(-1) + prec)) throw std::invalid_argument("log3: rotation angle too close to pi");
+  theta = tr <= S(3) - prec ? acos(cv) : sqrt(S(2) * (S(1) - cv) + eps * eps);
+  const V3<S> a = unSkew(R);
+  const S n2 = dot(a, a);
+  const S t = theta >= prec ? theta / sin(theta) : S(1) + n2 / S(6) + n2 * n2 * S(3) / S(40);
+  return t * a;
+}
+// log6 (spatial/log.hxx:189-230)
+template<class S> inline Motion<S> log6(const SE3<S>& M) {
+  using std::sin; using std::cos; using std::cbrt;
+  const S prec = cbrt(std::numeric_limits<S>::epsilon());
+  S theta;
+  const V3<S> w = log3(M.R, theta);
+  const V3<S> a = unSkew(M.R);
+  const S t2 = dot(a, a);
+  const S tr = M.R(0, 0) + M.R(1, 1) + M.R(2, 2);
+  const S st = sin(theta), ct = cos(theta);
+  const bool small = tr >= S(3) - prec;
+  const S alpha = small ? S(1) - t2 / S(12) - t2 * t2 / S(720) : theta * st / (S(2) * (S(1) - ct));
+  const S beta = small ? S(1) / S(12) + t2 / S(720) : S(1) / (theta * theta) - st / (S(2) * theta * (S(1) - ct));
+  const V3<S>& p = M.p;
+  return {alpha * p - S(0.5) * cross(w, p) + (beta * dot(w, p)) * w, w};
+}
+template<class S> inline SE3<S> inverse(const SE3<S>& M) { return SE3<S>{transpose(M.R), -mulT(M.R, M.p)}; }
+
+// rhs rows and Jacobian rows (nrows x nv, column major at Jc[i + k nrows]) of the
+// constraints, after the forward step (data.oMi, data.ov, data.oa, data.J filled).
+template<class S>
+inline void constraintTerms(const Model<S>& model, const Data<S>& data, int ncon, const RigidConstraint<S>* cons,
+                            int nrows, S* rhs, S* Jc) {
+  const int nv = model.nv;
+  std::fill(Jc, Jc + (size_t)nrows * nv, S(0));
+  std::vector<char> sup1(nv), sup2(nv);
+  int row = 0;
+  for (int c = 0; c < ncon; ++c) {
+    const RigidConstraint<S>& cm = cons[c];
+    const int j1 = cm.joint1, j2 = cm.joint2, dim = cm.size();
+    const SE3<S> oMc1 = j1 > 0 ? data.oMi[j1] * cm.placement1 : cm.placement1;
+    const SE3<S> oMc2 = j2 > 0 ? data.oMi[j2] * cm.placement2 : cm.placement2;
+    const SE3<S> c1Mc2 = inverse(oMc1) * oMc2;
+    const Motion<S> zm = zeroMotion<S>();
+    const Motion<S>& ov1 = j1 > 0 ? data.ov[j1] : zm;
+    const Motion<S>& ov2 = j2 > 0 ? data.ov[j2] : zm;
+    const Motion<S>& oa1 = j1 > 0 ? data.oa[j1] : zm;
+    const Motion<S>& oa2 = j2 > 0 ? data.oa[j2] : zm;
+    const Motion<S> vc1 = j1 > 0 ? actInv(oMc1, ov1) : zm;
+    const Motion<S> vc2 = j2 > 0 ? actInv(oMc2, ov2) : zm;
+    const Motion<S> vc2in1 = act(c1Mc2, vc2);
+    Motion<S> perr, verr, aerr = zm;
+    if (cm.type == CONTACT_6D) {
+      const Motion<S> l = log6(c1Mc2);
+      perr = {-l.lin, -l.ang};
+      verr = vc1 - vc2in1;
+    } else {
+      perr = {-c1Mc2.p, zero3<S>()};
+      verr = {vc1.lin - c1Mc2.R * vc2.lin, zero3<S>()};
+    }
+    bool gains = false;
+    for (int k = 0; k < dim; ++k) gains = gains || cm.Kp[k] != S(0) || cm.Kd[k] != S(0);
+    if (gains) {
+      for (int k = 0; k < 3; ++k) aerr.lin[k] = -cm.Kp[k] * perr.lin[k] - cm.Kd[k] * verr.lin[k];
+      if (cm.type == CONTACT_6D)
+        for (int k = 0; k < 3; ++k) aerr.ang[k] = -cm.Kp[3 + k] * perr.ang[k] - cm.Kd[3 + k] * verr.ang[k];
+    }
+    Motion<S> acc1, acc2;
+    if (cm.rf == RF_LOCAL_WORLD_ALIGNED) {
+      acc1.lin = oa1.lin + cross(oa1.ang, oMc1.p);
+      if (cm.type == CONTACT_3D) acc1.lin += cross(ov1.ang, ov1.lin + cross(ov1.ang, oMc1.p));
+      acc1.ang = oa1.ang;
+      if (cm.type == CONTACT_3D) {
+        acc2.lin = oa2.lin + cross(oa2.ang, oMc2.p) + cross(ov2.ang, ov2.lin + cross(ov2.ang, oMc2.p));
+        acc2.ang = zero3<S>();
+      } else {
+        acc2.lin = oa2.lin + cross(oa2.ang, oMc1.p);  // (the reference takes c1's origin here)
+        acc2.ang = oa2.ang;
+      }
+      aerr = {oMc1.R * aerr.lin, oMc1.R * aerr.ang};
+    } else if (cm.rf == RF_LOCAL) {
+      acc1 = actInv(oMc1, oa1);
+      if (cm.type == CONTACT_3D) {
+        acc1.lin += cross(vc1.ang, vc1.lin);
+        acc1.ang = zero3<S>();
+      } else {
+        acc1 = acc1 + crossMM(verr, vc2in1);
+      }
+      if (cm.type == CONTACT_3D) {
+        acc2.lin = mulT(oMc1.R, oa2.lin + cross(oa2.ang, oMc2.p) + cross(ov2.ang, ov2.lin + cross(ov2.ang, oMc2.p)));
+        acc2.ang = zero3<S>();
+      } else {
+        acc2 = actInv(oMc1, oa2);
+      }
+    } else {
+      throw std::invalid_argument("constraintDynamics operates from LOCAL or LOCAL_WORLD_ALIGNED frames");
+    }
+    S r6[6], a1[6], a2[6], e6[6];
+    toVec(acc1, a1);
+    toVec(acc2, a2);
+    toVec(aerr, e6);
+    for (int k = 0; k < 6; ++k) r6[k] = -a1[k] + a2[k] + e6[k] + cm.desired[k];
+    for (int k = 0; k < dim; ++k) rhs[row + k] = r6[k];
+    // Jacobian (contact-info.hpp:551-700)
+    std::fill(sup1.begin(), sup1.end(), 0);
+    std::fill(sup2.begin(), sup2.end(), 0);
+    for (int j = j1; j > 0; j = model.parents[j])
+      for (int k = 0; k < model.nvs[j]; ++k) sup1[model.idx_v[j] + k] = 1;
+    for (int j = j2; j > 0; j = model.parents[j])
+      for (int k = 0; k < model.nvs[j]; ++k) sup2[model.idx_v[j] + k] = 1;
+    for (int k = 0; k < nv; ++k) {
+      if (!sup1[k] && !sup2[k]) continue;
+      const int sign = sup1[k] != sup2[k] ? (sup1[k] ? 1 : -1) : 0;
+      const Motion<S> Jcol = colM(data.J, k);
+      S col[6] = {S(0), S(0), S(0), S(0), S(0), S(0)};
+      if (cm.type == CONTACT_3D) {
+        V3<S> l;
+        if (cm.rf == RF_LOCAL) {
+          if (sign == 0) l = actInv(oMc1, Jcol).lin - c1Mc2.R * actInv(oMc2, Jcol).lin;
+          else if (sign == 1) l = actInv(oMc1, Jcol).lin;
+          else l = -(c1Mc2.R * actInv(oMc2, Jcol).lin);
+        } else {
+          if (sign == 0) l = cross(oMc2.p - oMc1.p, Jcol.ang);
+          else l = S(sign) * (Jcol.lin - cross(sign == 1 ? oMc1.p : oMc2.p, Jcol.ang));
+        }
+        for (int r = 0; r < 3; ++r) col[r] = l[r];
+      } else {
+        Motion<S> m = cm.rf == RF_LOCAL ? actInv(oMc1, Jcol) : Motion<S>{Jcol.lin - cross(oMc1.p, Jcol.ang), Jcol.ang};
+        toVec(S(sign) * m, col);
+      }
+      for (int r = 0; r < dim; ++r) Jc[(row + r) + (size_t)k * nrows] = col[r];
+    }
+    row += dim;
+  }
+}
+
+template<class S>
+inline void constraintDynamics(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* tau, int ncon,
+                               const RigidConstraint<S>* cons, const ProximalSettings<S>& settings, S* ddq, S* lambda,
+                               int* iterations) {
+  using std::abs;
+  using std::sqrt;
+  const int nv = model.nv;
+  if (settings.mu < S(0)) throw std::invalid_argument("mu has to be positive");
+  int nrows = 0;
+  for (int c = 0; c < ncon; ++c) nrows += cons[c].size();
+  // forward step (constrained-dynamics.hxx:71-135) for ov, oa; the WORLD joint Jacobian
+  for (int i = 1; i < model.njoints; ++i) {
+    const int p = model.parents[i];
+    const JointCalc<S> jc = jointCalc(model, i, q, v);
+    data.liMi[i] = model.placements[i] * jc.M;
+    data.oMi[i] = p > 0 ? data.oMi[p] * data.liMi[i] : data.liMi[i];
+    data.ov[i] = act(data.oMi[i], jc.v);
+    if (p > 0) data.ov[i] = data.ov[i] + data.ov[p];
+    data.oa[i] = p > 0 ? crossMM(data.ov[p], data.ov[i]) + data.oa[p] : zeroMotion<S>();
+    for (int k = 0; k < jc.nv; ++k) setCol(data.J, model.idx_v[i] + k, act(data.oMi[i], jc.Scol[k]));
+  }
+  std::vector<S> rhs(nrows), Jc((size_t)nrows * nv), r(nv), zero(nv, S(0));
+  constraintTerms(model, data, ncon, cons, nrows, rhs.data(), Jc.data());
+  // M, nle, tree factor; r = M^-1 (tau - nle); Y = M^-1 J^T; G = J Y + mu I
+  rnea(model, data, q, v, zero.data(), r.data());
+  for (int k = 0; k < nv; ++k) r[k] = tau[k] - r[k];
+  crbaWorld(model, data, q);
+  choleskyDecompose(model, data);
+  std::vector<S> rv = r;
+  choleskySolve(model, data, r.data());
+  std::vector<S> Y((size_t)nv * nrows), G((size_t)nrows * nrows), L((size_t)nrows * nrows, S(0));
+  for (int i = 0; i < nrows; ++i) {
+    S* col = &Y[(size_t)i * nv];
+    for (int k = 0; k < nv; ++k) col[k] = Jc[i + (size_t)k * nrows];
+    choleskySolve(model, data, col);
+  }
+  for (int i = 0; i < nrows; ++i)
+    for (int j = 0; j < nrows; ++j) {
+      S g = S(0);
+      for (int k = 0; k < nv; ++k) g += Jc[i + (size_t)k * nrows] * Y[(size_t)j * nv + k];
+      G[i + (size_t)j * nrows] = g;
+    }
+  for (int j = 0; j < nrows; ++j) {
+    S d = G[j + (size_t)j * nrows] + settings.mu;
+    for (int k = 0; k < j; ++k) d -= L[j + (size_t)k * nrows] * L[j + (size_t)k * nrows];
+    L[j + (size_t)j * nrows] = sqrt(d);
+    for (int i = j + 1; i < nrows; ++i) {
+      S t = G[i + (size_t)j * nrows];
+      for (int k = 0; k < j; ++k) t -= L[i + (size_t)k * nrows] * L[j + (size_t)k * nrows];
+      L[i + (size_t)j * nrows] = t / L[j + (size_t)j * nrows];
+    }
+  }
+  // J r: the Schur complement's constant part
+  std::vector<S> Jr(nrows), x(nrows), prox(nrows, S(0));
+  for (int i = 0; i < nrows; ++i) {
+    S s = S(0);
+    for (int k = 0; k < nv; ++k) s += Jc[i + (size_t)k * nrows] * r[k];
+    Jr[i] = s;
+  }
+  int it = 0;
+  while (it < settings.max_iter) {
+    ++it;
+    // x = (G + mu I)^-1 (J r - rhs - mu lambda_prox)
+    for (int i = 0; i < nrows; ++i) x[i] = Jr[i] - rhs[i] - settings.mu * prox[i];
+    for (int i = 0; i < nrows; ++i) {
+      S s = x[i];
+      for (int k = 0; k < i; ++k) s -= L[i + (size_t)k * nrows] * x[k];
+      x[i] = s / L[i + (size_t)i * nrows];
+    }
+    for (int i = nrows - 1; i >= 0; --i) {
+      S s = x[i];
+      for (int k = i + 1; k < nrows; ++k) s -= L[k + (size_t)i * nrows] * x[k];
+      x[i] = s / L[i + (size_t)i * nrows];
+    }
+    S absr = S(0), relr = S(0);
+    for (int i = 0; i < nrows; ++i) {
+      S g = S(0);
+      for (int j = 0; j < nrows; ++j) g += G[i + (size_t)j * nrows] * x[j];
+      absr = std::max(absr, abs(g - rhs[i]));
+      relr = std::max(relr, abs(x[i] + prox[i]));
+    }
+    for (int i = 0; i < nrows; ++i) prox[i] = -x[i];
+    if (absr <= settings.absolute_accuracy || relr <= settings.relative_accuracy) break;
+  }
+  if (iterations) *iterations = it;
+  // ddq = M^-1 (tau - nle - J^T x), lambda = -x
+  for (int k = 0; k < nv; ++k) {
+    S s = rv[k];
+    for (int i = 0; i < nrows; ++i) s -= Jc[i + (size_t)k * nrows] * x[i];
+    ddq[k] = s;
+  }
+  choleskySolve(model, data, ddq);
+  for (int i = 0; i < nrows; ++i) lambda[i] = -x[i];
 }
 
 // ----------------------------------------------------------------------------------------

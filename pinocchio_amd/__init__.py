@@ -402,6 +402,31 @@ class Engine:
             _ptr(impulse_out), _i64(ldi), ctypes.c_uint32(self._flags(async_))))
         return out, impulse_out
 
+    def constraint_dynamics(self, q, v, tau, constraints, settings=None, out=None, lambda_out=None,
+                            iterations_out=None, async_=False):
+        """constraintDynamics(model, data, q, v, tau, contact_models, contact_datas, settings)
+        per configuration (constrained-dynamics.hxx:193-500).  constraints: a list of
+        RigidConstraintModel; settings: ProximalSettings (None: the reference's defaults).
+        Returns (ddq, lambda) -- lambda holds the constraints' rows in order -- and fills
+        iterations_out (int32, one per configuration) when given."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        _, ldt = self._vec(tau, m.nv, "tau")
+        table = _constraint_table(constraints)
+        nrows = sum(c.size() for c in constraints)
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        lambda_out = self._new_vec(q, nrows, B) if lambda_out is None else lambda_out
+        _, lda = self._vec(out, m.nv, "ddq")
+        _, ldl = self._vec(lambda_out, nrows, "lambda")
+        ps = None if settings is None else ctypes.byref(settings._c())
+        it = None if iterations_out is None else _ptr(iterations_out)
+        _check(lib().rbd_constraint_dynamics(
+            m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v), _i64(ldv), _ptr(tau),
+            _i64(ldt), len(constraints), table, ps, _ptr(out), _i64(lda), _ptr(lambda_out), _i64(ldl), it,
+            ctypes.c_uint32(self._flags(async_))))
+        return out, lambda_out
+
     @staticmethod
     def mats_to_numpy(t, layout, nv):
         """Device matrix output -> (B, nv, nv) numpy array with [b] the configuration's matrix."""
@@ -410,6 +435,91 @@ class Engine:
             return a.transpose(0, 2, 1)  # [b, c, r] -> [b, r, c]
         B = a.shape[1]
         return a.reshape(nv, nv, B).transpose(2, 1, 0)  # [c, r, b] -> [b, r, c]
+
+
+# ------------------------------------------------------------------------------------
+# Rigid constraints (RigidConstraintModel, ProximalSettings) for constraint_dynamics
+# ------------------------------------------------------------------------------------
+CONTACT_3D, CONTACT_6D = 0, 1
+WORLD, LOCAL, LOCAL_WORLD_ALIGNED = 0, 1, 2
+_IDENTITY = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+
+
+class _RigidConstraintC(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("joint1_id", ctypes.c_int32), ("joint2_id", ctypes.c_int32),
+                ("reference_frame", ctypes.c_int32), ("joint1_placement", ctypes.c_double * 12),
+                ("joint2_placement", ctypes.c_double * 12), ("Kp", ctypes.c_double * 6), ("Kd", ctypes.c_double * 6),
+                ("desired_acceleration", ctypes.c_double * 6)]
+
+
+class _ProximalSettingsC(ctypes.Structure):
+    _fields_ = [("absolute_accuracy", ctypes.c_double), ("relative_accuracy", ctypes.c_double),
+                ("mu", ctypes.c_double), ("max_iter", ctypes.c_int32)]
+
+
+class RigidConstraintModel:
+    """A rigid constraint (algorithm/contact-info.hpp:95-420): CONTACT_3D holds the point of
+    joint1's frame on the point of joint2's frame, CONTACT_6D the two frames together; joint
+    0 is the universe.  Placements are 12 numbers (R row major, then p) or 4 x 4 matrices;
+    Kp, Kd: Baumgarte corrector gains (3 or 6 numbers); desired_acceleration: the contact
+    data's contact_acceleration_desired."""
+
+    def __init__(self, type, joint1_id, joint1_placement=_IDENTITY, joint2_id=0, joint2_placement=_IDENTITY,
+                 reference_frame=LOCAL, Kp=(), Kd=(), desired_acceleration=()):
+        self.type, self.joint1_id, self.joint2_id = int(type), int(joint1_id), int(joint2_id)
+        self.reference_frame = int(reference_frame)
+        self.joint1_placement = _placement12(joint1_placement)
+        self.joint2_placement = _placement12(joint2_placement)
+        self.Kp, self.Kd = _pad6(Kp), _pad6(Kd)
+        self.desired_acceleration = _pad6(desired_acceleration)
+
+    def size(self):
+        return 6 if self.type == CONTACT_6D else 3
+
+
+class ProximalSettings:
+    """ProximalSettingsTpl (algorithm/proximal.hpp:20-60): the reference's defaults are
+    accuracy 1e-12, mu 0 and one iteration."""
+
+    def __init__(self, accuracy=1e-12, mu=0.0, max_iter=1, relative_accuracy=None):
+        self.absolute_accuracy = float(accuracy)
+        self.relative_accuracy = float(accuracy if relative_accuracy is None else relative_accuracy)
+        self.mu, self.max_iter = float(mu), int(max_iter)
+
+    def _c(self):
+        return _ProximalSettingsC(self.absolute_accuracy, self.relative_accuracy, self.mu, self.max_iter)
+
+
+def _placement12(x):
+    a = np.asarray(x, dtype=np.float64)
+    if a.shape == (4, 4):
+        a = np.concatenate([a[:3, :3].ravel(), a[:3, 3]])
+    a = a.ravel()
+    if a.size != 12:
+        raise ValueError("a placement is 12 numbers (R row major, p) or a 4 x 4 matrix")
+    return a
+
+
+def _pad6(x):
+    a = np.zeros(6)
+    v = np.asarray(x, dtype=np.float64).ravel()
+    if v.size > 6:
+        raise ValueError("at most 6 gains")
+    a[:v.size] = v
+    return a
+
+
+def _constraint_table(constraints):
+    arr = (_RigidConstraintC * len(constraints))()
+    for k, c in enumerate(constraints):
+        a = arr[k]
+        a.type, a.joint1_id, a.joint2_id, a.reference_frame = c.type, c.joint1_id, c.joint2_id, c.reference_frame
+        a.joint1_placement[:] = list(c.joint1_placement)
+        a.joint2_placement[:] = list(c.joint2_placement)
+        a.Kp[:] = list(c.Kp)
+        a.Kd[:] = list(c.Kd)
+        a.desired_acceleration[:] = list(c.desired_acceleration)
+    return arr
 
 
 # ------------------------------------------------------------------------------------

@@ -133,10 +133,11 @@ void launchKin(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> 
 
 template<class TP, class S>
 void launchKkt(hipStream_t st, GM, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J, CView<S> tau,
-               CView<S> nle, CView<S> gamma, double damping, int impulse, double rcoef, View<S> ddq, View<S> lam) {
+               CView<S> nle, CView<S> gamma, double damping, int impulse, double rcoef, const KktProx& prox,
+               View<S> ddq, View<S> lam) {
   using K = KktShape<S>;
   hipLaunchKernelGGL((k_kkt<TP, S>), dim3((unsigned)((B + K::T - 1) / K::T)), dim3(64 * K::W), 0, st, B, U, D, nc, J,
-                     tau, nle, gamma, damping, impulse, rcoef, ddq, lam);
+                     tau, nle, gamma, damping, impulse, rcoef, prox, ddq, lam);
 }
 
 template<class TP> TopoOps makeOpsBase() {

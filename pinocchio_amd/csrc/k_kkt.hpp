@@ -37,9 +37,21 @@ template<class S> struct KktShape {
 template<class TP, class S> constexpr int kktLdsBytes() {
   constexpr int T = KktShape<S>::T, NV = TP::T.nv, NC = kKktMaxConstraints;
   const int staged = (nnzOf<TP>() + NV) * T * 8 + NC * NV * T * (int)sizeof(S);
-  const int later = (NC + 1) * NV * T * (int)sizeof(S) + NC * (NC + 1) * T * 8 + NC * T * 8;
+  const int later = (NC + 1) * NV * T * (int)sizeof(S) + NC * (NC + 1) * T * 8 + 3 * NC * T * 8;
   return staged > later ? staged : later;
 }
+
+// Proximal iterations of constraintDynamics (constrained-dynamics.hxx:450-486) on top of
+// the damped solve: with G = J M^-1 J^T + mu I (mu = the damping), iteration k solves
+// lambda_k = G^-1 (b + mu lambda_{k-1}) (lambda_0 = 0, b = -J r - gamma) and stops after
+// maxIter iterations or when |gamma - (J M^-1 J^T) lambda_k|_inf <= absAcc or
+// |lambda_k - lambda_{k-1}|_inf <= relAcc -- the reference's absolute and relative
+// residuals, written for lambda = -x.  maxIter = 1 is the plain damped solve.
+struct KktProx {
+  int maxIter;
+  double absAcc, relAcc;
+  int* iters;  // per configuration (may be null)
+};
 
 // J is nc x nv per configuration, column major (element (i, k) at row i + k nc of gJ).
 // impulse != 0: impulseDynamics (contact-dynamics.hxx:288-331) -- gtau holds v_before, which
@@ -48,8 +60,8 @@ template<class TP, class S> constexpr int kktLdsBytes() {
 template<class TP, class S>
 __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<double> gU, CView<double> gD, int nc,
                                                             CView<S> gJ, CView<S> gtau, CView<S> gnle, CView<S> gg,
-                                                            double damping, int impulse, double rcoef, View<S> gddq,
-                                                            View<S> glam) {
+                                                            double damping, int impulse, double rcoef, KktProx prox,
+                                                            View<S> gddq, View<S> glam) {
   using K = KktShape<S>;
   constexpr int T = K::T, NT = 64 * K::W;
   constexpr int NV = TP::T.nv;
@@ -64,6 +76,8 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
   constexpr int kYDoubles = ((NC + 1) * NV * T * (int)sizeof(S) + 7) / 8;
   double* sP = smem + kYDoubles;
   double* sL = sP + NC * (NC + 1) * T;
+  double* sB = sL + NC * T;  // proximal iterations: b, then the previous lambda
+  double* sQ = sB + NC * T;
   const int64_t b0 = int64_t(blockIdx.x) * T;
   const int nb = (int)(B - b0 < T ? B - b0 : T);
   const int t = (int)threadIdx.x;
@@ -221,19 +235,52 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         G[gi(i, d)] = t2 * rd[d];
       }
     }
+    auto solveG = [&]() {
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
-      double s = l[i];
+      for (int i = 0; i < NC; ++i) {
+        double s = l[i];
 #pragma unroll
-      for (int k = 0; k < i; ++k) s -= G[gi(i, k)] * l[k];
-      l[i] = s * rd[i];
+        for (int k = 0; k < i; ++k) s -= G[gi(i, k)] * l[k];
+        l[i] = s * rd[i];
+      }
+#pragma unroll
+      for (int i = NC - 1; i >= 0; --i) {
+        double s = l[i];
+#pragma unroll
+        for (int k = i + 1; k < NC; ++k) s -= G[gi(k, i)] * l[k];
+        l[i] = s * rd[i];
+      }
+    };
+    if (prox.maxIter > 1) {
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        sB[i * T + t] = l[i];
+        sQ[i * T + t] = 0.0;
+      }
     }
+    solveG();
+    int it = 1;
+    while (it < prox.maxIter) {
+      double ares = 0.0, rres = 0.0;
 #pragma unroll
-    for (int i = NC - 1; i >= 0; --i) {
-      double s = l[i];
+      for (int i = 0; i < NC; ++i) {
+        if (i < nc) {
+          double g = double(gg.p[(int64_t)i * gg.rs + bt * gg.bs]);
 #pragma unroll
-      for (int k = i + 1; k < NC; ++k) s -= G[gi(k, i)] * l[k];
-      l[i] = s * rd[i];
+          for (int d = 0; d < NC; ++d)
+            if (d < nc) g -= sP[(i + d * NC) * T + t] * l[d];
+          ares = fmax(ares, fabs(g));
+          rres = fmax(rres, fabs(l[i] - sQ[i * T + t]));
+        }
+      }
+      if (ares <= prox.absAcc || rres <= prox.relAcc) break;
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        sQ[i * T + t] = l[i];
+        l[i] = sB[i * T + t] + damping * l[i];
+      }
+      solveG();
+      ++it;
     }
 #pragma unroll
     for (int i = 0; i < NC; ++i)
@@ -241,6 +288,7 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         sL[i * T + t] = l[i];
         stOut(&glam.p[(int64_t)i * glam.rs + bt * glam.bs], S(l[i]));
       }
+    if (prox.iters) prox.iters[bt] = it;
   }
   __syncthreads();
 

@@ -42,11 +42,13 @@ using LaunchKin = void (*)(hipStream_t, GM, const S* mc, int64_t B, CView<S> q, 
                            View<S> vo, View<S> ao, View<S> J);
 
 // forward dynamics under J ddq + gamma = 0 from the FP64 tree factor of M and nle: ddq,
-// lambda; impulse != 0: impulse dynamics with v_before in `tau` (nle and gamma unused)
+// lambda; impulse != 0: impulse dynamics with v_before in `tau` (nle and gamma unused);
+// prox: proximal iterations of constraintDynamics (k_kkt.hpp)
+struct KktProx;
 template<class S>
 using LaunchKkt = void (*)(hipStream_t, GM, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J,
                            CView<S> tau, CView<S> nle, CView<S> gamma, double damping, int impulse, double rcoef,
-                           View<S> ddq, View<S> lambda);
+                           const KktProx& prox, View<S> ddq, View<S> lambda);
 
 struct TopoOps {
   const char* name;

@@ -17,6 +17,7 @@
 #include "../../include/rbd.h"
 #include "k_dyn.hpp"
 #include "k_frames.hpp"
+#include "k_contact.hpp"
 #include "k_kkt.hpp"
 #include "ops.hpp"
 #include "runtime.hpp"
@@ -931,7 +932,7 @@ void cfdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t
   const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
   kkt(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
       CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{dg, vg.rs, vg.bs}, damping, 0, 0.0,
-      View<S>{da, va.rs, va.bs}, View<S>{dl, vl.rs, vl.bs});
+      KktProx{1, 0.0, 0.0, nullptr}, View<S>{da, va.rs, va.bs}, View<S>{dl, vl.rs, vl.bs});
   finishCall(ctx, flags, st);
 }
 
@@ -973,8 +974,115 @@ void impulseCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int
   }
   const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
   kkt(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nc, CView<S>{dJ, vJ.rs, vJ.bs},
-      CView<S>{dv, vv.rs, vv.bs}, CView<S>{}, CView<S>{}, damping, 1, rcoef, View<S>{da, va.rs, va.bs},
-      View<S>{di, vi.rs, vi.bs});
+      CView<S>{dv, vv.rs, vv.bs}, CView<S>{}, CView<S>{}, damping, 1, rcoef, KktProx{1, 0.0, 0.0, nullptr},
+      View<S>{da, va.rs, va.bs}, View<S>{di, vi.rs, vi.bs});
+  finishCall(ctx, flags, st);
+}
+
+// constraintDynamics(model, data, q, v, tau, contact_models, contact_datas, settings): the
+// kinematics pass at zero joint acceleration (oMi, local v and a, WORLD joint Jacobian),
+// the constraint rows (k_contact.hpp), CRBA, the FP64 tree factor, nle = RNEA at a = 0,
+// then the KKT kernel with mu as its damping and the proximal iterations.
+template<class S>
+void cdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
+            const void* tau, int64_t ldtau, int ncon, const rbd_rigid_constraint* cons, const rbd_proximal_settings& ps,
+            void* ddq, int64_t ldddq, void* lambda, int64_t ldl, int32_t* iters, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const int nv = m->nv, nj = m->njoints, nnz = m->ops->nnzU;
+  // constraint table and support flags (bit 0: joint1's support, bit 1: joint2's)
+  std::vector<RigidConstraintDev> tab(ncon);
+  std::vector<uint8_t> sup((size_t)ncon * nv, 0);
+  int nrows = 0;
+  for (int c = 0; c < ncon; ++c) {
+    const rbd_rigid_constraint& k = cons[c];
+    RigidConstraintDev& d = tab[c];
+    d.type = k.type;
+    d.j1 = k.joint1_id;
+    d.j2 = k.joint2_id;
+    d.rf = k.reference_frame;
+    d.row = nrows;
+    const int dim = k.type == RBD_CONTACT_6D ? 6 : 3;
+    d.gains = 0;
+    for (int e = 0; e < 6; ++e) {
+      d.Kp[e] = k.Kp[e];
+      d.Kd[e] = k.Kd[e];
+      d.des[e] = k.desired_acceleration[e];
+      if (e < dim && (k.Kp[e] != 0.0 || k.Kd[e] != 0.0)) d.gains = 1;
+    }
+    for (int e = 0; e < 12; ++e) {
+      d.M1[e] = k.joint1_placement[e];
+      d.M2[e] = k.joint2_placement[e];
+    }
+    for (int bit = 0; bit < 2; ++bit)
+      for (int a = bit ? k.joint2_id : k.joint1_id; a > 0; a = m->parents[a]) {
+        const int nva = m->jtype[a] == RBD_JOINT_FREEFLYER ? 6 : 1;
+        for (int e = 0; e < nva; ++e) sup[(size_t)c * nv + m->iv[a] + e] |= (uint8_t)(1 << bit);
+      }
+    nrows += dim;
+  }
+  if (nrows > kKktMaxConstraints) throw Error(RBD_EINVAL, "the constraints' total dimension must be at most 12");
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v");
+  auto vt = argView<S>(nv, B, ldtau, flags, "tau");
+  auto va = argView<S>(nv, B, ldddq, flags, "ddq");
+  auto vl = argView<S>(nrows, B, ldl, flags, "lambda");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const bool batchLayout = (flags & RBD_LAYOUT_BATCH) != 0;
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* dt = (const S*)st.in(tau, vt.extent * sizeof(S));
+  S* da = (S*)st.out(ddq, va.extent * sizeof(S), ldddq != nv || batchLayout);
+  S* dl = (S*)st.out(lambda, vl.extent * sizeof(S), ldl != nrows || batchLayout);
+  int* di = (int*)st.out(iters, (size_t)B * sizeof(int32_t), false);
+  auto table = [&](const void* h, size_t bytes) {
+    void* d = ctx->stageBuf(st.slot++, bytes);
+    HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return d;
+  };
+  const RigidConstraintDev* dtab = (const RigidConstraintDev*)table(tab.data(), sizeof(RigidConstraintDev) * ncon);
+  const uint8_t* dsup = (const uint8_t*)table(sup.data(), sup.size());
+  // workspace, batch-fastest: U, D (FP64); oMi, v, a of the joints, the WORLD Jacobian, the
+  // constraint rows and gamma, M, nle and one zero (S)
+  const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
+  const size_t nO = (size_t)12 * (nj - 1) * B, nVA = (size_t)6 * (nj - 1) * B, nJw = (size_t)6 * nv * B;
+  const size_t nJc = (size_t)nrows * nv * B, nG = (size_t)nrows * B, nM = (size_t)nv * nv * B, nN = (size_t)nv * B;
+  double* wU = (double*)ctx->workspace((nU + nD) * 8 + (nO + 2 * nVA + nJw + nJc + nG + nM + nN + 1) * sizeof(S));
+  double* wD = wU + nU;
+  S* wO = (S*)(wD + nD);
+  S* wV = wO + nO;
+  S* wA = wV + nVA;
+  S* wJw = wA + nVA;
+  S* wJc = wJw + nJw;
+  S* wG = wJc + nJc;
+  S* wM = wG + nG;
+  S* wN = wM + nM;
+  S* wZ = wN + nN;
+  const S* mc = f32 ? (const S*)m->mc32 : (const S*)m->mc64;
+  const TopoOps& o = *m->ops;
+  const CView<S> cq{dq, vq.rs, vq.bs}, cv{dv, vv.rs, vv.bs}, zero{wZ, 0, 0};
+  hipLaunchKernelGGL(k_fill_zero<S>, dim3(1), dim3(64), 0, ctx->stream, wZ, (int64_t)1);
+  const auto kin = f32 ? (LaunchKin<S>)o.kin_f32 : (LaunchKin<S>)o.kin_f64;
+  kin(ctx->stream, m->gen.get(), mc, B, cq, cv, zero, View<S>{wO, B, 1}, View<S>{wV, B, 1}, View<S>{wA, B, 1},
+      View<S>{wJw, B, 1});
+  hipLaunchKernelGGL(k_contact_terms<S>, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, B, nv, ncon,
+                     nrows, dtab, dsup, CView<S>{wO, B, 1}, CView<S>{wV, B, 1}, CView<S>{wA, B, 1},
+                     CView<S>{wJw, B, 1}, View<S>{wJc, B, 1}, View<S>{wG, B, 1});
+  if constexpr (f32) {
+    o.crba_f32(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f32(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    o.rnea_f32(ctx->stream, m->gen.get(), mc, B, cq, cv, zero, View<S>{wN, B, 1});
+  } else {
+    o.crba_f64(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f64(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+    o.rnea_f64(ctx->stream, m->gen.get(), mc, B, cq, cv, zero, View<S>{wN, B, 1});
+  }
+  const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
+  kkt(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nrows, CView<S>{wJc, B, 1},
+      CView<S>{dt, vt.rs, vt.bs}, CView<S>{wN, B, 1}, CView<S>{wG, B, 1}, ps.mu, 0, 0.0,
+      KktProx{ps.max_iter, ps.absolute_accuracy, ps.relative_accuracy, di}, View<S>{da, va.rs, va.bs},
+      View<S>{dl, vl.rs, vl.bs});
   finishCall(ctx, flags, st);
 }
 }  // namespace
@@ -1018,6 +1126,38 @@ int rbd_impulse_dynamics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype,
     else
       impulseCall<double>(ctx, m, B, q, ldq, v_before, ldv, nc, J, ldJ, r_coeff, inv_damping, dq_after, ldda,
                           impulse, ldimpulse, flags);
+  });
+}
+
+int rbd_constraint_dynamics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                            const void* v, int64_t ldv, const void* tau, int64_t ldtau, int ncontacts,
+                            const rbd_rigid_constraint* contacts, const rbd_proximal_settings* settings, void* ddq,
+                            int64_t ldddq, void* lambda, int64_t ldlambda, int32_t* iterations, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(ncontacts >= 1 && ncontacts <= kKktMaxConstraints, "between 1 and 12 constraints");
+    RBD_CHECK_ARG(contacts != nullptr, "null constraint table");
+    for (int c = 0; c < ncontacts; ++c) {
+      const rbd_rigid_constraint& k = contacts[c];
+      RBD_CHECK_ARG(k.type == RBD_CONTACT_3D || k.type == RBD_CONTACT_6D, "constraint type must be RBD_CONTACT_3D or RBD_CONTACT_6D");
+      RBD_CHECK_ARG(k.joint1_id >= 0 && k.joint1_id < m->njoints && k.joint2_id >= 0 && k.joint2_id < m->njoints,
+                    "The index of the Joint is outside the bounds.");
+      RBD_CHECK_ARG(k.reference_frame == RBD_LOCAL || k.reference_frame == RBD_LOCAL_WORLD_ALIGNED,
+                    "constraintDynamics is only operating from LOCAL or LOCAL_WORLD_ALIGNED reference frames.");
+    }
+    rbd_proximal_settings ps{1e-12, 1e-12, 0.0, 1};
+    if (settings) ps = *settings;
+    RBD_CHECK_ARG(ps.mu >= 0.0, "mu has to be positive");
+    RBD_CHECK_ARG(ps.max_iter >= 1, "max_iter must be at least 1");
+    RBD_CHECK_ARG(B == 0 || (q && v && tau && ddq && lambda), "null array argument");
+    if (!m->compact || !m->ops->kkt_f32 || !m->ops->crba_f32 || !m->ops->kin_f32)
+      throw Error(RBD_EUNSUPPORTED, "constraint dynamics: no kernels for this topology");
+    if (dtype == RBD_F32)
+      cdCall<float>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ncontacts, contacts, ps, ddq, ldddq, lambda, ldlambda,
+                    iterations, flags);
+    else
+      cdCall<double>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ncontacts, contacts, ps, ddq, ldddq, lambda, ldlambda,
+                     iterations, flags);
   });
 }
 

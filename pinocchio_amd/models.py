@@ -383,3 +383,31 @@ def sample_inputs(model: ModelDesc, batch: int, seed: int, tau_sigma: float = 1.
     a = rng.normal((model.nv, batch))
     tau = rng.normal((model.nv, batch), sigma=tau_sigma)
     return q, v, a, tau
+
+
+def perturb_configurations(model: ModelDesc, q0, batch: int, sigma: float, seed: int):
+    """nq x B configurations scattered around q0: revolute angles q0 + sigma N(0, 1),
+    free-flyer translation q0 + sigma N(0, 1) and orientation q0 rotated by a rotation
+    vector sigma N(0, 1) (unit quaternions, [x, y, z, w]).  For workloads whose constraints
+    hold near one reference posture (contacts, loop closures)."""
+    rng = Stream(seed)
+    q0 = np.asarray(q0, dtype=np.float64).reshape(model.nq)
+    q = np.repeat(q0[:, None], batch, axis=1)
+    iq = model.idx_q
+    for j in range(1, model.njoints):
+        if model.jtype[j] == JT_FREEFLYER:
+            q[iq[j]:iq[j] + 3] += sigma * rng.normal((3, batch))
+            w = sigma * rng.normal((3, batch))
+            ang = np.linalg.norm(w, axis=0)
+            axis = w / np.maximum(ang, 1e-300)
+            dq = np.concatenate([axis * np.sin(ang / 2), np.cos(ang / 2)[None]], axis=0)  # [x, y, z, w]
+            x0, y0, z0, w0 = q0[iq[j] + 3:iq[j] + 7]
+            x1, y1, z1, w1 = dq
+            q[iq[j] + 3:iq[j] + 7] = np.stack([w0 * x1 + x0 * w1 + y0 * z1 - z0 * y1,
+                                              w0 * y1 - x0 * z1 + y0 * w1 + z0 * x1,
+                                              w0 * z1 + x0 * y1 - y0 * x1 + z0 * w1,
+                                              w0 * w1 - x0 * x1 - y0 * y1 - z0 * z1])
+        else:
+            q[iq[j]] += sigma * rng.normal((batch,))
+    return q
+

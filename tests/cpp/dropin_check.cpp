@@ -106,6 +106,28 @@ int main(int argc, char** argv) {
                                              MMat{lam.data(), nc, B, nc});
     writeRaw(dir + "/out_cddq.f64", cddq);
     writeRaw(dir + "/out_lambda.f64", lam);
+    // constraintDynamics: a point of joint nj - 1 held on a world point (3-D, LOCAL,
+    // Baumgarte gains) and 10 proximal iterations with mu = 1e-3
+    rbd_rigid_constraint rc{};
+    rc.type = RBD_CONTACT_3D;
+    rc.joint1_id = nj - 1;
+    rc.joint2_id = 0;
+    rc.reference_frame = RBD_LOCAL;
+    for (int k = 0; k < 12; ++k) rc.joint1_placement[k] = rc.joint2_placement[k] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+    rc.joint2_placement[11] = 0.5;
+    for (int k = 0; k < 6; ++k) {
+      rc.Kp[k] = 4.0;
+      rc.Kd[k] = 1.5;
+    }
+    const rbd_proximal_settings ps{1e-10, 1e-10, 1e-3, 10};
+    std::vector<double> kddq((size_t)nv * B), klam((size_t)3 * B);
+    std::vector<int32_t> kit((size_t)B);
+    pinocchio_amd::constraintDynamicsInParallel(threads, pool, Q, V, T, {rc}, ps, MMat{kddq.data(), nv, B, nv},
+                                                MMat{klam.data(), 3, B, 3}, kit.data());
+    writeRaw(dir + "/out_kddq.f64", kddq);
+    writeRaw(dir + "/out_klambda.f64", klam);
+    std::vector<double> kitd(kit.begin(), kit.end());
+    writeRaw(dir + "/out_kiters.f64", kitd);
   }
   // the reference's argument errors (algorithm/parallel/rnea.hpp:53-68, aba.hpp:54-69)
   const bool e1 = throwsInvalid([&] { pinocchio_amd::rneaInParallel(threads + 1, pool, Q, V, A, MMat{tau_out.data(), nv, B, nv}); });

@@ -1,8 +1,10 @@
 """The C++ drop-in (include/pinocchio_amd/parallel.hpp) driven like a pinocchio caller:
 tests/cpp/dropin_check builds a ModelPool from a flattened model, runs rneaInParallel,
 abaInParallel, crbaInParallel, computeRNEADerivativesInParallel,
-computeABADerivativesInParallel, computeMinverseInParallel and forwardDynamicsInParallel
-(constrained) on column-major host matrices, and checks the reference's argument errors.
+computeABADerivativesInParallel, computeMinverseInParallel, forwardDynamicsInParallel
+(constrained) and constraintDynamicsInParallel (a point constraint with Baumgarte gains and
+proximal iterations) on column-major host matrices, and checks the reference's argument
+errors.
 Outputs against the FP64 oracle (the calls default to FP64, like the reference): RNEA,
 ABA and CRBA 1e-12, RNEA and ABA derivatives and Minv 1e-10 (also with outputs whose
 leading dimension exceeds nv), constrained ddq and lambda 1e-8 with a random J; with the
@@ -23,6 +25,10 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "dropin_check")
+
+
+def rd_cols(d, name, rows, B):
+    return np.fromfile(os.path.join(d, f"out_{name}.f64")).reshape(B, rows).T
 
 
 def blocks(x, nv, B):
@@ -83,3 +89,13 @@ def test_cpp_dropin(tmp_path, name):
     cddq = np.fromfile(os.path.join(d, "out_cddq.f64")).reshape(B, nv).T
     clam = np.fromfile(os.path.join(d, "out_lambda.f64")).reshape(B, nc).T
     assert rel_err_cols(cddq, ca).max() < 1e-8 and rel_err_cols(clam, cl).max() < 1e-8
+    # constraintDynamics: a point of the last joint held on a world point, 3-D LOCAL, gains,
+    # 10 proximal iterations
+    eye = [1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]
+    con = [dict(type=oracle.CONTACT_3D, joint1=desc.njoints - 1, reference_frame=oracle.LOCAL, placement1=eye,
+                placement2=eye[:11] + [0.5], Kp=[4.0] * 3, Kd=[1.5] * 3)]
+    kd, kl, kit = oracle.constraint_dynamics(desc, q, v, tau, con, mu=1e-3, max_iter=10, absolute_accuracy=1e-10,
+                                             relative_accuracy=1e-10)
+    assert rel_err_cols(rd_cols(d, "kddq", nv, B), kd).max() < 1e-8
+    assert rel_err_cols(rd_cols(d, "klambda", 3, B), kl).max() < 1e-8
+    assert np.array_equal(np.fromfile(os.path.join(d, "out_kiters.f64")).astype(np.int32), kit)
