@@ -77,6 +77,20 @@ def host_info():
     return info
 
 
+MIX_SPEC = [("arm7", 40000), ("quadruped18", 40000), ("humanoid36", 15000), ("random60", 5000)]
+
+
+def mixed_batch(torch, pa, M, ctx, dev, autotune=True):
+    """The 100,000-instance four-robot mix (RNEA + ABA per instance, FP32, seeded inputs)."""
+    groups = []
+    for j, (name, n) in enumerate(MIX_SPEC):
+        dsc = M.SAMPLE_MODELS[name]()
+        mdl = pa.Model(ctx, dsc)
+        xs = M.sample_inputs(dsc, n, M.INPUT_SEED_BASE + 1 + j)
+        groups.append((mdl, *(torch.tensor(x.astype(np.float32), device=dev) for x in xs)))
+    return pa.MixedBatch(ctx, groups, autotune=autotune), sum(n for _, n in MIX_SPEC)
+
+
 class Timer:
     """HIP-event timing on the engine's stream (torch events on an external stream)."""
 
@@ -166,7 +180,7 @@ def main():
     ap.add_argument("--sur-samples", type=int, default=1 << 22)
     ap.add_argument("--sur-epochs", type=int, default=8)
     ap.add_argument("--only", default=None,
-                    help="one part: step|aba|crba|chol|rnea|rnead|abad|cfd|mlp|surrogate|copy")
+                    help="one part: step|aba|crba|chol|rnea|rnead|abad|cfd|mix|mlp|surrogate|copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; the batch shards with no "
@@ -212,6 +226,23 @@ def main():
 
     if args.only == "surrogate":
         print(json.dumps({"only": "surrogate", **surrogate_bench(torch, pa, M, ctx, args)}))
+        return
+
+    if args.only == "mix":
+        # the mixed-robot batch twice: the autotuned plan and the one-launch plan with
+        # default tile widths (for kernel traces and counters of both)
+        out = {}
+        for tuned in (True, False):
+            mb, n_mix = mixed_batch(torch, pa, M, ctx, dev, autotune=tuned)
+            mb.run()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                mb.run(async_=True)
+            ctx.synchronize()
+            t = (time.perf_counter() - t0) / args.steps
+            out["tuned" if tuned else "fused_width1"] = {"ms": t * 1e3, "chosen": mb.tuning().get("chosen")}
+        print(json.dumps({"only": "mix", **out}))
         return
 
     desc = M.humanoid36()
@@ -466,19 +497,11 @@ def main():
                                             "note": "abaInParallel on float64 numpy arrays: conversion, PCIe both ways"}
 
     # mixed-robot batch: RNEA + ABA per instance, one graph-replayed launch
-    mix_spec = [("arm7", 40000), ("quadruped18", 40000), ("humanoid36", 15000), ("random60", 5000)]
-    groups = []
-    for j, (name, n) in enumerate(mix_spec):
-        dsc = M.SAMPLE_MODELS[name]()
-        mdl = pa.Model(ctx, dsc)
-        xs = M.sample_inputs(dsc, n, M.INPUT_SEED_BASE + 1 + j)
-        groups.append((mdl, *(torch.tensor(x.astype(np.float32), device=dev) for x in xs)))
-    mb = pa.MixedBatch(ctx, groups)
+    mb, n_mix = mixed_batch(torch, pa, M, ctx, dev)
     mb.run()
     tune = mb.tuning()
     t = time_call(lambda: mb.run(async_=True), reps=10)
-    n_mix = sum(n for _, n in mix_spec)
-    others["mixed_rnea_aba_f32"] = {"instances": n_mix, "groups": dict(mix_spec), "configs_per_s": n_mix / t,
+    others["mixed_rnea_aba_f32"] = {"instances": n_mix, "groups": dict(MIX_SPEC), "configs_per_s": n_mix / t,
                                     "ms": t * 1e3, "chosen": tune.get("chosen"),
                                     "candidates_ms": {c["variant"]: c["ms"] for c in tune.get("candidates", [])}}
     # run-time-topology kernels (the tree read as data; what every tree without prebuilt

@@ -260,6 +260,8 @@ int rbd_ctx_destroy(rbd_ctx* ctx) {
     for (void* p : ctx->stage)
       if (p) (void)hipFree(p);
     if (ctx->work) (void)hipFree(ctx->work);
+    for (hipStream_t s : ctx->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : ctx->forkJoin) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });
@@ -1193,6 +1195,7 @@ struct MultiPlan {
   std::vector<const rbd_model*> models;
   const MixOps* mix = nullptr;     // null: per-model launches only
   bool fused = true;
+  bool concurrent = false;         // per-model launches: one stream per distinct model
   std::vector<int> width;          // chunks of 64 instances per workgroup, per group
   MultiGroup* dGroups = nullptr;
   MultiTile* dTiles = nullptr;
@@ -1298,15 +1301,38 @@ void enqueue(rbd_ctx* ctx, MultiPlan& p, const rbd_batch_group* g, int ng) {
     if (p.ntiles > 0) p.mix->launch(ctx->stream, (unsigned)p.ntiles, p.dGroups, p.dTiles);
     return;
   }
+  // per-model launches; concurrent: the groups of each distinct model on a side stream of
+  // their own, forked from and joined back into the context's stream with events (a
+  // small group's kernels fill few CUs, so the groups' kernels overlap; groups of one
+  // model share its run-time-topology lane memory and stay in order on one stream)
+  std::vector<const rbd_model*> distinct;
+  auto streamOf = [&](const rbd_model* m) {
+    if (!p.concurrent) return ctx->stream;
+    int k = 0;
+    while (k < (int)distinct.size() && distinct[k] != m) ++k;
+    if (k == (int)distinct.size()) distinct.push_back(m);
+    return ctx->sideStream(k);
+  };
+  if (p.concurrent) {
+    HIP_CHECK(hipEventRecord(ctx->event(0), ctx->stream));
+    for (int k = 0; k < ng; ++k) (void)streamOf(g[k].model);
+    for (size_t s = 0; s < distinct.size(); ++s) HIP_CHECK(hipStreamWaitEvent(ctx->sideStream((int)s), ctx->event(0), 0));
+  }
   for (int k = 0; k < ng; ++k) {
     const rbd_model* m = g[k].model;
     const int64_t n = g[k].count;
     if (n == 0) continue;
-    m->ops->rnea_f32(ctx->stream, m->gen.get(), m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
+    hipStream_t st = streamOf(m);
+    m->ops->rnea_f32(st, m->gen.get(), m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
                      CView<float>{g[k].a, n, 1}, View<float>{g[k].tau_out, n, 1});
-    m->ops->aba_f32(ctx->stream, m->gen.get(), m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
+    m->ops->aba_f32(st, m->gen.get(), m->mc32, n, CView<float>{g[k].q, n, 1}, CView<float>{g[k].v, n, 1},
                     CView<float>{g[k].tau, n, 1}, View<float>{g[k].ddq_out, n, 1});
   }
+  if (p.concurrent)
+    for (size_t s = 0; s < distinct.size(); ++s) {
+      HIP_CHECK(hipEventRecord(ctx->event(1 + (int)s), ctx->sideStream((int)s)));
+      HIP_CHECK(hipStreamWaitEvent(ctx->stream, ctx->event(1 + (int)s), 0));
+    }
 }
 
 void uploadTiles(MultiPlan& p, const rbd_batch_group* g, int ng) {
@@ -1386,19 +1412,28 @@ void autotune(rbd_ctx* ctx, MultiPlan& p, const rbd_batch_group* g, int ng, bool
       }
     }
   }
+  bool bestConcurrent = false;
   if (tune || !p.mix) {
     p.fused = false;
-    const float ms = tune ? timeMs(ctx, p, g, ng, 3) : 0.f;
-    if (tune) note("per-model launches", ms);
-    if (!p.mix || ms < best) {
-      best = ms;
-      bestFused = false;
+    for (bool conc : {false, true}) {
+      if (!tune && conc) break;
+      p.concurrent = conc;
+      const float ms = tune ? timeMs(ctx, p, g, ng, 3) : 0.f;
+      if (tune) note(conc ? "concurrent per-model launches" : "per-model launches", ms);
+      if ((!p.mix && !conc) || ms < best) {
+        best = ms;
+        bestFused = false;
+        bestConcurrent = conc;
+      }
     }
   }
   p.fused = bestFused;
+  p.concurrent = !bestFused && bestConcurrent;
   p.width = bestW;
   if (p.fused) uploadTiles(p, g, ng);
-  rep += "], \"chosen\": \"" + std::string(p.fused ? "fused " + widthStr(p.width) : "per-model launches") +
+  const std::string chosen = p.fused ? "fused " + widthStr(p.width)
+                             : p.concurrent ? "concurrent per-model launches" : "per-model launches";
+  rep += "], \"chosen\": \"" + chosen +
          "\", \"mix\": \"" + (p.mix ? p.mix->name : "none") + "\", \"tiles\": " + std::to_string(p.ntiles) + "}";
   p.report = rep;
 }

@@ -66,6 +66,26 @@ struct rbd_ctx {
     }
     return work;
   }
+  // side streams and fork / join events for concurrent launches (default priority,
+  // non-blocking), created on first use
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> forkJoin;
+  hipStream_t sideStream(int k) {
+    while ((int)side.size() <= k) {
+      hipStream_t s = nullptr;
+      HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      side.push_back(s);
+    }
+    return side[k];
+  }
+  hipEvent_t event(int k) {
+    while ((int)forkJoin.size() <= k) {
+      hipEvent_t e = nullptr;
+      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      forkJoin.push_back(e);
+    }
+    return forkJoin[k];
+  }
   void* stageBuf(int slot, size_t bytes) {
     if ((int)stage.size() <= slot) {
       stage.resize(slot + 1, nullptr);
