@@ -415,6 +415,14 @@ def main():
     Bd = args.deriv_batch
     Qd, Vd, Ad, Td = (x[:, :Bd].contiguous() for x in (Q, V, A, T))
     others = {}
+    # the headline step recorded once (rbd_capture_begin / _end) and replayed as one HIP
+    # graph per step (launch gaps between the three kernels shrink)
+    with ctx.capture() as cap:
+        step()
+    t = time_call(lambda: cap.graph.launch(async_=True), reps=args.steps)
+    cap.graph.close()
+    others["step_graph_replay"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3,
+                                   "note": "ABA + CRBA + FP64 Cholesky captured once, replayed per step"}
     t = time_call(lambda: eng.rnea(Q, V, A, async_=True))
     others["rnea_humanoid36_f32"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3}
     t = time_call(lambda: eng.aba(Q, V, T, out=ddq, async_=True))

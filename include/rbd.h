@@ -232,6 +232,22 @@ int rbd_constrained_forward_dynamics(rbd_ctx* ctx, const rbd_model* model, int64
                                      double inv_damping, void* ddq, int64_t ldddq, void* lambda, int64_t ldlambda,
                                      uint32_t flags);
 
+/* Stream capture of engine calls (no counterpart in the reference; for control loops and
+ * serving paths that replay one fixed sequence of calls): between rbd_capture_begin and
+ * rbd_capture_end the calls made on `ctx` are recorded into a HIP graph instead of run;
+ * rbd_graph_launch replays the graph on the context's stream, reading the recorded input
+ * buffers again at every replay.  During a capture, calls must take device pointers and
+ * must not need to allocate: run each call once, uncaptured, with the same sizes first
+ * (that reserves the workspace).  Host arrays, frame and constraint tables, mixed batches,
+ * the surrogate, model creation and destruction and rbd_ctx_synchronize are refused during
+ * a capture (RBD_EINVAL).  A graph holds device addresses: destroy it before the buffers
+ * and models it uses. */
+typedef struct rbd_graph rbd_graph;
+int rbd_capture_begin(rbd_ctx* ctx);
+int rbd_capture_end(rbd_ctx* ctx, rbd_graph** graph);
+int rbd_graph_launch(rbd_ctx* ctx, const rbd_graph* graph, uint32_t flags); /* RBD_ASYNC or 0 */
+int rbd_graph_destroy(rbd_graph* graph);
+
 /* Rigid constraints of constraintDynamics: RigidConstraintModel (contact-info.hpp:95-420)
  * with the RigidConstraintData field constraintDynamics reads (contact_acceleration_desired,
  * zero by default).  A CONTACT_3D constraint holds the point of joint1's frame on the point

@@ -74,10 +74,60 @@ class Context:
     def synchronize(self):
         _check(lib().rbd_ctx_synchronize(self.handle))
 
+    def capture(self):
+        """Record the engine calls made on this context inside a ``with`` block into a HIP
+        graph (rbd_capture_begin / _end) instead of running them::
+
+            with ctx.capture() as cap:
+                eng.aba(Q, V, T, out=ddq, async_=True)
+            cap.graph.launch()        # replays; the inputs are read again each time
+
+        Calls inside the block take preallocated device outputs (``out=``) and must have run
+        once, uncaptured, with the same sizes."""
+        return _Capture(self)
+
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
             _lib.rbd_ctx_destroy(self.handle)
             self.handle = None
+
+
+class Graph:
+    """A captured sequence of engine calls (rbd_graph)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx, self.handle = ctx, handle
+
+    def launch(self, async_=False):
+        _check(lib().rbd_graph_launch(self.ctx.handle, self.handle, ctypes.c_uint32(RBD_ASYNC if async_ else 0)))
+
+    def close(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.rbd_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class _Capture:
+    def __init__(self, ctx: Context):
+        self.ctx, self.graph = ctx, None
+
+    def __enter__(self):
+        _check(lib().rbd_capture_begin(self.ctx.handle))
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = ctypes.c_void_p()
+        rc = lib().rbd_capture_end(self.ctx.handle, ctypes.byref(h))
+        if exc_type is not None:  # the block failed: drop whatever was recorded
+            if rc == 0:
+                Graph(self.ctx, h).close()
+            return False
+        _check(rc)
+        self.graph = Graph(self.ctx, h)
+        return False
 
 
 RBD_MODEL_RUNTIME_TOPOLOGY = 0x1
@@ -273,15 +323,16 @@ class Engine:
                                           ctypes.c_uint32(self._flags(async_))))
         return tuple(outs)
 
-    def aba_derivatives(self, q, v, tau, async_=False):
-        """(ddq_dq, ddq_dv, Minv, ddq) per configuration (computeABADerivatives)."""
+    def aba_derivatives(self, q, v, tau, out=None, ddq_out=None, async_=False):
+        """(ddq_dq, ddq_dv, Minv, ddq) per configuration (computeABADerivatives); `out`: three
+        preallocated matrix outputs, `ddq_out`: a preallocated ddq (else allocated)."""
         m = self.model
         B, ldq = self._vec(q, m.nq, "q")
         _, ldv = self._vec(v, m.nv, "v")
         _, ldt = self._vec(tau, m.nv, "tau")
-        outs = [self._new_mat(q, B) for _ in range(3)]
+        outs = [self._new_mat(q, B) for _ in range(3)] if out is None else list(out)
         ldo = self._mat_arg(outs[0], B, "out")
-        ddq = self._new_vec(q, m.nv, B)
+        ddq = self._new_vec(q, m.nv, B) if ddq_out is None else ddq_out
         _, ldd = self._vec(ddq, m.nv, "ddq")
         _check(lib().rbd_aba_derivatives(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v),
                                          _i64(ldv), _ptr(tau), _i64(ldt), *[_ptr(o) for o in outs], _i64(ldo),

@@ -30,6 +30,10 @@ template<class TP, int G> constexpr bool abaSplitPays() {
   for (int k = 0; k < G; ++k) busiest = WS.load[k] > busiest ? WS.load[k] : busiest;
   return 5 * (trunk + busiest) <= 3 * (TP::NJ - 1) && abaWavesLdsBytes<TP, G>() <= 160 * 1024;
 }
+// ... and its LDS (sized for 4-byte scalars) fits at the scalar's width
+template<class TP, class S, int G> constexpr bool abaSplitPaysFor() {
+  return abaSplitPays<TP, G>() && abaWavesLdsBytes<TP, G>() / 4 * (int)sizeof(S) <= 160 * 1024;
+}
 // RNEA derivatives: the same test, with the LDS of the staged inputs and the 43-float
 // subtree slots
 template<class TP, int G> constexpr bool rneadSplitPays() {
@@ -54,28 +58,37 @@ inline int residentWaves(const void* kernel, int threads) {
 // two waves per SIMD; ABA at 65,536 configurations 67.7 -> 61.7 us).
 template<class TP, class S>
 void launchAba(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
-  bool denser = false;
-  if constexpr (abaSplitPays<TP, 4>()) {
-    static const bool d = [] {
-      const int one = residentWaves((const void*)&k_aba<TP, S>, kTile);
-      return one > 0 && residentWaves((const void*)&k_aba_waves<TP, S, 4>, kTile * 4) >= 2 * one;
-    }();
-    denser = d;
-  }
-  const int waves = denser ? 4 : wavesForBatch(B);
-  if constexpr (abaSplitPays<TP, 4>()) {
-    if (waves >= 4) {
+  if constexpr (std::is_same_v<S, double>) {
+    // FP64: the one-wave kernel spills its per-joint state (twice the registers of FP32);
+    // the four-wave split holds it, where the tree splits
+    if constexpr (abaSplitPaysFor<TP, S, 4>())
       hipLaunchKernelGGL((k_aba_waves<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, tau, out);
-      return;
+    else
+      hipLaunchKernelGGL((k_aba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, tau, out);
+  } else {
+    bool denser = false;
+    if constexpr (abaSplitPays<TP, 4>()) {
+      static const bool d = [] {
+        const int one = residentWaves((const void*)&k_aba<TP, S>, kTile);
+        return one > 0 && residentWaves((const void*)&k_aba_waves<TP, S, 4>, kTile * 4) >= 2 * one;
+      }();
+      denser = d;
     }
-  }
-  if constexpr (abaSplitPays<TP, 2>()) {
-    if (waves >= 2) {
-      hipLaunchKernelGGL((k_aba_waves<TP, S, 2>), dim3(gridFor(B)), dim3(kTile * 2), 0, st, mc, B, q, v, tau, out);
-      return;
+    const int waves = denser ? 4 : wavesForBatch(B);
+    if constexpr (abaSplitPays<TP, 4>()) {
+      if (waves >= 4) {
+        hipLaunchKernelGGL((k_aba_waves<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, tau, out);
+        return;
+      }
     }
+    if constexpr (abaSplitPays<TP, 2>()) {
+      if (waves >= 2) {
+        hipLaunchKernelGGL((k_aba_waves<TP, S, 2>), dim3(gridFor(B)), dim3(kTile * 2), 0, st, mc, B, q, v, tau, out);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_aba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, tau, out);
   }
-  hipLaunchKernelGGL((k_aba<TP, S>), dim3(gridFor(B)), dim3(kTile), 0, st, mc, B, q, v, tau, out);
 }
 // CRBA: two waves per tile (output emission split) keep two waves resident per SIMD at
 // large batches
@@ -159,6 +172,7 @@ template<class TP> void add_dyn(TopoOps& o) {
   o.rnea_f32 = &launchRnea<TP, float>;
   o.rnea_f64 = &launchRnea<TP, double>;
   o.aba_f32 = &launchAba<TP, float>;
+  o.aba_f64 = &launchAba<TP, double>;
 }
 template<class TP> void add_crba(TopoOps& o) {
   o.crba_f32 = &launchCrba<TP, float>;

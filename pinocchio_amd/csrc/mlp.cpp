@@ -99,7 +99,10 @@ struct rbd_mlp {
 
 namespace {
 
-void setDev(rbd_ctx* ctx) { HIP_CHECK(hipSetDevice(ctx->device)); }
+void setDev(rbd_ctx* ctx) {
+  RBD_CHECK_ARG(!ctx->capturing, "capture: the surrogate's calls cannot be recorded");
+  HIP_CHECK(hipSetDevice(ctx->device));
+}
 
 template<int BM, int BN, bool AK, bool BK, int EPI> void gemm(hipStream_t st, const GemmArgs& g, int splits = 1) {
   if (g.M % BM || g.N % BN || g.K % kStage || g.kChunk % kStage || (int64_t)g.kChunk * splits < g.K)

@@ -184,6 +184,7 @@ struct Staged {
 void finishCall(rbd_ctx* ctx, uint32_t flags, Staged& st) {
   HIP_CHECK(hipGetLastError());
   st.finish();
+  if (ctx->capturing) return;  // recorded into the graph being captured, not run
   if (!(flags & RBD_ASYNC) || (flags & RBD_HOST_PTRS)) HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -260,6 +261,7 @@ int rbd_ctx_destroy(rbd_ctx* ctx) {
     for (void* p : ctx->stage)
       if (p) (void)hipFree(p);
     if (ctx->work) (void)hipFree(ctx->work);
+    for (void* p : ctx->retired) (void)hipFree(p);
     for (hipStream_t s : ctx->side) (void)hipStreamDestroy(s);
     for (hipEvent_t e : ctx->forkJoin) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -272,7 +274,76 @@ void* rbd_ctx_stream(rbd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; 
 int rbd_ctx_synchronize(rbd_ctx* ctx) {
   return guarded([&] {
     RBD_CHECK_ARG(ctx, "null context");
+    RBD_CHECK_ARG(!ctx->capturing, "capture in progress: nothing runs until rbd_graph_launch");
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// ---- stream capture of engine calls --------------------------------------------------
+struct rbd_graph {
+  rbd_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int rbd_capture_begin(rbd_ctx* ctx) {
+  return guarded([&] {
+    RBD_CHECK_ARG(ctx, "null context");
+    RBD_CHECK_ARG(!ctx->capturing, "a capture is already in progress on this context");
+    HIP_CHECK(hipSetDevice(ctx->device));
+    HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+  });
+}
+
+int rbd_capture_end(rbd_ctx* ctx, rbd_graph** out) {
+  return guarded([&] {
+    RBD_CHECK_ARG(ctx, "null context");
+    RBD_CHECK_ARG(ctx->capturing, "no capture in progress on this context");
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess || !g) {
+      (void)hipGetLastError();
+      if (g) (void)hipGraphDestroy(g);
+      throw Error(RBD_EDEVICE, std::string("capture failed (a call inside it could not be recorded): ") +
+                                   hipGetErrorString(e));
+    }
+    RBD_CHECK_ARG(out != nullptr, "null output pointer");
+    auto* r = new rbd_graph();
+    r->ctx = ctx;
+    r->graph = g;
+    const hipError_t ie = hipGraphInstantiate(&r->exec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      delete r;
+      throw Error(RBD_EDEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+    }
+    ++ctx->graphs;
+    *out = r;
+  });
+}
+
+int rbd_graph_launch(rbd_ctx* ctx, const rbd_graph* graph, uint32_t flags) {
+  return guarded([&] {
+    RBD_CHECK_ARG(ctx && graph, "null argument");
+    RBD_CHECK_ARG(graph->ctx == ctx, "graph captured on another context");
+    RBD_CHECK_ARG(!ctx->capturing, "capture in progress on this context");
+    HIP_CHECK(hipSetDevice(ctx->device));
+    HIP_CHECK(hipGraphLaunch(graph->exec, ctx->stream));
+    if (!(flags & RBD_ASYNC)) HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int rbd_graph_destroy(rbd_graph* graph) {
+  return guarded([&] {
+    if (!graph) return;
+    (void)hipSetDevice(graph->ctx->device);
+    (void)hipStreamSynchronize(graph->ctx->stream);
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    --graph->ctx->graphs;
+    delete graph;
   });
 }
 
@@ -313,6 +384,7 @@ bool isCompact(const std::vector<int>& parents) {
 int createModel(rbd_ctx* ctx, const rbd_model_desc* d, uint32_t mflags, rbd_model** out) {
   return guarded([&] {
     RBD_CHECK_ARG(ctx && d && out, "null argument");
+    RBD_CHECK_ARG(!ctx->capturing, "capture in progress: create models before rbd_capture_begin");
     RBD_CHECK_ARG(d->njoints >= 2, "model needs at least one moving joint");
     RBD_CHECK_ARG(d->parents && d->jtype && d->placement && d->inertia, "null model array");
     const int nj = d->njoints;
@@ -417,6 +489,7 @@ int rbd_model_kernels(const rbd_model* m, int32_t* prebuilt) {
 int rbd_model_destroy(rbd_model* m) {
   return guarded([&] {
     if (!m) return;
+    RBD_CHECK_ARG(!m->ctx->capturing, "capture in progress: destroy the model after rbd_capture_end");
     (void)hipSetDevice(m->ctx->device);
     (void)hipStreamSynchronize(m->ctx->stream);  // kernels in flight may still read the constants
     dropMultiPlans(m->ctx, m);
@@ -1445,6 +1518,7 @@ extern "C" {
 int rbd_multi_rnea_aba(rbd_ctx* ctx, int32_t ngroups, const rbd_batch_group* groups, uint32_t flags) {
   return guarded([&] {
     RBD_CHECK_ARG(ctx != nullptr, "null context");
+    RBD_CHECK_ARG(!ctx->capturing, "capture: mixed batches replay their own graph and cannot be recorded");
     RBD_CHECK_ARG(ngroups >= 1 && ngroups <= 8 && groups, "between 1 and 8 groups are supported");
     std::vector<const TopoOps*> need;
     for (int k = 0; k < ngroups; ++k) {

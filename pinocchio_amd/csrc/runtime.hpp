@@ -57,9 +57,21 @@ struct rbd_ctx {
   // device workspace for intermediate results (ABA-derivative factor, ddq), grown on demand
   void* work = nullptr;
   size_t workBytes = 0;
+  // stream capture (rbd_capture_begin / _end): calls are recorded, not run; graphs that
+  // exist may reference the workspace, so a grown workspace retires the old buffer
+  // instead of freeing it (freed with the context)
+  bool capturing = false;
+  int graphs = 0;
+  std::vector<void*> retired;
   void* workspace(size_t bytes) {
     if (workBytes < bytes) {
-      if (work) HIP_CHECK(hipFree(work));
+      if (capturing)
+        throw Error(RBD_EINVAL, "capture: the call needs more workspace than the calls before the capture "
+                                "reserved (run it once, uncaptured, with the same batch first)");
+      if (work) {
+        if (graphs > 0) retired.push_back(work);
+        else HIP_CHECK(hipFree(work));
+      }
       work = nullptr;
       HIP_CHECK(hipMalloc(&work, bytes));
       workBytes = bytes;
@@ -87,6 +99,9 @@ struct rbd_ctx {
     return forkJoin[k];
   }
   void* stageBuf(int slot, size_t bytes) {
+    if (capturing)
+      throw Error(RBD_EINVAL, "capture: host arrays and host-side tables (frames, constraints) cannot be "
+                              "recorded; use device pointers");
     if ((int)stage.size() <= slot) {
       stage.resize(slot + 1, nullptr);
       stageBytes.resize(slot + 1, 0);
