@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--sur-samples", type=int, default=1 << 22)
     ap.add_argument("--sur-epochs", type=int, default=8)
     ap.add_argument("--only", default=None,
-                    help="one part: step|aba|crba|chol|rnea|rnead|abad|cfd|mix|mlp|surrogate|copy")
+                    help="one part: step|aba|crba|chol|rnea|rnead|abad|abad64|cfd|mix|mlp|surrogate|copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; the batch shards with no "
@@ -329,12 +329,14 @@ def main():
         elapsed = time.perf_counter() - t0
     if only is not None:
         extra = {}
-        if only in ("rnea", "rnead", "abad", "cfd"):
+        if only in ("rnea", "rnead", "abad", "abad64", "cfd"):
             Bd = args.deriv_batch
             Qd, Vd, Ad, Td = Q[:, :Bd].contiguous(), V[:, :Bd].contiguous(), A[:, :Bd].contiguous(), T[:, :Bd].contiguous()
+            Qd64, Vd64, Td64 = (x.double() for x in (Qd, Vd, Td))
             f = {"rnea": lambda: eng.rnea(Q, V, A, async_=True),
                  "rnead": lambda: eng.rnea_derivatives(Qd, Vd, Ad, async_=True),
                  "abad": lambda: eng.aba_derivatives(Qd, Vd, Td, async_=True),
+                 "abad64": lambda: eng.aba_derivatives(Qd64, Vd64, Td64, async_=True),
                  "cfd": lambda: eng.constrained_forward_dynamics(Q, V, T, Jrows, gam0, out=cddq, lambda_out=clam,
                                                                  async_=True)}[only]
             for _ in range(args.warmup):
@@ -479,6 +481,39 @@ def main():
         "batch": B, "constraints": "two 6D feet, Baumgarte Kp 10 Kd 2", "configs_per_s": B / t, "ms": t * 1e3,
         "note": "kinematics, CRBA, RNEA(a=0) in FP32; constraint rows, FP64 tree factor and KKT in FP64"}
 
+    # contact problems: the Delassus matrix of four foot points (two per foot, LOCAL) at the
+    # near-posture configurations, and batched ADMM / PGS on it (FP64, 12 rows, mu 0.7)
+    corners = [np.concatenate([np.eye(3).ravel(), [x, 0.0, -0.08]]) for x in (0.06, -0.04)]
+    pts = [pa.RigidConstraintModel(pa.CONTACT_3D, j, c, reference_frame=pa.LOCAL) for j in (7, 13) for c in corners]
+    Qc64 = Qc.double()
+    Gd = torch.empty((144, B), dtype=torch.float64, device=dev)
+    t = time_call(lambda: eng.delassus(Qc64, pts, out=Gd, async_=True), reps=10)
+    others["delassus_humanoid36_f64"] = {"batch": B, "constraints": "four foot points (12 rows)",
+                                         "configs_per_s": B / t, "ms": t * 1e3}
+    Gp = Gd.T.contiguous()  # (B, 144): element (i, j) at i + 12 j
+    gen = torch.Generator(device=dev).manual_seed(11)
+    gv = torch.randn((B, 12), dtype=torch.float64, device=dev, generator=gen) * 0.05
+    gv[:, 2::3] -= 0.2  # pressing contacts
+    mus = torch.full((B, 4), 0.7, dtype=torch.float64, device=dev)
+    admm = pa.ADMMContactSolver(ctx, 12)
+    admm.setAbsolutePrecision(1e-8)
+    admm.setMaxIterations(500)
+    t = time_call(lambda: admm.solve(Gp, gv, mus, async_=True), reps=3)
+    its = admm.getIterationCount().float()
+    conv = admm.stats[:, 3]
+    others["admm_contact_humanoid36"] = {"problems": B, "cones": 4, "ms": t * 1e3, "problems_per_s": B / t,
+                                         "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+                                         "converged_frac": float(conv.mean()), "absolute_precision": 1e-8}
+    pgs = pa.PGSContactSolver(ctx, 12)
+    pgs.setAbsolutePrecision(1e-8)
+    pgs.setMaxIterations(500)
+    xg = torch.zeros((B, 12), dtype=torch.float64, device=dev)
+    t = time_call(lambda: (xg.zero_(), pgs.solve(Gp, gv, mus, xg, async_=True)), reps=3)
+    its = pgs.getIterationCount().float()
+    others["pgs_contact_humanoid36"] = {"problems": B, "cones": 4, "ms": t * 1e3, "problems_per_s": B / t,
+                                        "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+                                        "converged_frac": float(pgs.stats[:, 3].mean()), "absolute_precision": 1e-8}
+
     idq = torch.empty_like(cddq)
     iimp = torch.empty_like(clam)
     t = time_call(lambda: eng.impulse_dynamics(Q, V, Jrows, r_coeff=0.0, out=idq, impulse_out=iimp, async_=True),
@@ -529,7 +564,8 @@ def main():
         rt[name] = {"batch": bb, "prebuilt_configs_per_s": bb / tp, "runtime_topology_configs_per_s": bb / tr,
                     "runtime_over_prebuilt": tp / tr}
     others["runtime_topology_humanoid36_f32"] = rt
-    # FP64 ABA and ABA derivatives (run-time-topology kernels; the reference's precision)
+    # FP64 ABA (prebuilt four-wave kernel) and ABA derivatives (FP64 RNEA derivatives and
+    # solve on the run-time-topology kernels), the reference's precision
     Q64, V64, T64 = (torch.tensor(x, dtype=torch.float64, device=dev) for x in (q, v, tau))
     t = time_call(lambda: eng.aba(Q64, V64, T64, async_=True), reps=5)
     others["aba_humanoid36_f64"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3}

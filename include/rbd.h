@@ -21,6 +21,9 @@
  *   rbd_frames            updateFramePlacements,    algorithm/frames.hxx:15-35,
  *                         getFrameJacobian          algorithm/frames.hxx:156-175
  *   rbd_constraint_dynamics constraintDynamics      algorithm/constrained-dynamics.hxx:193-500
+ *   rbd_delassus          computeDelassusMatrix     algorithm/delassus.hpp:41-60, delassus.hxx:118-300
+ *   rbd_pgs_solve         PGSContactSolverTpl::solve algorithm/pgs-solver.hpp:16-59, pgs-solver.hxx:19-141
+ *   rbd_admm_solve        ADMMContactSolverTpl::solve algorithm/admm-solver.hpp:120-316, admm-solver.hxx:22-335
  *   rbd_impulse_dynamics  impulseDynamics(q, v_before, J, r_coeff, inv_damping)
  *                                                   algorithm/contact-dynamics.hxx:264-331
  *   rbd_constrained_forward_dynamics
@@ -287,6 +290,64 @@ int rbd_constraint_dynamics(rbd_ctx* ctx, const rbd_model* model, int64_t batch,
                             int64_t ldq, const void* v, int64_t ldv, const void* tau, int64_t ldtau, int ncontacts,
                             const rbd_rigid_constraint* contacts, const rbd_proximal_settings* settings, void* ddq,
                             int64_t ldddq, void* lambda, int64_t ldlambda, int32_t* iterations, uint32_t flags);
+
+/* computeDelassusMatrix(model, data, q, contact_models, contact_datas, G, mu)
+ * (delassus.hpp:41-60, delassus.hxx:118-300) per configuration: the constraints'
+ * operational-space inverse inertia J (M + mu J^T J)^-1 J^T, i.e. J M^-1 J^T for mu = 0 --
+ * the reference adds a contact inertia of mass mu (rotational inertia mu I for 6-D
+ * constraints) at each constraint frame.  J is the constraints' Jacobian as
+ * rbd_constraint_dynamics forms it: the reference's Delassus reads joint1's side only
+ * (world contacts, joint2 = 0) in the LOCAL frame, where the two agree.  G holds nrows^2
+ * rows per configuration (element (i, j) at row i + j nrows), in the input precision;
+ * the factor and the products are FP64.  Prebuilt topologies only. */
+int rbd_delassus(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                 int ncontacts, const rbd_rigid_constraint* contacts, double mu, void* G, int64_t ldG, uint32_t flags);
+
+/* Frictional contact problems over Coulomb friction cones (constraints/coulomb-friction-
+ * cone.hpp:20-290), batched: `batch` independent problems of `ncones` cones (1..4, problem
+ * size n = 3 ncones; each cone's rows are tangent x, tangent y, normal), FP64 only.
+ * Per problem: the Delassus matrix G (n^2 rows, element (i, j) at row i + j n), the free
+ * velocity g (n rows), the friction coefficients mu (ncones rows).  Layouts and flags as
+ * for the other batched arguments (RBD_LAYOUT_BATCH, RBD_HOST_PTRS, RBD_ASYNC).
+ * stats: RBD_NCP_STATS rows per problem -- iterations (the reference's getIterationCount),
+ * absolute residual, relative residual, converged (1 / 0, the solve's return value), then
+ * (ADMM) rho, rho_power, the largest eigenvalue of G and the number of factorizations. */
+enum { RBD_NCP_STATS = 8 };
+
+/* PGSContactSolverTpl (pgs-solver.hpp:16-59, contact-solver-base.hpp:28-33); NULL = the
+ * reference's defaults (max_it 1000, precisions 1e-6, over_relax 1). */
+typedef struct rbd_pgs_settings {
+  int32_t max_it;
+  double absolute_precision, relative_precision, over_relax;
+} rbd_pgs_settings;
+
+/* PGSContactSolverTpl::solve(G, g, cones, x, over_relax) (pgs-solver.hxx:19-141): x (n rows)
+ * holds the initial guess and receives the solution. */
+int rbd_pgs_solve(rbd_ctx* ctx, int64_t batch, int ncones, const double* G, int64_t ldG, const double* g,
+                  int64_t ldg, const double* mu, int64_t ldmu, double* x, int64_t ldx,
+                  const rbd_pgs_settings* settings, double* stats, int64_t ldstats, uint32_t flags);
+
+/* ADMMContactSolverTpl (admm-solver.hpp:120-127) parameters with the solver base's
+ * (contact-solver-base.hpp:28-33); NULL = the reference's defaults (mu_prox 1e-6, tau 0.5,
+ * rho_power 0.2, rho_power_factor 0.05, ratio_primal_dual 10, 20 power iterations,
+ * max_it 1000, precisions 1e-6). */
+typedef struct rbd_admm_settings {
+  double mu_prox, tau, rho_power, rho_power_factor, ratio_primal_dual;
+  int32_t max_it_largest_eigen_value, max_it;
+  double absolute_precision, relative_precision;
+} rbd_admm_settings;
+
+/* ADMMContactSolverTpl::solve(delassus, g, cones, R, primal_guess, dual_guess)
+ * (admm-solver.hxx:22-335) with a fresh solver on a fresh DelassusOperatorDense of G per
+ * problem (the largest eigenvalue by power iteration from the uniform vector): outputs y
+ * (getPrimalSolution), z (getDualSolution) and, if `s` is not NULL, the complementarity
+ * shift (getComplementarityShift), n rows each.  R (n rows) is the compliance;
+ * primal_guess / dual_guess (n rows, ldguess) may be NULL for none. */
+int rbd_admm_solve(rbd_ctx* ctx, int64_t batch, int ncones, const double* G, int64_t ldG, const double* g,
+                   int64_t ldg, const double* mu, int64_t ldmu, const double* R, int64_t ldR,
+                   const double* primal_guess, const double* dual_guess, int64_t ldguess,
+                   const rbd_admm_settings* settings, double* y, double* z, double* s, int64_t ldout, double* stats,
+                   int64_t ldstats, uint32_t flags);
 
 /* impulseDynamics (contact-dynamics.hpp:241-300, contact-dynamics.hxx:264-331): the velocity
  * after an impact, dq_after = v_before + M^-1 J^T impulse with

@@ -352,3 +352,79 @@ def constraint_dynamics(model, q, v, tau, constraints, mu=0.0, max_iter=1, absol
                                          ctypes.c_double(mu), int(max_iter), _p(ddq), _p(lam),
                                          its.ctypes.data_as(ctypes.c_void_p), SCALARS[scalar], nthreads))
     return np.ascontiguousarray(ddq), np.ascontiguousarray(lam), its
+
+
+def delassus(model, q, constraints, mu=0.0, scalar="f64", nthreads=0):
+    """computeDelassusMatrix(model, data, q, contact_models, contact_datas, G, mu) per column
+    (delassus.hxx:118-300): J (M + mu J^T J)^-1 J^T.  Returns (B, nrows, nrows)."""
+    q = _cols(q, model.nq)
+    B = q.shape[1]
+    arr = constraint_table(constraints)
+    nrows = sum(6 if c["type"] == CONTACT_6D else 3 for c in constraints)
+    G = np.zeros(B * nrows * nrows)
+    d = model.cstruct()
+    _check(lib().orc_delassus(ctypes.byref(d), ctypes.c_int64(B), _p(q), len(constraints), arr, ctypes.c_double(mu),
+                              _p(G), SCALARS[scalar], nthreads))
+    return _mats(G, B, nrows)
+
+
+def _problems(G, g, mu, nc):
+    """(B, n, n) row-indexed Delassus matrices, (B, n) vectors, (B, nc) or (nc,) friction
+    coefficients -> column-major contiguous copies."""
+    n = 3 * nc
+    G = np.asarray(G, dtype=np.float64)
+    if G.ndim == 2:
+        G = G[None]
+    B = G.shape[0]
+    assert G.shape == (B, n, n), (G.shape, n)
+    Gc = np.ascontiguousarray(G.transpose(0, 2, 1))
+    g = np.ascontiguousarray(np.broadcast_to(np.asarray(g, dtype=np.float64), (B, n)))
+    mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (B, nc)))
+    return B, Gc, g, mu
+
+
+def pgs_solve(G, g, mu, x0=None, over_relax=1.0, max_it=1000, absolute_precision=1e-6, relative_precision=1e-6,
+              nthreads=0):
+    """PGSContactSolverTpl::solve (pgs-solver.hxx:19-141) on B problems: G (B, n, n), g (B, n),
+    mu (B, nc) or (nc,), x0 (B, n) initial guess (zero if None).  Returns dict x, it,
+    absolute_residual, relative_residual, converged."""
+    nc = np.asarray(mu).shape[-1]
+    B, Gc, g, mu = _problems(G, g, mu, nc)
+    n = 3 * nc
+    x = np.zeros((B, n)) if x0 is None else np.array(np.broadcast_to(x0, (B, n)), dtype=np.float64)
+    it = np.zeros(B, dtype=np.int32)
+    res = np.zeros((B, 2))
+    conv = np.zeros(B, dtype=np.uint8)
+    _check(lib().orc_pgs_solve(ctypes.c_int64(B), nc, _p(Gc), _p(g), _p(mu), _p(x), ctypes.c_double(over_relax),
+                               int(max_it), ctypes.c_double(absolute_precision), ctypes.c_double(relative_precision),
+                               _p(it), _p(res), _p(conv), nthreads))
+    return dict(x=x, it=it, absolute_residual=res[:, 0], relative_residual=res[:, 1], converged=conv.astype(bool))
+
+
+def admm_solve(G, g, mu, R=None, primal_guess=None, dual_guess=None, mu_prox=1e-6, tau=0.5, rho_power=0.2,
+               rho_power_factor=0.05, ratio_primal_dual=10.0, max_it_largest_eigen_value=20, max_it=1000,
+               absolute_precision=1e-6, relative_precision=1e-6, nthreads=0):
+    """ADMMContactSolverTpl::solve (admm-solver.hxx:22-335) on B problems, each with a fresh
+    solver and a fresh dense Delassus operator.  R (B, n) compliance (zero if None).
+    Returns dict y (primal solution), z (dual), s (shift), it, absolute_residual,
+    relative_residual, rho, rho_power, largest_eigen_value, cholesky_update_count,
+    converged."""
+    nc = np.asarray(mu).shape[-1]
+    B, Gc, g, mu = _problems(G, g, mu, nc)
+    n = 3 * nc
+    R = np.zeros((B, n)) if R is None else np.ascontiguousarray(np.broadcast_to(np.asarray(R, float), (B, n)))
+    pg = None if primal_guess is None else np.ascontiguousarray(np.broadcast_to(primal_guess, (B, n)), dtype=float)
+    dg = None if dual_guess is None else np.ascontiguousarray(np.broadcast_to(dual_guess, (B, n)), dtype=float)
+    st = np.array([mu_prox, tau, rho_power, rho_power_factor, ratio_primal_dual, absolute_precision,
+                   relative_precision], dtype=np.float64)
+    y, z, s = (np.zeros((B, n)) for _ in range(3))
+    it = np.zeros(B, dtype=np.int32)
+    stats = np.zeros((B, 5))
+    chol = np.zeros(B, dtype=np.int32)
+    conv = np.zeros(B, dtype=np.uint8)
+    _check(lib().orc_admm_solve(ctypes.c_int64(B), nc, _p(Gc), _p(g), _p(mu), _p(R), _p(pg), _p(dg), _p(st),
+                                int(max_it), int(max_it_largest_eigen_value), _p(y), _p(z), _p(s), _p(it),
+                                _p(stats), _p(chol), _p(conv), nthreads))
+    return dict(y=y, z=z, s=s, it=it, absolute_residual=stats[:, 0], relative_residual=stats[:, 1], rho=stats[:, 2],
+                rho_power=stats[:, 3], largest_eigen_value=stats[:, 4], cholesky_update_count=chol,
+                converged=conv.astype(bool))

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "contact_solvers.hpp"
 #include "rbd_oracle.hpp"
 
 using namespace oracle;
@@ -605,12 +606,9 @@ struct OrcConstraint {
 };
 
 namespace {
-template<class S>
-int cdT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau, int ncon,
-        const OrcConstraint* oc, double absAcc, double relAcc, double mu, int maxIter, double* ddq, double* lambda,
-        int32_t* iters, int nt) {
+template<class S> std::vector<RigidConstraint<S>> toConstraints(int ncon, const OrcConstraint* oc, int& nrows) {
   std::vector<RigidConstraint<S>> cons(ncon);
-  int nrows = 0;
+  nrows = 0;
   for (int c = 0; c < ncon; ++c) {
     RigidConstraint<S>& r = cons[c];
     r.type = oc[c].type;
@@ -633,6 +631,15 @@ int cdT(const ModelDesc* d, int64_t B, const double* q, const double* v, const d
     }
     nrows += r.size();
   }
+  return cons;
+}
+
+template<class S>
+int cdT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* tau, int ncon,
+        const OrcConstraint* oc, double absAcc, double relAcc, double mu, int maxIter, double* ddq, double* lambda,
+        int32_t* iters, int nt) {
+  int nrows = 0;
+  const std::vector<RigidConstraint<S>> cons = toConstraints<S>(ncon, oc, nrows);
   const ProximalSettings<S> ps{S(absAcc), S(relAcc), S(mu), maxIter};
   return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
     for (const auto& c : cons)
@@ -669,6 +676,132 @@ extern "C" int orc_constraint_dynamics(const ModelDesc* d, int64_t B, const doub
 #undef CALL
     g_err = "bad scalar";
     return 1;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// computeDelassusMatrix per configuration: G (nrows x nrows, column major) back to back
+namespace {
+template<class S>
+int delT(const ModelDesc* d, int64_t B, const double* q, int ncon, const OrcConstraint* oc, double mu, double* G,
+         int nt) {
+  int nrows = 0;
+  const std::vector<RigidConstraint<S>> cons = toConstraints<S>(ncon, oc, nrows);
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    for (const auto& c : cons)
+      if (c.joint1 < 0 || c.joint1 >= m.njoints || c.joint2 < 0 || c.joint2 >= m.njoints)
+        throw std::invalid_argument("constraint joint out of range");
+    static thread_local Buf<S> bq;
+    static thread_local std::vector<S> g;
+    g.resize((size_t)nrows * nrows);
+    delassusMatrix(m, data, bq.load(q + b * m.nq, m.nq), ncon, cons.data(), S(mu), g.data());
+    store(g.data(), G + b * nrows * nrows, nrows * nrows);
+  });
+}
+}  // namespace
+
+extern "C" int orc_delassus(const ModelDesc* d, int64_t B, const double* q, int ncon, const OrcConstraint* cons,
+                            double mu, double* G, int scalar, int nthreads) {
+  try {
+    if (ncon < 1 || !cons) throw std::invalid_argument("at least one constraint");
+    switch (scalar) {
+      case 0: return delT<double>(d, B, q, ncon, cons, mu, G, nthreads);
+      case 1: return delT<float>(d, B, q, ncon, cons, mu, G, nthreads);
+      case 2: return delT<long double>(d, B, q, ncon, cons, mu, G, nthreads);
+    }
+    g_err = "bad scalar";
+    return 1;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Frictional contact solvers over B independent problems of nc cones (n = 3 nc), each
+// problem's arrays back to back: G n x n column major, g and x / y / z / s n, mu nc, R n.
+// PGS: x in/out; it, residuals (absolute, relative) and converged per problem.
+extern "C" int orc_pgs_solve(int64_t B, int nc, const double* G, const double* g, const double* mu, double* x,
+                             double over_relax, int max_it, double absolute_precision, double relative_precision,
+                             int32_t* it, double* residuals, uint8_t* converged, int nthreads) {
+  try {
+    if (nc < 1) throw std::invalid_argument("at least one cone");
+    if (max_it < 1) throw std::invalid_argument("max_it should be greater than 0.");
+    const int n = 3 * nc;
+    std::string err;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (int64_t b = 0; b < B; ++b) {
+      try {
+        const auto r = orc::pgsSolve<double>(nc, G + b * n * n, g + b * n, mu + b * nc, x + b * n, over_relax, max_it,
+                                             absolute_precision, relative_precision);
+        if (it) it[b] = r.it;
+        if (residuals) {
+          residuals[2 * b] = r.absolute_residual;
+          residuals[2 * b + 1] = r.relative_residual;
+        }
+        if (converged) converged[b] = r.converged ? 1 : 0;
+      } catch (const std::exception& e) {
+#pragma omp critical
+        err = e.what();
+      }
+    }
+    if (!err.empty()) throw std::invalid_argument(err);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// ADMM: settings = {mu_prox, tau, rho_power, rho_power_factor, ratio_primal_dual,
+// absolute_precision, relative_precision}; primal_guess / dual_guess may be null; outputs
+// y (primal), z (dual), s (shift), it, stats (absolute residual, relative residual, rho,
+// rho_power, largest eigenvalue) and the factor count per problem.
+extern "C" int orc_admm_solve(int64_t B, int nc, const double* G, const double* g, const double* mu, const double* R,
+                              const double* primal_guess, const double* dual_guess, const double* settings,
+                              int max_it, int max_it_eig, double* y, double* z, double* s, int32_t* it,
+                              double* stats, int32_t* chol_updates, uint8_t* converged, int nthreads) {
+  try {
+    if (nc < 1) throw std::invalid_argument("at least one cone");
+    if (max_it < 1) throw std::invalid_argument("max_it should be greater than 0.");
+    const int n = 3 * nc;
+    orc::AdmmSettings<double> st;
+    st.mu_prox = settings[0];
+    st.tau = settings[1];
+    st.rho_power = settings[2];
+    st.rho_power_factor = settings[3];
+    st.ratio_primal_dual = settings[4];
+    st.absolute_precision = settings[5];
+    st.relative_precision = settings[6];
+    st.max_it = max_it;
+    st.max_it_largest_eigen_value = max_it_eig;
+    std::string err;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (int64_t b = 0; b < B; ++b) {
+      try {
+        const auto r = orc::admmSolve<double>(nc, G + b * n * n, g + b * n, mu + b * nc, R + b * n,
+                                              primal_guess ? primal_guess + b * n : nullptr,
+                                              dual_guess ? dual_guess + b * n : nullptr, st, y + b * n, z + b * n,
+                                              s ? s + b * n : nullptr);
+        if (it) it[b] = r.it;
+        if (stats) {
+          double* o = stats + 5 * b;
+          o[0] = r.absolute_residual;
+          o[1] = r.relative_residual;
+          o[2] = r.rho;
+          o[3] = r.rho_power;
+          o[4] = r.largest_eigen_value;
+        }
+        if (chol_updates) chol_updates[b] = r.cholesky_update_count;
+        if (converged) converged[b] = r.converged ? 1 : 0;
+      } catch (const std::exception& e) {
+#pragma omp critical
+        err = e.what();
+      }
+    }
+    if (!err.empty()) throw std::invalid_argument(err);
+    return 0;
   } catch (const std::exception& e) {
     g_err = e.what();
     return 1;

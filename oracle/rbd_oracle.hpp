@@ -1146,8 +1146,9 @@ template<class S> inline Motion<S> log6(const SE3<S>& M) {
 }
 template<class S> inline SE3<S> inverse(const SE3<S>& M) { return SE3<S>{transpose(M.R), -mulT(M.R, M.p)}; }
 
-// rhs rows and Jacobian rows (nrows x nv, column major at Jc[i + k nrows]) of the
-// constraints, after the forward step (data.oMi, data.ov, data.oa, data.J filled).
+// rhs rows (skipped when rhs is null) and Jacobian rows (nrows x nv, column major at
+// Jc[i + k nrows]) of the constraints, after the forward step (data.oMi, data.ov, data.oa,
+// data.J filled).
 template<class S>
 inline void constraintTerms(const Model<S>& model, const Data<S>& data, int ncon, const RigidConstraint<S>* cons,
                             int nrows, S* rhs, S* Jc) {
@@ -1168,59 +1169,61 @@ inline void constraintTerms(const Model<S>& model, const Data<S>& data, int ncon
     const Motion<S>& oa2 = j2 > 0 ? data.oa[j2] : zm;
     const Motion<S> vc1 = j1 > 0 ? actInv(oMc1, ov1) : zm;
     const Motion<S> vc2 = j2 > 0 ? actInv(oMc2, ov2) : zm;
-    const Motion<S> vc2in1 = act(c1Mc2, vc2);
-    Motion<S> perr, verr, aerr = zm;
-    if (cm.type == CONTACT_6D) {
-      const Motion<S> l = log6(c1Mc2);
-      perr = {-l.lin, -l.ang};
-      verr = vc1 - vc2in1;
-    } else {
-      perr = {-c1Mc2.p, zero3<S>()};
-      verr = {vc1.lin - c1Mc2.R * vc2.lin, zero3<S>()};
-    }
-    bool gains = false;
-    for (int k = 0; k < dim; ++k) gains = gains || cm.Kp[k] != S(0) || cm.Kd[k] != S(0);
-    if (gains) {
-      for (int k = 0; k < 3; ++k) aerr.lin[k] = -cm.Kp[k] * perr.lin[k] - cm.Kd[k] * verr.lin[k];
-      if (cm.type == CONTACT_6D)
-        for (int k = 0; k < 3; ++k) aerr.ang[k] = -cm.Kp[3 + k] * perr.ang[k] - cm.Kd[3 + k] * verr.ang[k];
-    }
-    Motion<S> acc1, acc2;
-    if (cm.rf == RF_LOCAL_WORLD_ALIGNED) {
-      acc1.lin = oa1.lin + cross(oa1.ang, oMc1.p);
-      if (cm.type == CONTACT_3D) acc1.lin += cross(ov1.ang, ov1.lin + cross(ov1.ang, oMc1.p));
-      acc1.ang = oa1.ang;
-      if (cm.type == CONTACT_3D) {
-        acc2.lin = oa2.lin + cross(oa2.ang, oMc2.p) + cross(ov2.ang, ov2.lin + cross(ov2.ang, oMc2.p));
-        acc2.ang = zero3<S>();
+    if (rhs) {  // drift, errors and correction (not needed for the Jacobian alone)
+      const Motion<S> vc2in1 = act(c1Mc2, vc2);
+      Motion<S> perr, verr, aerr = zm;
+      if (cm.type == CONTACT_6D) {
+        const Motion<S> l = log6(c1Mc2);
+        perr = {-l.lin, -l.ang};
+        verr = vc1 - vc2in1;
       } else {
-        acc2.lin = oa2.lin + cross(oa2.ang, oMc1.p);  // (the reference takes c1's origin here)
-        acc2.ang = oa2.ang;
+        perr = {-c1Mc2.p, zero3<S>()};
+        verr = {vc1.lin - c1Mc2.R * vc2.lin, zero3<S>()};
       }
-      aerr = {oMc1.R * aerr.lin, oMc1.R * aerr.ang};
-    } else if (cm.rf == RF_LOCAL) {
-      acc1 = actInv(oMc1, oa1);
-      if (cm.type == CONTACT_3D) {
-        acc1.lin += cross(vc1.ang, vc1.lin);
-        acc1.ang = zero3<S>();
+      bool gains = false;
+      for (int k = 0; k < dim; ++k) gains = gains || cm.Kp[k] != S(0) || cm.Kd[k] != S(0);
+      if (gains) {
+        for (int k = 0; k < 3; ++k) aerr.lin[k] = -cm.Kp[k] * perr.lin[k] - cm.Kd[k] * verr.lin[k];
+        if (cm.type == CONTACT_6D)
+          for (int k = 0; k < 3; ++k) aerr.ang[k] = -cm.Kp[3 + k] * perr.ang[k] - cm.Kd[3 + k] * verr.ang[k];
+      }
+      Motion<S> acc1, acc2;
+      if (cm.rf == RF_LOCAL_WORLD_ALIGNED) {
+        acc1.lin = oa1.lin + cross(oa1.ang, oMc1.p);
+        if (cm.type == CONTACT_3D) acc1.lin += cross(ov1.ang, ov1.lin + cross(ov1.ang, oMc1.p));
+        acc1.ang = oa1.ang;
+        if (cm.type == CONTACT_3D) {
+          acc2.lin = oa2.lin + cross(oa2.ang, oMc2.p) + cross(ov2.ang, ov2.lin + cross(ov2.ang, oMc2.p));
+          acc2.ang = zero3<S>();
+        } else {
+          acc2.lin = oa2.lin + cross(oa2.ang, oMc1.p);  // (the reference takes c1's origin here)
+          acc2.ang = oa2.ang;
+        }
+        aerr = {oMc1.R * aerr.lin, oMc1.R * aerr.ang};
+      } else if (cm.rf == RF_LOCAL) {
+        acc1 = actInv(oMc1, oa1);
+        if (cm.type == CONTACT_3D) {
+          acc1.lin += cross(vc1.ang, vc1.lin);
+          acc1.ang = zero3<S>();
+        } else {
+          acc1 = acc1 + crossMM(verr, vc2in1);
+        }
+        if (cm.type == CONTACT_3D) {
+          acc2.lin = mulT(oMc1.R, oa2.lin + cross(oa2.ang, oMc2.p) + cross(ov2.ang, ov2.lin + cross(ov2.ang, oMc2.p)));
+          acc2.ang = zero3<S>();
+        } else {
+          acc2 = actInv(oMc1, oa2);
+        }
       } else {
-        acc1 = acc1 + crossMM(verr, vc2in1);
+        throw std::invalid_argument("constraintDynamics operates from LOCAL or LOCAL_WORLD_ALIGNED frames");
       }
-      if (cm.type == CONTACT_3D) {
-        acc2.lin = mulT(oMc1.R, oa2.lin + cross(oa2.ang, oMc2.p) + cross(ov2.ang, ov2.lin + cross(ov2.ang, oMc2.p)));
-        acc2.ang = zero3<S>();
-      } else {
-        acc2 = actInv(oMc1, oa2);
-      }
-    } else {
-      throw std::invalid_argument("constraintDynamics operates from LOCAL or LOCAL_WORLD_ALIGNED frames");
+      S r6[6], a1[6], a2[6], e6[6];
+      toVec(acc1, a1);
+      toVec(acc2, a2);
+      toVec(aerr, e6);
+      for (int k = 0; k < 6; ++k) r6[k] = -a1[k] + a2[k] + e6[k] + cm.desired[k];
+      for (int k = 0; k < dim; ++k) rhs[row + k] = r6[k];
     }
-    S r6[6], a1[6], a2[6], e6[6];
-    toVec(acc1, a1);
-    toVec(acc2, a2);
-    toVec(aerr, e6);
-    for (int k = 0; k < 6; ++k) r6[k] = -a1[k] + a2[k] + e6[k] + cm.desired[k];
-    for (int k = 0; k < dim; ++k) rhs[row + k] = r6[k];
     // Jacobian (contact-info.hpp:551-700)
     std::fill(sup1.begin(), sup1.end(), 0);
     std::fill(sup2.begin(), sup2.end(), 0);
@@ -1254,17 +1257,11 @@ inline void constraintTerms(const Model<S>& model, const Data<S>& data, int ncon
   }
 }
 
+// forward step (constrained-dynamics.hxx:71-135) for ov, oa and the WORLD joint Jacobian,
+// then the constraint rows (rhs, Jc) at q, v
 template<class S>
-inline void constraintDynamics(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* tau, int ncon,
-                               const RigidConstraint<S>* cons, const ProximalSettings<S>& settings, S* ddq, S* lambda,
-                               int* iterations) {
-  using std::abs;
-  using std::sqrt;
-  const int nv = model.nv;
-  if (settings.mu < S(0)) throw std::invalid_argument("mu has to be positive");
-  int nrows = 0;
-  for (int c = 0; c < ncon; ++c) nrows += cons[c].size();
-  // forward step (constrained-dynamics.hxx:71-135) for ov, oa; the WORLD joint Jacobian
+inline void constraintForward(const Model<S>& model, Data<S>& data, const S* q, const S* v, int ncon,
+                              const RigidConstraint<S>* cons, int nrows, S* rhs, S* Jc) {
   for (int i = 1; i < model.njoints; ++i) {
     const int p = model.parents[i];
     const JointCalc<S> jc = jointCalc(model, i, q, v);
@@ -1275,8 +1272,81 @@ inline void constraintDynamics(const Model<S>& model, Data<S>& data, const S* q,
     data.oa[i] = p > 0 ? crossMM(data.ov[p], data.ov[i]) + data.oa[p] : zeroMotion<S>();
     for (int k = 0; k < jc.nv; ++k) setCol(data.J, model.idx_v[i] + k, act(data.oMi[i], jc.Scol[k]));
   }
+  constraintTerms(model, data, ncon, cons, nrows, rhs, Jc);
+}
+
+// computeDelassusMatrix (delassus.hxx:118-300): the operational-space inverse inertia of the
+// constraints, J (M + mu J^T J)^-1 J^T -- the reference adds a contact inertia of mass mu
+// (rotational inertia mu I for 6-D constraints) at each constraint frame and runs its OSIM
+// sweeps.  Restated as D0 = J M^-1 J^T (tree factor, one solve per row) and, for mu > 0,
+// D0 (I + mu D0)^-1 (Woodbury).  G: nrows x nrows, column major.
+template<class S>
+inline void delassusMatrix(const Model<S>& model, Data<S>& data, const S* q, int ncon, const RigidConstraint<S>* cons,
+                           S mu, S* G) {
+  if (mu < S(0)) throw std::invalid_argument("mu has to be positive");
+  const int nv = model.nv;
+  int nrows = 0;
+  for (int c = 0; c < ncon; ++c) nrows += cons[c].size();
+  std::vector<S> Jc((size_t)nrows * nv), zero(nv, S(0)), Y((size_t)nv * nrows);
+  constraintForward(model, data, q, zero.data(), ncon, cons, nrows, (S*)nullptr, Jc.data());
+  crbaWorld(model, data, q);
+  choleskyDecompose(model, data);
+  for (int i = 0; i < nrows; ++i) {
+    S* col = &Y[(size_t)i * nv];
+    for (int k = 0; k < nv; ++k) col[k] = Jc[i + (size_t)k * nrows];
+    choleskySolve(model, data, col);
+  }
+  std::vector<S> D0((size_t)nrows * nrows);
+  for (int i = 0; i < nrows; ++i)
+    for (int j = 0; j < nrows; ++j) {
+      S g = S(0);
+      for (int k = 0; k < nv; ++k) g += Jc[i + (size_t)k * nrows] * Y[(size_t)j * nv + k];
+      D0[i + (size_t)j * nrows] = g;
+    }
+  if (mu == S(0)) {
+    std::copy(D0.begin(), D0.end(), G);
+    return;
+  }
+  // A = I + mu D0 (symmetric positive definite); G = A^-1 D0, column by column (LLT)
+  std::vector<S> L((size_t)nrows * nrows, S(0));
+  for (int j = 0; j < nrows; ++j) {
+    S d = S(1) + mu * D0[j + (size_t)j * nrows];
+    for (int k = 0; k < j; ++k) d -= L[j + (size_t)k * nrows] * L[j + (size_t)k * nrows];
+    L[j + (size_t)j * nrows] = std::sqrt(d);
+    for (int i = j + 1; i < nrows; ++i) {
+      S t = mu * D0[i + (size_t)j * nrows];
+      for (int k = 0; k < j; ++k) t -= L[i + (size_t)k * nrows] * L[j + (size_t)k * nrows];
+      L[i + (size_t)j * nrows] = t / L[j + (size_t)j * nrows];
+    }
+  }
+  for (int j = 0; j < nrows; ++j) {
+    S* x = G + (size_t)j * nrows;
+    for (int i = 0; i < nrows; ++i) x[i] = D0[i + (size_t)j * nrows];
+    for (int i = 0; i < nrows; ++i) {
+      S s = x[i];
+      for (int k = 0; k < i; ++k) s -= L[i + (size_t)k * nrows] * x[k];
+      x[i] = s / L[i + (size_t)i * nrows];
+    }
+    for (int i = nrows - 1; i >= 0; --i) {
+      S s = x[i];
+      for (int k = i + 1; k < nrows; ++k) s -= L[k + (size_t)i * nrows] * x[k];
+      x[i] = s / L[i + (size_t)i * nrows];
+    }
+  }
+}
+
+template<class S>
+inline void constraintDynamics(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* tau, int ncon,
+                               const RigidConstraint<S>* cons, const ProximalSettings<S>& settings, S* ddq, S* lambda,
+                               int* iterations) {
+  using std::abs;
+  using std::sqrt;
+  const int nv = model.nv;
+  if (settings.mu < S(0)) throw std::invalid_argument("mu has to be positive");
+  int nrows = 0;
+  for (int c = 0; c < ncon; ++c) nrows += cons[c].size();
   std::vector<S> rhs(nrows), Jc((size_t)nrows * nv), r(nv), zero(nv, S(0));
-  constraintTerms(model, data, ncon, cons, nrows, rhs.data(), Jc.data());
+  constraintForward(model, data, q, v, ncon, cons, nrows, rhs.data(), Jc.data());
   // M, nle, tree factor; r = M^-1 (tau - nle); Y = M^-1 J^T; G = J Y + mu I
   rnea(model, data, q, v, zero.data(), r.data());
   for (int k = 0; k < nv; ++k) r[k] = tau[k] - r[k];

@@ -478,6 +478,22 @@ class Engine:
             ctypes.c_uint32(self._flags(async_))))
         return out, lambda_out
 
+    def delassus(self, q, constraints, mu=0.0, out=None, async_=False):
+        """computeDelassusMatrix(model, data, q, contact_models, contact_datas, G, mu) per
+        configuration (delassus.hxx:118-300): J (M + mu J^T J)^-1 J^T for the constraints'
+        Jacobian (RigidConstraintModel list, LOCAL or LOCAL_WORLD_ALIGNED rows).  Returns G
+        with nrows^2 rows per configuration (element (i, j) at row i + j nrows)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        table = _constraint_table(constraints)
+        nrows = sum(c.size() for c in constraints)
+        out = self._new_vec(q, nrows * nrows, B) if out is None else out
+        _, ldG = self._vec(out, nrows * nrows, "G")
+        _check(lib().rbd_delassus(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                  len(constraints), table, ctypes.c_double(mu), _ptr(out), _i64(ldG),
+                                  ctypes.c_uint32(self._flags(async_))))
+        return out
+
     @staticmethod
     def mats_to_numpy(t, layout, nv):
         """Device matrix output -> (B, nv, nv) numpy array with [b] the configuration's matrix."""
@@ -576,6 +592,228 @@ def _constraint_table(constraints):
 # ------------------------------------------------------------------------------------
 # Mixed-robot batches (one launch over instances of several models)
 # ------------------------------------------------------------------------------------
+# ------------------------------------------------------------------------------------
+# Frictional contact solvers (PGS, ADMM) over batches of problems
+# ------------------------------------------------------------------------------------
+RBD_NCP_STATS = 8
+
+
+class CoulombFrictionCone:
+    """CoulombFrictionConeTpl (constraints/coulomb-friction-cone.hpp:20-199): {|f_t| <= mu f_n}
+    on a contact's (tangent x, tangent y, normal) components."""
+
+    def __init__(self, mu: float):
+        if mu < 0:
+            raise ValueError("mu must be positive")
+        self.mu = float(mu)
+
+
+class _PgsSettingsC(ctypes.Structure):
+    _fields_ = [("max_it", ctypes.c_int32), ("absolute_precision", ctypes.c_double),
+                ("relative_precision", ctypes.c_double), ("over_relax", ctypes.c_double)]
+
+
+class _AdmmSettingsC(ctypes.Structure):
+    _fields_ = [("mu_prox", ctypes.c_double), ("tau", ctypes.c_double), ("rho_power", ctypes.c_double),
+                ("rho_power_factor", ctypes.c_double), ("ratio_primal_dual", ctypes.c_double),
+                ("max_it_largest_eigen_value", ctypes.c_int32), ("max_it", ctypes.c_int32),
+                ("absolute_precision", ctypes.c_double), ("relative_precision", ctypes.c_double)]
+
+
+class _ContactSolverBase:
+    """ContactSolverBaseTpl (contact-solver-base.hpp:20-120) over a batch: the problems are the
+    rows of (B, ...) float64 device tensors -- G (B, n, n), g (B, n) -- and every per-problem
+    result (iteration count, residuals) is an array of B values."""
+
+    def __init__(self, ctx: Context, problem_dim: int):
+        if problem_dim % 3 or not 3 <= problem_dim <= 12:
+            raise ValueError("the problem size must be 3 ncones with 1 <= ncones <= 4")
+        self.ctx = ctx
+        self.problem_dim = int(problem_dim)
+        self.max_it = 1000
+        self.absolute_precision = 1e-6
+        self.relative_precision = 1e-6
+        self.stats = None
+
+    def getProblemSize(self):
+        return self.problem_dim
+
+    def setMaxIterations(self, max_it):
+        if max_it <= 0:
+            raise ValueError("max_it should be greater than 0.")
+        self.max_it = int(max_it)
+
+    def getMaxIterations(self):
+        return self.max_it
+
+    def setAbsolutePrecision(self, p):
+        if p < 0:
+            raise ValueError("absolute_precision should be positive.")
+        self.absolute_precision = float(p)
+
+    def getAbsolutePrecision(self):
+        return self.absolute_precision
+
+    def setRelativePrecision(self, p):
+        if p < 0:
+            raise ValueError("relative_precision should be positive.")
+        self.relative_precision = float(p)
+
+    def getRelativePrecision(self):
+        return self.relative_precision
+
+    def _stat(self, k):
+        if self.stats is None:
+            raise RuntimeError("solve() has not run")
+        return self.stats[:, k]
+
+    def getIterationCount(self):
+        return self._stat(0).to(_torch_int32())
+
+    def getAbsoluteConvergenceResidual(self):
+        return self._stat(1)
+
+    def getRelativeConvergenceResidual(self):
+        return self._stat(2)
+
+    def _problem(self, G, g, cones):
+        import torch
+        n = self.problem_dim
+        nc = n // 3
+        if G.dim() == 3:
+            G = G.transpose(1, 2)  # element (i, j) at i + j n
+        G = G.reshape(G.shape[0], n * n).to(torch.float64).contiguous()
+        B = G.shape[0]
+        g = g.to(torch.float64).contiguous()
+        if tuple(g.shape) != (B, n):
+            raise ValueError(f"g: expected ({B}, {n}), got {tuple(g.shape)}")
+        if isinstance(cones, (list, tuple)):
+            if len(cones) != nc:
+                raise ValueError(f"{nc} cones expected, got {len(cones)}")
+            mu = torch.tensor([[c.mu for c in cones]], dtype=torch.float64, device=G.device).expand(B, nc)
+        else:
+            mu = cones
+        mu = mu.to(torch.float64).contiguous()
+        if tuple(mu.shape) != (B, nc):
+            raise ValueError(f"friction coefficients: expected ({B}, {nc}), got {tuple(mu.shape)}")
+        return B, nc, G, g, mu
+
+
+def _torch_int32():
+    import torch
+    return torch.int32
+
+
+class PGSContactSolver(_ContactSolverBase):
+    """PGSContactSolverTpl (pgs-solver.hpp:16-59) over a batch of problems."""
+
+    def __init__(self, ctx: Context, problem_dim: int):
+        super().__init__(ctx, problem_dim)
+
+    def solve(self, G, g, cones, x, over_relax=1.0, async_=False):
+        """PGSContactSolverTpl::solve(G, g, cones, x, over_relax) (pgs-solver.hxx:19-141) on each
+        problem: x (B, n) float64 holds the initial guess and receives the solution.  cones:
+        a list of CoulombFrictionCone shared by every problem, or a (B, ncones) tensor of
+        friction coefficients.  Returns a bool tensor: converged per problem."""
+        import torch
+        B, nc, G, g, mu = self._problem(G, g, cones)
+        n = self.problem_dim
+        if x.dtype != torch.float64 or not x.is_contiguous() or tuple(x.shape) != (B, n):
+            raise ValueError(f"x: a contiguous float64 ({B}, {n}) tensor")
+        self.stats = torch.empty((B, RBD_NCP_STATS), dtype=torch.float64, device=G.device)
+        st = _PgsSettingsC(self.max_it, self.absolute_precision, self.relative_precision, over_relax)
+        _check(lib().rbd_pgs_solve(self.ctx.handle, _i64(B), nc, _ptr(G), _i64(n * n), _ptr(g), _i64(n), _ptr(mu),
+                                   _i64(nc), _ptr(x), _i64(n), ctypes.byref(st), _ptr(self.stats),
+                                   _i64(RBD_NCP_STATS), ctypes.c_uint32(RBD_ASYNC if async_ else 0)))
+        return self.stats[:, 3] != 0
+
+
+class ADMMContactSolver(_ContactSolverBase):
+    """ADMMContactSolverTpl (admm-solver.hpp:120-127 parameters) over a batch of problems;
+    each problem is solved by a fresh solver on a fresh dense Delassus operator."""
+
+    def __init__(self, ctx: Context, problem_dim: int, mu_prox=1e-6, tau=0.5, rho_power=0.2, rho_power_factor=0.05,
+                 ratio_primal_dual=10.0, max_it_largest_eigen_value_solver=20):
+        super().__init__(ctx, problem_dim)
+        self.mu_prox, self.tau = float(mu_prox), float(tau)
+        self.rho_power, self.rho_power_factor = float(rho_power), float(rho_power_factor)
+        self.ratio_primal_dual = float(ratio_primal_dual)
+        self.max_it_largest_eigen_value = int(max_it_largest_eigen_value_solver)
+        self.y = self.z = self.s = None
+
+    def solve(self, G, g, cones, R=None, primal_guess=None, dual_guess=None, async_=False):
+        """ADMMContactSolverTpl::solve(delassus, g, cones, R, primal_guess, dual_guess)
+        (admm-solver.hxx:22-335) on each problem.  R: (B, n) compliance (zero if None).
+        Returns a bool tensor: converged per problem."""
+        import torch
+        B, nc, G, g, mu = self._problem(G, g, cones)
+        n = self.problem_dim
+        dev = G.device
+        R = torch.zeros((B, n), dtype=torch.float64, device=dev) if R is None else R.to(torch.float64).contiguous()
+        pg = None if primal_guess is None else primal_guess.to(torch.float64).contiguous()
+        dg = None if dual_guess is None else dual_guess.to(torch.float64).contiguous()
+        self.y, self.z, self.s = (torch.empty((B, n), dtype=torch.float64, device=dev) for _ in range(3))
+        self.stats = torch.empty((B, RBD_NCP_STATS), dtype=torch.float64, device=dev)
+        st = _AdmmSettingsC(self.mu_prox, self.tau, self.rho_power, self.rho_power_factor, self.ratio_primal_dual,
+                            self.max_it_largest_eigen_value, self.max_it, self.absolute_precision,
+                            self.relative_precision)
+        _check(lib().rbd_admm_solve(self.ctx.handle, _i64(B), nc, _ptr(G), _i64(n * n), _ptr(g), _i64(n), _ptr(mu),
+                                    _i64(nc), _ptr(R), _i64(n), None if pg is None else _ptr(pg),
+                                    None if dg is None else _ptr(dg), _i64(n), ctypes.byref(st), _ptr(self.y),
+                                    _ptr(self.z), _ptr(self.s), _i64(n), _ptr(self.stats), _i64(RBD_NCP_STATS),
+                                    ctypes.c_uint32(RBD_ASYNC if async_ else 0)))
+        return self.stats[:, 3] != 0
+
+    def setTau(self, tau):
+        self.tau = float(tau)
+
+    def getTau(self):
+        return self.tau
+
+    def setRhoPowerFactor(self, f):
+        self.rho_power_factor = float(f)
+
+    def getRhoPowerFactor(self):
+        return self.rho_power_factor
+
+    def setRatioPrimalDual(self, r):
+        if r <= 0:
+            raise ValueError("The ratio primal/dual should be positive strictly")
+        self.ratio_primal_dual = float(r)
+
+    def getRatioPrimalDual(self):
+        return self.ratio_primal_dual
+
+    def setProximalValue(self, mu):
+        if mu < 0:
+            raise ValueError("mu_prox should be positive.")
+        self.mu_prox = float(mu)
+
+    def getProximalValue(self):
+        return self.mu_prox
+
+    def getPrimalSolution(self):
+        return self.y
+
+    def getDualSolution(self):
+        return self.z
+
+    def getComplementarityShift(self):
+        return self.s
+
+    def getRho(self):
+        return self._stat(4)
+
+    def getRhoPower(self):
+        return self._stat(5)
+
+    def getLargestEigenValue(self):
+        return self._stat(6)
+
+    def getCholeskyUpdateCount(self):
+        return self._stat(7).to(_torch_int32())
+
+
 class _BatchGroup(ctypes.Structure):
     _fields_ = [("model", ctypes.c_void_p), ("count", ctypes.c_int64), ("q", ctypes.c_void_p),
                 ("v", ctypes.c_void_p), ("a", ctypes.c_void_p), ("tau", ctypes.c_void_p),

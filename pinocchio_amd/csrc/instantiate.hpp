@@ -133,8 +133,9 @@ void launchRneaD(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S
 template<class TP, class S>
 void launchAbaDSolve(hipStream_t st, GM, int64_t B, CView<S> U, CView<S> D, View<S> Xq, View<S> Xv, View<S> Minv,
                      int jobBegin) {
-  hipLaunchKernelGGL((k_abad_solve<TP, S>), dim3(gridFor(B)), dim3(kTile * kSolveWaves), 0, st, B, U, D, Xq, Xv,
-                     Minv, jobBegin);
+  constexpr int CT = SolveTile<S>::CT;
+  hipLaunchKernelGGL((k_abad_solve<TP, S>), dim3((unsigned)((B + CT - 1) / CT)), dim3(kTile * kSolveWaves), 0, st, B, U,
+                     D, Xq, Xv, Minv, jobBegin);
 }
 
 
@@ -146,11 +147,11 @@ void launchKin(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> 
 
 template<class TP, class S>
 void launchKkt(hipStream_t st, GM, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J, CView<S> tau,
-               CView<S> nle, CView<S> gamma, double damping, int impulse, double rcoef, const KktProx& prox,
+               CView<S> nle, CView<S> gamma, double damping, int mode, double rcoef, const KktProx& prox,
                View<S> ddq, View<S> lam) {
   using K = KktShape<S>;
   hipLaunchKernelGGL((k_kkt<TP, S>), dim3((unsigned)((B + K::T - 1) / K::T)), dim3(64 * K::W), 0, st, B, U, D, nc, J,
-                     tau, nle, gamma, damping, impulse, rcoef, prox, ddq, lam);
+                     tau, nle, gamma, damping, mode, rcoef, prox, ddq, lam);
 }
 
 template<class TP> TopoOps makeOpsBase() {
@@ -187,7 +188,10 @@ template<class TP> void add_chol(TopoOps& o) {
   o.nnzU = nnzOf<TP>();
 }
 template<class TP> void add_rnead(TopoOps& o) { o.rnead_f32 = &launchRneaD<TP, float>; }
-template<class TP> void add_abads(TopoOps& o) { o.abad_solve_f32 = &launchAbaDSolve<TP, float>; }
+template<class TP> void add_abads(TopoOps& o) {
+  o.abad_solve_f32 = &launchAbaDSolve<TP, float>;
+  if constexpr (abadSolveLdsBytes<TP, double>() <= 160 * 1024) o.abad_solve_f64 = &launchAbaDSolve<TP, double>;
+}
 // (topologies whose staged FP64 factor fits the LDS)
 template<class TP> void add_kkt(TopoOps& o) {
   if constexpr (kktLdsBytes<TP, float>() <= 160 * 1024 && kktLdsBytes<TP, double>() <= 160 * 1024) {

@@ -327,18 +327,28 @@ __global__ __launch_bounds__(64 * G) void k_rnead(const S* __restrict__ mc_in, i
 }
 
 // ---------------------------------------------------------------------------------------
-// ABA-derivative solve.  Input: the packed FP32 tree factor of M (k_chol<.., float,
-// float>, batch-fastest workspace), dtau_dq in gXq and dtau_dv in gXv.  Output, in
+// ABA-derivative solve.  Input: the packed tree factor of M (k_chol, batch-fastest
+// workspace, in the scalar of the outputs), dtau_dq in gXq and dtau_dv in gXv.  Output, in
 // place: -Minv dtau_dq, -Minv dtau_dv, and Minv in gMinv.  Jobs (right-hand sides) before
 // jobBegin are skipped: jobBegin = 2 nv computes Minv alone (computeMinverse).
 //
-// A workgroup owns 64 consecutive configurations: their factors (nnz + nv scalars each)
-// are staged in LDS as [entry][configuration], and its waves share out the 3 nv
-// right-hand sides, two at a time; lane b of a wave solves configuration b0 + b, so
-// every column load and store is one coalesced 256-byte row in the batch-fastest
-// layout and every factor read is a conflict-free LDS row.
+// A workgroup owns a tile of consecutive configurations: their factors (nnz + nv scalars
+// each) are staged in LDS as [entry][configuration], and its waves share out the 3 nv
+// right-hand sides; every column load and store is a coalesced run of the batch-fastest
+// layout and every factor read a conflict-free LDS row.  FP32: 64 configurations per
+// tile, lane b of a wave solves configuration b0 + b, two right-hand sides at a time.
+// FP64: 32 configurations per tile (the factor of 64 would not fit the LDS), the two
+// halves of a wave on two right-hand sides of the same 32 configurations, one each.
 // ---------------------------------------------------------------------------------------
 constexpr int kSolveWaves = 12;  // 168 VGPRs: three waves per SIMD at one workgroup per CU
+
+template<class S> struct SolveTile {
+  static constexpr int CT = sizeof(S) == 4 ? kTile : kTile / 2;  // configurations per tile
+  static constexpr int NR = sizeof(S) == 4 ? 2 : 1;              // right-hand sides per lane
+};
+template<class TP, class S> constexpr int abadSolveLdsBytes() {
+  return (nnzOf<TP>() + TP::T.nv) * SolveTile<S>::CT * (int)sizeof(S);
+}
 
 template<class TP, class S>
 __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CView<S> gU, CView<S> gD, View<S> gXq,
@@ -346,42 +356,46 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
   constexpr auto& T = TP::T;
   constexpr int NV = T.nv;
   constexpr int NNZ = nnzOf<TP>();
+  constexpr int CT = SolveTile<S>::CT, NR = SolveTile<S>::NR, HALVES = kTile / CT;
   using P = CholPattern<TP>;
-  __shared__ S sU[(NNZ + NV) * kTile];  // U entries then D^-1, [entry][lane]
-  const int64_t b0 = int64_t(blockIdx.x) * kTile;
-  const int nb = (int)(B - b0 < kTile ? B - b0 : kTile);
+  __shared__ S sU[(NNZ + NV) * CT];  // U entries then D^-1, [entry][configuration]
+  const int64_t b0 = int64_t(blockIdx.x) * CT;
+  const int nb = (int)(B - b0 < CT ? B - b0 : CT);
   const int L = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
+  const int cl = L % CT, h = L / CT;
   // staged kStageU entries per thread at a time, all loads issued before the first is
   // used (lanes past the batch end read the last configuration's entry and store 1)
   {
-    constexpr int kTot = (NNZ + NV) * kTile, kStep = 64 * kSolveWaves;
-    const int b = threadIdx.x & 63;
+    constexpr int kTot = (NNZ + NV) * CT, kStep = 64 * kSolveWaves;
+    static_assert(kStep % CT == 0, "a thread's entries share its configuration");
+    const int b = threadIdx.x % CT;
     const int bc = b < nb ? b : nb - 1;
     static_for<0, (kTot + kStep * kStageU - 1) / (kStep * kStageU)>([&](auto I) {
       S x[kStageU];
       static_for<0, kStageU>([&](auto U) {
         const int e = (int(I) * kStageU + int(U)) * kStep + (int)threadIdx.x;
-        const int k = (e < kTot ? e : kTot - 1) >> 6;
+        const int k = (e < kTot ? e : kTot - 1) / CT;
         x[U] = k < NNZ ? gU.p[(int64_t)k * gU.rs + (b0 + bc) * gU.bs] : gD.p[(int64_t)(k - NNZ) * gD.rs + (b0 + bc) * gD.bs];
       });
       static_for<0, kStageU>([&](auto U) {
         const int e = (int(I) * kStageU + int(U)) * kStep + (int)threadIdx.x;
         if (e < kTot) {
-          const int k = e >> 6;
+          const int k = e / CT;
           sU[e] = b >= nb ? S(1) : (k < NNZ ? x[U] : S(1) / x[U]);
         }
       });
     });
   }
   __syncthreads();
-  const bool live = L < nb;
-  const int64_t b = b0 + L;
+  const bool live = cl < nb;
+  const int64_t b = b0 + cl;
 
-  // two right-hand sides at a time: x <- M^-1 x (Uiv, D^-1, Utiv; cholesky.hxx:466-520).
-  // A compiler memory clobber at every elimination step keeps the factor reads where they
-  // are: without it they are hoisted out of the job loop and all 337 values stay live.
-  const S* su = sU + L;
+  // right-hand sides two (FP32) or one (FP64) at a time: x <- M^-1 x (Uiv, D^-1, Utiv;
+  // cholesky.hxx:466-520).  A compiler memory clobber at every elimination step keeps the
+  // factor reads where they are: without it they are hoisted out of the job loop and all
+  // 337 values stay live.
+  const S* su = sU + cl;
   // U^-1 runs by columns: once x[jc] is final it updates its ancestors' entries
   // (independent multiply-adds, where the reference's row dot products chain).
   auto solve2 = [&](S (&x)[NV], S (&y)[NV]) {
@@ -393,7 +407,7 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
         constexpr int i = II;
         if constexpr (P::anc(i, jc)) {
           constexpr int pk = P::packed(i, jc);
-          const S u = su[pk * kTile];
+          const S u = su[pk * CT];
           x[i] -= u * x[jc];
           y[i] -= u * y[jc];
         }
@@ -403,7 +417,7 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
     asm volatile("" ::: "memory");
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      const S di = su[(NNZ + k) * kTile];
+      const S di = su[(NNZ + k) * CT];
       x[k] *= di;
       y[k] *= di;
     }
@@ -415,10 +429,41 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
         constexpr int e2 = EE;
         if constexpr (P::anc(k, e2)) {
           constexpr int pk = P::packed(k, e2);
-          const S u = su[pk * kTile];
+          const S u = su[pk * CT];
           x[e2] -= u * x[k];
           y[e2] -= u * y[k];
           pin(x[e2], y[e2]);
+        }
+      });
+    });
+  };
+  auto solve1 = [&](S (&x)[NV]) {
+    static_for<0, NV>([&](auto JJ) {
+      constexpr int jc = NV - 1 - JJ;
+      stepFence();
+      asm volatile("" ::: "memory");
+      static_for<0, jc>([&](auto II) {
+        constexpr int i = II;
+        if constexpr (P::anc(i, jc)) {
+          constexpr int pk = P::packed(i, jc);
+          x[i] -= su[pk * CT] * x[jc];
+        }
+      });
+      pin(x[jc]);
+    });
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int k = 0; k < NV; ++k) x[k] *= su[(NNZ + k) * CT];
+    static_for<0, NV>([&](auto KK) {
+      constexpr int k = KK;
+      stepFence();
+      asm volatile("" ::: "memory");
+      static_for<k + 1, NV>([&](auto EE) {
+        constexpr int e2 = EE;
+        if constexpr (P::anc(k, e2)) {
+          constexpr int pk = P::packed(k, e2);
+          x[e2] -= su[pk * CT] * x[k];
+          pin(x[e2]);
         }
       });
     });
@@ -449,14 +494,24 @@ __global__ __launch_bounds__(64 * kSolveWaves) void k_abad_solve(int64_t B, CVie
 #pragma unroll
     for (int r = 0; r < NV; ++r) stOut(&p[r * rs], sgn * x[r]);
   };
+  if constexpr (NR == 2) {
 #pragma unroll 1
-  for (int j = jobBegin + 2 * w; j < 3 * NV; j += 2 * kSolveWaves) {
-    S x[NV], y[NV];
-    load(j, x);
-    load(j + 1 < 3 * NV ? j + 1 : j, y);
-    solve2(x, y);
-    store(j, x);
-    if (j + 1 < 3 * NV) store(j + 1, y);
+    for (int j = jobBegin + 2 * w; j < 3 * NV; j += 2 * kSolveWaves) {
+      S x[NV], y[NV];
+      load(j, x);
+      load(j + 1 < 3 * NV ? j + 1 : j, y);
+      solve2(x, y);
+      store(j, x);
+      if (j + 1 < 3 * NV) store(j + 1, y);
+    }
+  } else {
+#pragma unroll 1
+    for (int j = jobBegin + HALVES * w + h; j < 3 * NV; j += HALVES * kSolveWaves) {
+      S x[NV];
+      load(j, x);
+      solve1(x);
+      store(j, x);
+    }
   }
 }
 

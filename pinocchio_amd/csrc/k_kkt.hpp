@@ -37,7 +37,7 @@ template<class S> struct KktShape {
 template<class TP, class S> constexpr int kktLdsBytes() {
   constexpr int T = KktShape<S>::T, NV = TP::T.nv, NC = kKktMaxConstraints;
   const int staged = (nnzOf<TP>() + NV) * T * 8 + NC * NV * T * (int)sizeof(S);
-  const int later = (NC + 1) * NV * T * (int)sizeof(S) + NC * (NC + 1) * T * 8 + 3 * NC * T * 8;
+  const int later = (NC + 1) * NV * T * (int)sizeof(S) + NC * (NC + 1) * T * 8 + NC * T * 8;
   return staged > later ? staged : later;
 }
 
@@ -54,13 +54,18 @@ struct KktProx {
 };
 
 // J is nc x nv per configuration, column major (element (i, k) at row i + k nc of gJ).
-// impulse != 0: impulseDynamics (contact-dynamics.hxx:288-331) -- gtau holds v_before, which
-// is taken as r unsolved, the right-hand side is -(1 + rcoef) J v_before (gnle and gg
-// unused), and gddq receives dq_after = v_before + M^-1 J^T lambda.
+// mode 0: forwardDynamics / constraintDynamics.
+// mode 1: impulseDynamics (contact-dynamics.hxx:288-331) -- gtau holds v_before, which is
+// taken as r unsolved, the right-hand side is -(1 + rcoef) J v_before (gnle and gg unused),
+// and gddq receives dq_after = v_before + M^-1 J^T lambda.
+// mode 2: the Delassus matrix of the constraints (computeDelassusMatrix, delassus.hxx:
+// 118-300, with damping = its mu): J (M + mu J^T J)^-1 J^T = (I + mu D0)^-1 D0 with
+// D0 = J M^-1 J^T, written to glam (element (i, j) at row i + j nc); gtau, gnle, gg and
+// gddq unused.
 template<class TP, class S>
 __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<double> gU, CView<double> gD, int nc,
                                                             CView<S> gJ, CView<S> gtau, CView<S> gnle, CView<S> gg,
-                                                            double damping, int impulse, double rcoef, KktProx prox,
+                                                            double damping, int mode, double rcoef, KktProx prox,
                                                             View<S> gddq, View<S> glam) {
   using K = KktShape<S>;
   constexpr int T = K::T, NT = 64 * K::W;
@@ -76,8 +81,6 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
   constexpr int kYDoubles = ((NC + 1) * NV * T * (int)sizeof(S) + 7) / 8;
   double* sP = smem + kYDoubles;
   double* sL = sP + NC * (NC + 1) * T;
-  double* sB = sL + NC * T;  // proximal iterations: b, then the previous lambda
-  double* sQ = sB + NC * T;
   const int64_t b0 = int64_t(blockIdx.x) * T;
   const int nb = (int)(B - b0 < T ? B - b0 : T);
   const int t = (int)threadIdx.x;
@@ -120,11 +123,11 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
   __syncthreads();
 
   // ---- phase 2: right-hand side j of configuration c (j < nc: row j of J; j = nc: tau - nle,
-  // or v_before, unsolved, for impulses)
+  // or v_before, unsolved, for impulses; none in mode 2)
   const int L = t & 63, w = t >> 6;
   const int c = L % T;
   const int j = w * K::RPW + L / T;
-  const bool active = j <= nc && c < nb;
+  const bool active = (j < nc || (j == nc && mode != 2)) && c < nb;
   const int64_t b = b0 + c;
   double x[NV];
   double p[NC];
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
     if (j < nc) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) x[k] = double(sJ[(j + k * NC) * T + c]);
-    } else if (impulse) {
+    } else if (mode == 1) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) x[k] = double(gtau.p[(int64_t)k * gtau.rs + b * gtau.bs]);
     } else {
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         x[k] = double(gtau.p[(int64_t)k * gtau.rs + b * gtau.bs]) - double(gnle.p[(int64_t)k * gnle.rs + b * gnle.bs]);
     }
   }
-  if (active && !(impulse && j == nc)) {
+  if (active && !(mode == 1 && j == nc)) {
     // x <- M^-1 x: Uiv, D^-1, Utiv (cholesky.hxx:193-317, 466-520).  U^-1 runs by columns:
     // once x[jc] is final it updates its ancestors' entries, independent multiply-adds
     // instead of the reference's row dot products (one dependent chain per row).  A
@@ -202,95 +205,107 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
   }
   __syncthreads();
 
-  // ---- phase 4: one lane per configuration: G = J Y + damping I (padded to NC with a unit
-  // diagonal), LLT, lambda = G^-1 (-J r - gamma).  No pivoting and no rank check, like the
-  // reference's LLT: a rank-deficient J with zero damping gives non-finite results.
-  if (t < nb) {
-    constexpr auto gi = [](int r, int cc) { return r * (r + 1) / 2 + cc; };  // (r, cc), cc <= r
-    const int64_t bt = b0 + t;
-    double G[NC * (NC + 1) / 2], l[NC];
+  // ---- phase 4: sixteen lanes per configuration, a group of adjacent lanes of one wave
+  // (lane r < NC holds row r of G = J Y + damping I, padded to NC rows with a unit
+  // diagonal; lanes NC.. ride along).  Right-looking LLT: at step d the pivot's lane
+  // broadcasts L_dd and each L_ed to the group, and every lane updates its own row; lane d
+  // keeps the broadcast L_ed, which is column d of L.  No pivoting and no rank check, like
+  // the reference's LLT: a rank-deficient J with zero damping gives non-finite results.
+  // Then lambda = G^-1 (-J r - gamma) by two triangular sweeps of broadcasts, and the
+  // proximal iterations with the group's residuals reduced across its lanes.
+  // mode 2 (Delassus): J M^-1 J^T out instead, or for damping mu > 0
+  // (I + mu J M^-1 J^T)^-1 J M^-1 J^T.
+  static_assert(NT == 16 * T, "sixteen lanes per configuration");
+  if (t < 16 * nb) {  // whole groups: the configuration's sixteen lanes
+    const int r = t & 15, cg = t >> 4;
+    const bool row = r < nc;
+    const int64_t bt = b0 + cg;
+    const bool delassus = mode == 2;
+    double a[NC], pr[NC], lc[NC];
+    double rhs = 0.0, gam = 0.0, rid = 1.0;
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
-#pragma unroll
-      for (int d = 0; d <= i; ++d)
-        G[gi(i, d)] = i < nc ? sP[(i + d * NC) * T + t] + (d == i ? damping : 0.0) : (d == i ? 1.0 : 0.0);
-      l[i] = i >= nc ? 0.0
-             : impulse ? -(1.0 + rcoef) * sP[(i + nc * NC) * T + t]
-                       : -sP[(i + nc * NC) * T + t] - double(gg.p[(int64_t)i * gg.rs + bt * gg.bs]);
+    for (int e = 0; e < NC; ++e) {
+      pr[e] = row && e < nc ? sP[(r + e * NC) * T + cg] : 0.0;
+      const double ge = delassus ? (e == r ? 1.0 : 0.0) + damping * pr[e] : pr[e] + (e == r ? damping : 0.0);
+      a[e] = row ? (e <= r ? ge : 0.0) : (e == r ? 1.0 : 0.0);
+      lc[e] = 0.0;
     }
-    // column by column: L_dd = sqrt(.), then the column below scaled by 1 / L_dd (kept)
-    double rd[NC];
-#pragma unroll
-    for (int d = 0; d < NC; ++d) {
-      double s = G[gi(d, d)];
-#pragma unroll
-      for (int k = 0; k < d; ++k) s -= G[gi(d, k)] * G[gi(d, k)];
-      const double ld = sqrt(s);
-      rd[d] = 1.0 / ld;
-#pragma unroll
-      for (int i = d + 1; i < NC; ++i) {
-        double t2 = G[gi(i, d)];
-#pragma unroll
-        for (int k = 0; k < d; ++k) t2 -= G[gi(i, k)] * G[gi(d, k)];
-        G[gi(i, d)] = t2 * rd[d];
-      }
+    if (row && !delassus) {
+      rhs = mode == 1 ? -(1.0 + rcoef) * sP[(r + nc * NC) * T + cg]
+                      : -sP[(r + nc * NC) * T + cg] - double(gg.p[(int64_t)r * gg.rs + bt * gg.bs]);
+      if (prox.maxIter > 1) gam = double(gg.p[(int64_t)r * gg.rs + bt * gg.bs]);
     }
-    auto solveG = [&]() {
+    static_for<0, NC>([&](auto D) {
+      constexpr int d = D;
+      const double rd = 1.0 / sqrt(__shfl(a[d], d, 16));
+      if (r == d) rid = rd;
+      if (r > d) a[d] *= rd;
+      static_for<d + 1, NC>([&](auto E) {
+        constexpr int e = E;
+        const double le = __shfl(a[d], e, 16);
+        if (r >= e) a[e] -= a[d] * le;
+        if (r == d) lc[e] = le;
+      });
+    });
+    {
+      // G^-1 b for b spread one row per lane: L y = b, then L^T x = y
+      auto solve = [&](double bv) {
+        double s = bv, y = 0.0;
+        static_for<0, NC>([&](auto I) {
+          constexpr int i = I;
+          const double yi = __shfl(s * rid, i, 16);
+          if (r == i) y = yi;
+          else if (r > i) s -= a[i] * yi;
+        });
+        s = y;
+        double x = 0.0;
+        static_for<0, NC>([&](auto I) {
+          constexpr int i = NC - 1 - I;
+          const double xi = __shfl(s * rid, i, 16);
+          if (r == i) x = xi;
+          else if (r < i) s -= lc[i] * xi;
+        });
+        return x;
+      };
+      if (delassus) {
+        // column j of the output: the Delassus column (mu = 0), or (I + mu G)^-1 of it
+        for (int j = 0; j < nc; ++j) {
+          double gj = 0.0;
 #pragma unroll
-      for (int i = 0; i < NC; ++i) {
-        double s = l[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) s -= G[gi(i, k)] * l[k];
-        l[i] = s * rd[i];
-      }
-#pragma unroll
-      for (int i = NC - 1; i >= 0; --i) {
-        double s = l[i];
-#pragma unroll
-        for (int k = i + 1; k < NC; ++k) s -= G[gi(k, i)] * l[k];
-        l[i] = s * rd[i];
-      }
-    };
-    if (prox.maxIter > 1) {
-#pragma unroll
-      for (int i = 0; i < NC; ++i) {
-        sB[i * T + t] = l[i];
-        sQ[i * T + t] = 0.0;
-      }
-    }
-    solveG();
-    int it = 1;
-    while (it < prox.maxIter) {
-      double ares = 0.0, rres = 0.0;
-#pragma unroll
-      for (int i = 0; i < NC; ++i) {
-        if (i < nc) {
-          double g = double(gg.p[(int64_t)i * gg.rs + bt * gg.bs]);
-#pragma unroll
-          for (int d = 0; d < NC; ++d)
-            if (d < nc) g -= sP[(i + d * NC) * T + t] * l[d];
-          ares = fmax(ares, fabs(g));
-          rres = fmax(rres, fabs(l[i] - sQ[i * T + t]));
+          for (int e = 0; e < NC; ++e) gj = e == j ? pr[e] : gj;
+          const double xj = damping > 0.0 ? solve(gj) : gj;
+          if (row) stOut(&glam.p[(int64_t)(r + j * nc) * glam.rs + bt * glam.bs], S(xj));
         }
       }
-      if (ares <= prox.absAcc || rres <= prox.relAcc) break;
+      double lam = delassus ? 0.0 : solve(rhs);
+      double prev = 0.0;
+      int it = 1;
+      while (!delassus && it < prox.maxIter) {
+        double g = gam;
+        static_for<0, NC>([&](auto D) {
+          constexpr int d = D;
+          g -= pr[d] * __shfl(lam, d, 16);
+        });
+        double ares = row ? fabs(g) : 0.0, rres = row ? fabs(lam - prev) : 0.0;
 #pragma unroll
-      for (int i = 0; i < NC; ++i) {
-        sQ[i * T + t] = l[i];
-        l[i] = sB[i * T + t] + damping * l[i];
+        for (int m = 8; m >= 1; m >>= 1) {
+          ares = fmax(ares, __shfl_xor(ares, m, 16));
+          rres = fmax(rres, __shfl_xor(rres, m, 16));
+        }
+        if (ares <= prox.absAcc || rres <= prox.relAcc) break;
+        prev = lam;
+        lam = solve(rhs + damping * lam);
+        ++it;
       }
-      solveG();
-      ++it;
+      if (row && !delassus) {
+        sL[r * T + cg] = lam;
+        stOut(&glam.p[(int64_t)r * glam.rs + bt * glam.bs], S(lam));
+      }
+      if (r == 0 && prox.iters) prox.iters[bt] = it;
     }
-#pragma unroll
-    for (int i = 0; i < NC; ++i)
-      if (i < nc) {
-        sL[i * T + t] = l[i];
-        stOut(&glam.p[(int64_t)i * glam.rs + bt * glam.bs], S(l[i]));
-      }
-    if (prox.iters) prox.iters[bt] = it;
   }
   __syncthreads();
+  if (mode == 2) return;  // (uniform over the workgroup)
 
   // ---- phase 5: ddq = r + Y lambda, fixed order
   for (int e = t; e < NV * T; e += NT) {

@@ -42,13 +42,22 @@ using LaunchKin = void (*)(hipStream_t, GM, const S* mc, int64_t B, CView<S> q, 
                            View<S> vo, View<S> ao, View<S> J);
 
 // forward dynamics under J ddq + gamma = 0 from the FP64 tree factor of M and nle: ddq,
-// lambda; impulse != 0: impulse dynamics with v_before in `tau` (nle and gamma unused);
-// prox: proximal iterations of constraintDynamics (k_kkt.hpp)
+// lambda; mode 1: impulse dynamics with v_before in `tau` (nle and gamma unused); mode 2:
+// the Delassus matrix into `lambda` (k_kkt.hpp); prox: proximal iterations of
+// constraintDynamics
 struct KktProx;
 template<class S>
 using LaunchKkt = void (*)(hipStream_t, GM, int64_t B, CView<double> U, CView<double> D, int nc, CView<S> J,
-                           CView<S> tau, CView<S> nle, CView<S> gamma, double damping, int impulse, double rcoef,
+                           CView<S> tau, CView<S> nle, CView<S> gamma, double damping, int mode, double rcoef,
                            const KktProx& prox, View<S> ddq, View<S> lambda);
+
+// frictional contact solvers over B problems of nc cones (1..4), FP64 (k_ncp.hpp)
+struct AdmmParams;
+void launchPgs(hipStream_t st, int nc, int64_t B, CView<double> G, CView<double> g, CView<double> mu, View<double> x,
+               double over_relax, int max_it, double abs_prec, double rel_prec, View<double> stats);
+void launchAdmm(hipStream_t st, int nc, int64_t B, CView<double> G, CView<double> g, CView<double> mu, CView<double> R,
+                CView<double> primal_guess, CView<double> dual_guess, const AdmmParams& P, View<double> y,
+                View<double> z, View<double> s, View<double> stats);
 
 struct TopoOps {
   const char* name;

@@ -19,6 +19,7 @@
 #include "k_frames.hpp"
 #include "k_contact.hpp"
 #include "k_kkt.hpp"
+#include "k_ncp.hpp"
 #include "ops.hpp"
 #include "runtime.hpp"
 
@@ -1058,15 +1059,13 @@ void impulseCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int
 // kinematics pass at zero joint acceleration (oMi, local v and a, WORLD joint Jacobian),
 // the constraint rows (k_contact.hpp), CRBA, the FP64 tree factor, nle = RNEA at a = 0,
 // then the KKT kernel with mu as its damping and the proximal iterations.
-template<class S>
-void cdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
-            const void* tau, int64_t ldtau, int ncon, const rbd_rigid_constraint* cons, const rbd_proximal_settings& ps,
-            void* ddq, int64_t ldddq, void* lambda, int64_t ldl, int32_t* iters, uint32_t flags) {
-  constexpr bool f32 = std::is_same_v<S, float>;
-  const int nv = m->nv, nj = m->njoints, nnz = m->ops->nnzU;
-  // constraint table and support flags (bit 0: joint1's support, bit 1: joint2's)
-  std::vector<RigidConstraintDev> tab(ncon);
-  std::vector<uint8_t> sup((size_t)ncon * nv, 0);
+// Device table of the constraints and their support flags (bit 0: the dof is in joint1's
+// support, bit 1: in joint2's); returns the total row count.
+int constraintTable(const rbd_model* m, int ncon, const rbd_rigid_constraint* cons, std::vector<RigidConstraintDev>& tab,
+                    std::vector<uint8_t>& sup) {
+  const int nv = m->nv;
+  tab.assign(ncon, RigidConstraintDev{});
+  sup.assign((size_t)ncon * nv, 0);
   int nrows = 0;
   for (int c = 0; c < ncon; ++c) {
     const rbd_rigid_constraint& k = cons[c];
@@ -1095,6 +1094,18 @@ void cdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t 
       }
     nrows += dim;
   }
+  return nrows;
+}
+
+template<class S>
+void cdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v, int64_t ldv,
+            const void* tau, int64_t ldtau, int ncon, const rbd_rigid_constraint* cons, const rbd_proximal_settings& ps,
+            void* ddq, int64_t ldddq, void* lambda, int64_t ldl, int32_t* iters, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const int nv = m->nv, nj = m->njoints, nnz = m->ops->nnzU;
+  std::vector<RigidConstraintDev> tab;
+  std::vector<uint8_t> sup;
+  const int nrows = constraintTable(m, ncon, cons, tab, sup);
   if (nrows > kKktMaxConstraints) throw Error(RBD_EINVAL, "the constraints' total dimension must be at most 12");
   auto vq = argView<S>(m->nq, B, ldq, flags, "q");
   auto vv = argView<S>(nv, B, ldv, flags, "v");
@@ -1159,6 +1170,80 @@ void cdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t 
       KktProx{ps.max_iter, ps.absolute_accuracy, ps.relative_accuracy, di}, View<S>{da, va.rs, va.bs},
       View<S>{dl, vl.rs, vl.bs});
   finishCall(ctx, flags, st);
+}
+
+// computeDelassusMatrix: kinematics at zero velocity, the constraint rows, CRBA and the FP64
+// tree factor, then the KKT kernel in its Delassus mode with mu as its damping.
+template<class S>
+void delassusCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, int ncon,
+                  const rbd_rigid_constraint* cons, double mu, void* G, int64_t ldG, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const int nv = m->nv, nj = m->njoints, nnz = m->ops->nnzU;
+  std::vector<RigidConstraintDev> tab;
+  std::vector<uint8_t> sup;
+  const int nrows = constraintTable(m, ncon, cons, tab, sup);
+  if (nrows > kKktMaxConstraints) throw Error(RBD_EINVAL, "the constraints' total dimension must be at most 12");
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vG = argView<S>(nrows * nrows, B, ldG, flags, "G");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const bool batchLayout = (flags & RBD_LAYOUT_BATCH) != 0;
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  S* dG = (S*)st.out(G, vG.extent * sizeof(S), ldG != nrows * nrows || batchLayout);
+  auto table = [&](const void* h, size_t bytes) {
+    void* d = ctx->stageBuf(st.slot++, bytes);
+    HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return d;
+  };
+  const RigidConstraintDev* dtab = (const RigidConstraintDev*)table(tab.data(), sizeof(RigidConstraintDev) * ncon);
+  const uint8_t* dsup = (const uint8_t*)table(sup.data(), sup.size());
+  // workspace, batch-fastest: U, D (FP64); oMi, v, a of the joints, the WORLD Jacobian, the
+  // constraint rows and their (unused) drifts, M and one zero (S)
+  const size_t nU = (size_t)(nnz > 0 ? nnz : 1) * B, nD = (size_t)nv * B;
+  const size_t nO = (size_t)12 * (nj - 1) * B, nVA = (size_t)6 * (nj - 1) * B, nJw = (size_t)6 * nv * B;
+  const size_t nJc = (size_t)nrows * nv * B, nGm = (size_t)nrows * B, nM = (size_t)nv * nv * B;
+  double* wU = (double*)ctx->workspace((nU + nD) * 8 + (nO + 2 * nVA + nJw + nJc + nGm + nM + 1) * sizeof(S));
+  double* wD = wU + nU;
+  S* wO = (S*)(wD + nD);
+  S* wV = wO + nO;
+  S* wA = wV + nVA;
+  S* wJw = wA + nVA;
+  S* wJc = wJw + nJw;
+  S* wGm = wJc + nJc;
+  S* wM = wGm + nGm;
+  S* wZ = wM + nM;
+  const S* mc = f32 ? (const S*)m->mc32 : (const S*)m->mc64;
+  const TopoOps& o = *m->ops;
+  const CView<S> cq{dq, vq.rs, vq.bs}, zero{wZ, 0, 0};
+  hipLaunchKernelGGL(k_fill_zero<S>, dim3(1), dim3(64), 0, ctx->stream, wZ, (int64_t)1);
+  const auto kin = f32 ? (LaunchKin<S>)o.kin_f32 : (LaunchKin<S>)o.kin_f64;
+  kin(ctx->stream, m->gen.get(), mc, B, cq, zero, zero, View<S>{wO, B, 1}, View<S>{wV, B, 1}, View<S>{wA, B, 1},
+      View<S>{wJw, B, 1});
+  hipLaunchKernelGGL(k_contact_terms<S>, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, B, nv, ncon,
+                     nrows, dtab, dsup, CView<S>{wO, B, 1}, CView<S>{wV, B, 1}, CView<S>{wA, B, 1},
+                     CView<S>{wJw, B, 1}, View<S>{wJc, B, 1}, View<S>{wGm, B, 1});
+  if constexpr (f32) {
+    o.crba_f32(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f32(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+  } else {
+    o.crba_f64(ctx->stream, m->gen.get(), mc, B, cq, View<S>{wM, B, 1}, 0);
+    o.chol_f64(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
+  }
+  const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
+  kkt(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nrows, CView<S>{wJc, B, 1},
+      CView<S>{}, CView<S>{}, CView<S>{}, mu, 2, 0.0, KktProx{1, 0.0, 0.0, nullptr}, View<S>{},
+      View<S>{dG, vG.rs, vG.bs});
+  finishCall(ctx, flags, st);
+}
+
+static_assert(kNcpStats == RBD_NCP_STATS, "per-problem statistics rows");
+
+// the cone count of a contact problem and its checks
+void checkNcp(rbd_ctx* ctx, int64_t B, int ncones) {
+  RBD_CHECK_ARG(ctx != nullptr, "null context");
+  RBD_CHECK_ARG(B >= 0, "negative batch size");
+  RBD_CHECK_ARG(ncones >= 1 && ncones <= kNcpMaxCones, "between 1 and 4 friction cones per problem");
 }
 }  // namespace
 
@@ -1233,6 +1318,107 @@ int rbd_constraint_dynamics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dty
     else
       cdCall<double>(ctx, m, B, q, ldq, v, ldv, tau, ldtau, ncontacts, contacts, ps, ddq, ldddq, lambda, ldlambda,
                      iterations, flags);
+  });
+}
+
+int rbd_delassus(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, int ncontacts,
+                 const rbd_rigid_constraint* contacts, double mu, void* G, int64_t ldG, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(ncontacts >= 1 && ncontacts <= kKktMaxConstraints, "between 1 and 12 constraints");
+    RBD_CHECK_ARG(contacts != nullptr, "null constraint table");
+    for (int c = 0; c < ncontacts; ++c) {
+      const rbd_rigid_constraint& k = contacts[c];
+      RBD_CHECK_ARG(k.type == RBD_CONTACT_3D || k.type == RBD_CONTACT_6D, "constraint type must be RBD_CONTACT_3D or RBD_CONTACT_6D");
+      RBD_CHECK_ARG(k.joint1_id >= 0 && k.joint1_id < m->njoints && k.joint2_id >= 0 && k.joint2_id < m->njoints,
+                    "The index of the Joint is outside the bounds.");
+      RBD_CHECK_ARG(k.reference_frame == RBD_LOCAL || k.reference_frame == RBD_LOCAL_WORLD_ALIGNED,
+                    "the constraint rows are taken in LOCAL or LOCAL_WORLD_ALIGNED frames");
+    }
+    RBD_CHECK_ARG(mu >= 0.0, "mu has to be positive");
+    RBD_CHECK_ARG(B == 0 || (q && G), "null array argument");
+    if (!m->compact || !m->ops->kkt_f32 || !m->ops->crba_f32 || !m->ops->kin_f32)
+      throw Error(RBD_EUNSUPPORTED, "Delassus matrix: no kernels for this topology");
+    if (dtype == RBD_F32)
+      delassusCall<float>(ctx, m, B, q, ldq, ncontacts, contacts, mu, G, ldG, flags);
+    else
+      delassusCall<double>(ctx, m, B, q, ldq, ncontacts, contacts, mu, G, ldG, flags);
+  });
+}
+
+int rbd_pgs_solve(rbd_ctx* ctx, int64_t B, int ncones, const double* G, int64_t ldG, const double* g, int64_t ldg,
+                  const double* mu, int64_t ldmu, double* x, int64_t ldx, const rbd_pgs_settings* settings,
+                  double* stats, int64_t ldstats, uint32_t flags) {
+  return guarded([&] {
+    checkNcp(ctx, B, ncones);
+    rbd_pgs_settings ps{1000, 1e-6, 1e-6, 1.0};
+    if (settings) ps = *settings;
+    RBD_CHECK_ARG(ps.over_relax < 2.0 && ps.over_relax > 0.0, "over_relax should lie in ]0,2[.");
+    RBD_CHECK_ARG(ps.max_it > 0, "max_it should be greater than 0.");
+    RBD_CHECK_ARG(ps.absolute_precision >= 0.0 && ps.relative_precision >= 0.0, "precisions should be positive.");
+    RBD_CHECK_ARG(B == 0 || (G && g && mu && x && stats), "null array argument");
+    const int n = 3 * ncones;
+    auto vG = argView<double>(n * n, B, ldG, flags, "G");
+    auto vg = argView<double>(n, B, ldg, flags, "g");
+    auto vm = argView<double>(ncones, B, ldmu, flags, "mu");
+    auto vx = argView<double>(n, B, ldx, flags, "x");
+    auto vs = argView<double>(kNcpStats, B, ldstats, flags, "stats");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const double* dG = (const double*)st.in(G, vG.extent * 8);
+    const double* dg = (const double*)st.in(g, vg.extent * 8);
+    const double* dm = (const double*)st.in(mu, vm.extent * 8);
+    double* dx = (double*)st.out(x, vx.extent * 8, true);
+    double* ds = (double*)st.out(stats, vs.extent * 8, true);
+    launchPgs(ctx->stream, ncones, B, CView<double>{dG, vG.rs, vG.bs}, CView<double>{dg, vg.rs, vg.bs},
+              CView<double>{dm, vm.rs, vm.bs}, View<double>{dx, vx.rs, vx.bs}, ps.over_relax, ps.max_it,
+              ps.absolute_precision, ps.relative_precision, View<double>{ds, vs.rs, vs.bs});
+    finishCall(ctx, flags, st);
+  });
+}
+
+int rbd_admm_solve(rbd_ctx* ctx, int64_t B, int ncones, const double* G, int64_t ldG, const double* g, int64_t ldg,
+                   const double* mu, int64_t ldmu, const double* R, int64_t ldR, const double* primal_guess,
+                   const double* dual_guess, int64_t ldguess, const rbd_admm_settings* settings, double* y, double* z,
+                   double* s, int64_t ldout, double* stats, int64_t ldstats, uint32_t flags) {
+  return guarded([&] {
+    checkNcp(ctx, B, ncones);
+    rbd_admm_settings as{1e-6, 0.5, 0.2, 0.05, 10.0, 20, 1000, 1e-6, 1e-6};
+    if (settings) as = *settings;
+    RBD_CHECK_ARG(as.tau <= 1.0 && as.tau > 0.0, "tau should lie in ]0,1].");
+    RBD_CHECK_ARG(as.mu_prox >= 0.0, "mu_prox should be positive.");
+    RBD_CHECK_ARG(as.ratio_primal_dual > 0.0, "The ratio primal/dual should be positive strictly");
+    RBD_CHECK_ARG(as.max_it > 0 && as.max_it_largest_eigen_value >= 1, "max_it should be greater than 0.");
+    RBD_CHECK_ARG(B == 0 || (G && g && mu && R && y && z && stats), "null array argument");
+    const int n = 3 * ncones;
+    auto vG = argView<double>(n * n, B, ldG, flags, "G");
+    auto vg = argView<double>(n, B, ldg, flags, "g");
+    auto vm = argView<double>(ncones, B, ldmu, flags, "mu");
+    auto vR = argView<double>(n, B, ldR, flags, "R");
+    auto vu = argView<double>(n, B, ldguess, flags, "guess");
+    auto vo = argView<double>(n, B, ldout, flags, "outputs");
+    auto vs = argView<double>(kNcpStats, B, ldstats, flags, "stats");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const double* dG = (const double*)st.in(G, vG.extent * 8);
+    const double* dg = (const double*)st.in(g, vg.extent * 8);
+    const double* dm = (const double*)st.in(mu, vm.extent * 8);
+    const double* dR = (const double*)st.in(R, vR.extent * 8);
+    const double* dpg = (const double*)st.in(primal_guess, vu.extent * 8);
+    const double* ddg = (const double*)st.in(dual_guess, vu.extent * 8);
+    double* dy = (double*)st.out(y, vo.extent * 8, true);
+    double* dz = (double*)st.out(z, vo.extent * 8, true);
+    double* dsh = (double*)st.out(s, vo.extent * 8, true);
+    double* ds = (double*)st.out(stats, vs.extent * 8, true);
+    const AdmmParams P{as.mu_prox, as.tau, as.rho_power, as.rho_power_factor, as.ratio_primal_dual,
+                       as.absolute_precision, as.relative_precision, as.max_it, as.max_it_largest_eigen_value};
+    launchAdmm(ctx->stream, ncones, B, CView<double>{dG, vG.rs, vG.bs}, CView<double>{dg, vg.rs, vg.bs},
+               CView<double>{dm, vm.rs, vm.bs}, CView<double>{dR, vR.rs, vR.bs},
+               CView<double>{dpg, vu.rs, vu.bs}, CView<double>{ddg, vu.rs, vu.bs}, P, View<double>{dy, vo.rs, vo.bs},
+               View<double>{dz, vo.rs, vo.bs}, View<double>{dsh, vo.rs, vo.bs}, View<double>{ds, vs.rs, vs.bs});
+    finishCall(ctx, flags, st);
   });
 }
 
