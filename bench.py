@@ -167,6 +167,44 @@ def surrogate_bench(torch, pa, M, ctx, args):
     }
 
 
+def contact_bench(torch, pa, ctx, eng, Qc, B, dev, time_call):
+    """contact problems: the Delassus matrix of four foot points (two per foot, LOCAL) at
+    the near-posture configurations Qc, and batched ADMM / PGS on it (FP64, 12 rows, mu 0.7)"""
+    others = {}
+    corners = [np.concatenate([np.eye(3).ravel(), [x, 0.0, -0.08]]) for x in (0.06, -0.04)]
+    pts = [pa.RigidConstraintModel(pa.CONTACT_3D, j, c, reference_frame=pa.LOCAL) for j in (7, 13) for c in corners]
+    Qc64 = Qc.double()
+    Gd = torch.empty((144, B), dtype=torch.float64, device=dev)
+    t = time_call(lambda: eng.delassus(Qc64, pts, out=Gd, async_=True), reps=10)
+    others["delassus_humanoid36_f64"] = {"batch": B, "constraints": "four foot points (12 rows)",
+                                         "configs_per_s": B / t, "ms": t * 1e3}
+    Gp = Gd.T.contiguous()  # (B, 144): element (i, j) at i + 12 j
+    gen = torch.Generator(device=dev).manual_seed(11)
+    gv = torch.randn((B, 12), dtype=torch.float64, device=dev, generator=gen) * 0.05
+    gv[:, 2::3] -= 0.2  # pressing contacts
+    mus = torch.full((B, 4), 0.7, dtype=torch.float64, device=dev)
+    admm = pa.ADMMContactSolver(ctx, 12)
+    admm.setAbsolutePrecision(1e-8)
+    admm.setMaxIterations(500)
+    t = time_call(lambda: admm.solve(Gp, gv, mus, async_=True), reps=3)
+    its = admm.getIterationCount().float()
+    others["admm_contact_humanoid36"] = {"problems": B, "cones": 4, "ms": t * 1e3, "problems_per_s": B / t,
+                                         "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+                                         "converged_frac": float(admm.stats[:, 3].mean()),
+                                         "absolute_precision": 1e-8}
+    pgs = pa.PGSContactSolver(ctx, 12)
+    pgs.setAbsolutePrecision(1e-8)
+    pgs.setMaxIterations(500)
+    xg = torch.zeros((B, 12), dtype=torch.float64, device=dev)
+    t = time_call(lambda: (xg.zero_(), pgs.solve(Gp, gv, mus, xg, async_=True)), reps=3)
+    its = pgs.getIterationCount().float()
+    others["pgs_contact_humanoid36"] = {"problems": B, "cones": 4, "ms": t * 1e3, "problems_per_s": B / t,
+                                        "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+                                        "converged_frac": float(pgs.stats[:, 3].mean()),
+                                        "absolute_precision": 1e-8}
+    return {"others": others, "Gp": Gp, "g": gv, "mu": mus, "admm": admm, "xg": xg}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,7 +218,7 @@ def main():
     ap.add_argument("--sur-samples", type=int, default=1 << 22)
     ap.add_argument("--sur-epochs", type=int, default=8)
     ap.add_argument("--only", default=None,
-                    help="one part: step|aba|crba|chol|rnea|rnead|abad|abad64|cfd|mix|mlp|surrogate|copy")
+                    help="one part: step|aba|crba|chol|rnea|rnead|abad|abad64|cfd|contact|mix|mlp|surrogate|copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; the batch shards with no "
@@ -329,6 +367,19 @@ def main():
         elapsed = time.perf_counter() - t0
     if only is not None:
         extra = {}
+        if only == "contact":
+            def tc(f, reps=10):
+                f()
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    f()
+                ctx.synchronize()
+                return (time.perf_counter() - t0) / reps
+            qc = M.perturb_configurations(desc, q32[:, 0], B, 0.25, M.INPUT_SEED_BASE + 900).astype(np.float32)
+            cres = contact_bench(torch, pa, ctx, eng, torch.tensor(qc, device=dev).contiguous(), B, dev, tc)
+            print(json.dumps({"only": "contact", **cres["others"]}))
+            return
         if only in ("rnea", "rnead", "abad", "abad64", "cfd"):
             Bd = args.deriv_batch
             Qd, Vd, Ad, Td = Q[:, :Bd].contiguous(), V[:, :Bd].contiguous(), A[:, :Bd].contiguous(), T[:, :Bd].contiguous()
@@ -481,38 +532,9 @@ def main():
         "batch": B, "constraints": "two 6D feet, Baumgarte Kp 10 Kd 2", "configs_per_s": B / t, "ms": t * 1e3,
         "note": "kinematics, CRBA, RNEA(a=0) in FP32; constraint rows, FP64 tree factor and KKT in FP64"}
 
-    # contact problems: the Delassus matrix of four foot points (two per foot, LOCAL) at the
-    # near-posture configurations, and batched ADMM / PGS on it (FP64, 12 rows, mu 0.7)
-    corners = [np.concatenate([np.eye(3).ravel(), [x, 0.0, -0.08]]) for x in (0.06, -0.04)]
-    pts = [pa.RigidConstraintModel(pa.CONTACT_3D, j, c, reference_frame=pa.LOCAL) for j in (7, 13) for c in corners]
-    Qc64 = Qc.double()
-    Gd = torch.empty((144, B), dtype=torch.float64, device=dev)
-    t = time_call(lambda: eng.delassus(Qc64, pts, out=Gd, async_=True), reps=10)
-    others["delassus_humanoid36_f64"] = {"batch": B, "constraints": "four foot points (12 rows)",
-                                         "configs_per_s": B / t, "ms": t * 1e3}
-    Gp = Gd.T.contiguous()  # (B, 144): element (i, j) at i + 12 j
-    gen = torch.Generator(device=dev).manual_seed(11)
-    gv = torch.randn((B, 12), dtype=torch.float64, device=dev, generator=gen) * 0.05
-    gv[:, 2::3] -= 0.2  # pressing contacts
-    mus = torch.full((B, 4), 0.7, dtype=torch.float64, device=dev)
-    admm = pa.ADMMContactSolver(ctx, 12)
-    admm.setAbsolutePrecision(1e-8)
-    admm.setMaxIterations(500)
-    t = time_call(lambda: admm.solve(Gp, gv, mus, async_=True), reps=3)
-    its = admm.getIterationCount().float()
-    conv = admm.stats[:, 3]
-    others["admm_contact_humanoid36"] = {"problems": B, "cones": 4, "ms": t * 1e3, "problems_per_s": B / t,
-                                         "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
-                                         "converged_frac": float(conv.mean()), "absolute_precision": 1e-8}
-    pgs = pa.PGSContactSolver(ctx, 12)
-    pgs.setAbsolutePrecision(1e-8)
-    pgs.setMaxIterations(500)
-    xg = torch.zeros((B, 12), dtype=torch.float64, device=dev)
-    t = time_call(lambda: (xg.zero_(), pgs.solve(Gp, gv, mus, xg, async_=True)), reps=3)
-    its = pgs.getIterationCount().float()
-    others["pgs_contact_humanoid36"] = {"problems": B, "cones": 4, "ms": t * 1e3, "problems_per_s": B / t,
-                                        "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
-                                        "converged_frac": float(pgs.stats[:, 3].mean()), "absolute_precision": 1e-8}
+    cres = contact_bench(torch, pa, ctx, eng, Qc, B, dev, time_call)
+    others.update(cres["others"])
+    Gp, gv, mus, admm, xg = cres["Gp"], cres["g"], cres["mu"], cres["admm"], cres["xg"]
 
     idq = torch.empty_like(cddq)
     iimp = torch.empty_like(clam)
@@ -622,6 +644,16 @@ def main():
         ka, kl, _ = oracle.constraint_dynamics(desc, qc[:, :nd].astype(np.float64), vs[:, :nd], ts[:, :nd], rcon)
         errors["constraint_dynamics_ddq"] = rel_cols(kddq[:, :nd].double().cpu().numpy(), ka)
         errors["constraint_dynamics_lambda"] = rel_cols(klam[:, :nd].double().cpu().numpy(), kl)
+        # ADMM and PGS on the first 1,024 four-foot-point problems (solutions of the timed runs)
+        Gh = Gp[:1024].cpu().numpy().reshape(1024, 12, 12).transpose(0, 2, 1)
+        gh, mh = gv[:1024].cpu().numpy(), mus[:1024].cpu().numpy()
+        ra = oracle.admm_solve(Gh, gh, mh, max_it=500, absolute_precision=1e-8)
+        ya = admm.getPrimalSolution()[:1024].cpu().numpy()
+        errors["admm_contact_y"] = float(np.abs(ya - ra["y"]).max() / np.abs(ra["y"]).max())
+        errors["admm_contact_iterations_equal"] = float((admm.getIterationCount()[:1024].cpu().numpy() ==
+                                                         ra["it"]).mean())
+        rp = oracle.pgs_solve(Gh, gh, mh, max_it=500, absolute_precision=1e-8)
+        errors["pgs_contact_x"] = float(np.abs(xg[:1024].cpu().numpy() - rp["x"]).max() / np.abs(rp["x"]).max())
         rm = oracle.rnea(man, qm[:, :1024], vm[:, :1024], am[:, :1024])
         errors["rnea_manipulator6_f64"] = rel_cols(em.rnea(Qm[:, :1024].contiguous(), Vm[:, :1024].contiguous(),
                                                            Am[:, :1024].contiguous()).cpu().numpy(), rm)
@@ -668,6 +700,20 @@ def main():
         gpu_key["constrained_forward_dynamics"] = "constrained_forward_dynamics_humanoid36_f32"
         for alg, c in cpu_alg.items():
             c["gpu_over_cpu"] = others[gpu_key[alg]]["configs_per_s"] / c["configs_per_s"]
+        # the contact solvers on a sample of the same four-foot-point problems
+        ns = 8192
+        Gh = Gp[:ns].cpu().numpy().reshape(ns, 12, 12).transpose(0, 2, 1)
+        gh, mh = gv[:ns].cpu().numpy(), mus[:ns].cpu().numpy()
+        for name, fn in (("admm_contact", lambda: oracle.admm_solve(Gh, gh, mh, max_it=500, absolute_precision=1e-8,
+                                                                    nthreads=cores)),
+                         ("pgs_contact", lambda: oracle.pgs_solve(Gh, gh, mh, max_it=500, absolute_precision=1e-8,
+                                                                  nthreads=cores))):
+            fn()
+            t0 = time.perf_counter()
+            fn()
+            s_ = time.perf_counter() - t0
+            cpu_alg[name] = {"batch": ns, "problems_per_s": ns / s_, "cores": cores,
+                             "gpu_over_cpu": others[name + "_humanoid36"]["problems_per_s"] / (ns / s_)}
         others["cpu_fp64_oracle"] = cpu_alg
         # ABA followed by computeABADerivatives per configuration, as one pipeline
         g = 1.0 / (1.0 / others["aba_humanoid36_f32"]["configs_per_s"] +

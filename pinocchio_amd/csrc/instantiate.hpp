@@ -121,7 +121,8 @@ template<class TP, class S>
 void launchRneaD(hipStream_t st, GM, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> dq, View<S> dv,
                  View<S> M, int mask, int waves) {
   if (waves <= 0) waves = wavesForBatch(B);
-  if constexpr (rneadSplitPays<TP, 4>()) {
+  // (FP64: one wave per tile; the split's LDS, sized for 4-byte scalars, does not fit)
+  if constexpr (std::is_same_v<S, float> && rneadSplitPays<TP, 4>()) {
     if (waves >= 2) {
       hipLaunchKernelGGL((k_rnead<TP, S, 4>), dim3(gridFor(B)), dim3(kTile * 4), 0, st, mc, B, q, v, a, dq, dv, M,
                          mask);
@@ -187,7 +188,10 @@ template<class TP> void add_chol(TopoOps& o) {
   o.chol32_f32 = &launchChol32<TP>;
   o.nnzU = nnzOf<TP>();
 }
-template<class TP> void add_rnead(TopoOps& o) { o.rnead_f32 = &launchRneaD<TP, float>; }
+template<class TP> void add_rnead(TopoOps& o) {
+  o.rnead_f32 = &launchRneaD<TP, float>;
+  o.rnead_f64 = &launchRneaD<TP, double>;
+}
 template<class TP> void add_abads(TopoOps& o) {
   o.abad_solve_f32 = &launchAbaDSolve<TP, float>;
   if constexpr (abadSolveLdsBytes<TP, double>() <= 160 * 1024) o.abad_solve_f64 = &launchAbaDSolve<TP, double>;

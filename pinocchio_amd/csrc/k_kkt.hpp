@@ -17,6 +17,7 @@
 // FP32 inputs, whose outputs are FP32 too).
 #pragma once
 
+#include "dpp.hpp"
 #include "k_crba.hpp"
 
 namespace rbd {
@@ -212,7 +213,8 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
   // keeps the broadcast L_ed, which is column d of L.  No pivoting and no rank check, like
   // the reference's LLT: a rank-deficient J with zero damping gives non-finite results.
   // Then lambda = G^-1 (-J r - gamma) by two triangular sweeps of broadcasts, and the
-  // proximal iterations with the group's residuals reduced across its lanes.
+  // proximal iterations with the group's residuals reduced across its lanes.  The group is
+  // a DPP row: broadcasts and reductions are VALU operand modifiers (dpp.hpp).
   // mode 2 (Delassus): J M^-1 J^T out instead, or for damping mu > 0
   // (I + mu J M^-1 J^T)^-1 J M^-1 J^T.
   static_assert(NT == 16 * T, "sixteen lanes per configuration");
@@ -237,12 +239,12 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
     }
     static_for<0, NC>([&](auto D) {
       constexpr int d = D;
-      const double rd = 1.0 / sqrt(__shfl(a[d], d, 16));
+      const double rd = 1.0 / sqrt(bcast16<d>(a[d]));
       if (r == d) rid = rd;
       if (r > d) a[d] *= rd;
       static_for<d + 1, NC>([&](auto E) {
         constexpr int e = E;
-        const double le = __shfl(a[d], e, 16);
+        const double le = bcast16<e>(a[d]);
         if (r >= e) a[e] -= a[d] * le;
         if (r == d) lc[e] = le;
       });
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         double s = bv, y = 0.0;
         static_for<0, NC>([&](auto I) {
           constexpr int i = I;
-          const double yi = __shfl(s * rid, i, 16);
+          const double yi = bcast16<i>(s * rid);
           if (r == i) y = yi;
           else if (r > i) s -= a[i] * yi;
         });
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         double x = 0.0;
         static_for<0, NC>([&](auto I) {
           constexpr int i = NC - 1 - I;
-          const double xi = __shfl(s * rid, i, 16);
+          const double xi = bcast16<i>(s * rid);
           if (r == i) x = xi;
           else if (r < i) s -= lc[i] * xi;
         });
@@ -284,14 +286,9 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         double g = gam;
         static_for<0, NC>([&](auto D) {
           constexpr int d = D;
-          g -= pr[d] * __shfl(lam, d, 16);
+          g -= pr[d] * bcast16<d>(lam);
         });
-        double ares = row ? fabs(g) : 0.0, rres = row ? fabs(lam - prev) : 0.0;
-#pragma unroll
-        for (int m = 8; m >= 1; m >>= 1) {
-          ares = fmax(ares, __shfl_xor(ares, m, 16));
-          rres = fmax(rres, __shfl_xor(rres, m, 16));
-        }
+        const double ares = max16(row ? fabs(g) : 0.0), rres = max16(row ? fabs(lam - prev) : 0.0);
         if (ares <= prox.absAcc || rres <= prox.relAcc) break;
         prev = lam;
         lam = solve(rhs + damping * lam);

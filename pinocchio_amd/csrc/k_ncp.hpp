@@ -9,10 +9,10 @@
 // 249-270 and contact-solver-utils.hpp:25-110.
 //
 // PGS is Gauss-Seidel over the cones, so the work within one problem is a chain of cone
-// steps.  Four lanes hold a problem: lane k keeps the three rows of G of cone k in
+// steps.  A quad of lanes holds a problem: lane k keeps the three rows of G of cone k in
 // registers and every lane a copy of x; at cone k's step lane k forms the three velocity
 // rows, updates the cone (normal, then tangential with the disc projection) and broadcasts
-// its three new components.  64 problems per 256-thread workgroup.
+// its three new components (DPP quad broadcasts).  64 problems per 256-thread workgroup.
 //
 // ADMM spends its time in the dense solves and products, which spread over rows: sixteen
 // lanes hold a problem, lane r row r of G (rows n..15 ride along with a unit diagonal).
@@ -20,12 +20,15 @@
 // G + diag(R + mu_prox + tau rho) right-looking with the pivot's column broadcast (lane r
 // keeps row r and column r of the factor), the two triangular sweeps as chains of
 // broadcasts, the cone projections on each cone's three lanes, and the residuals reduced
-// across the sixteen lanes.  16 problems per 256-thread workgroup.
+// across the sixteen lanes.  All cross-lane moves are DPP row broadcasts and mirrors
+// (dpp.hpp): VALU operand modifiers, where LDS permutes would put an LDS round trip on
+// every step of the chains.  16 problems per 256-thread workgroup.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dpp.hpp"
 #include "kernels_common.hpp"
 
 namespace rbd {
@@ -160,13 +163,10 @@ __global__ __launch_bounds__(256) void k_pgs(int64_t B, CView<double> gG, CView<
         df = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
       }
 #pragma unroll
-      for (int e = 0; e < 3; ++e) x[r0 + e] = __shfl(x[r0 + e], c, kPgsLanes);
+      for (int e = 0; e < 3; ++e) x[r0 + e] = bcast4<c>(x[r0 + e]);
     });
-#pragma unroll
-    for (int m = 1; m < kPgsLanes; m <<= 1) {
-      cc = fmax(cc, __shfl_xor(cc, m, kPgsLanes));
-      df = fmax(df, __shfl_xor(df, m, kPgsLanes));
-    }
+    cc = max4(cc);
+    df = max4(df);
     compl_ = cc;
     dualf = df;
     absR = compl_ <= abs_prec && dualf <= abs_prec;
@@ -197,21 +197,6 @@ __global__ __launch_bounds__(256) void k_pgs(int64_t B, CView<double> gG, CView<
 // ---------------------------------------------------------------------------------------
 constexpr int kAdmmLanes = 16;
 
-__device__ __forceinline__ double gsum16(double v) {
-#pragma unroll
-  for (int m = 1; m < kAdmmLanes; m <<= 1) v += __shfl_xor(v, m, kAdmmLanes);
-  return v;
-}
-__device__ __forceinline__ double gmax16(double v) {
-#pragma unroll
-  for (int m = 1; m < kAdmmLanes; m <<= 1) v = fmax(v, __shfl_xor(v, m, kAdmmLanes));
-  return v;
-}
-__device__ __forceinline__ double gmin16(double v) {
-#pragma unroll
-  for (int m = 1; m < kAdmmLanes; m <<= 1) v = fmin(v, __shfl_xor(v, m, kAdmmLanes));
-  return v;
-}
 
 // gG: n^2 rows; gg, gR: n rows; gmu: nc rows; gpg / gdg: initial primal / dual guesses (n
 // rows, p == nullptr for none); outputs gy (primal solution), gz (dual solution), gs (the
@@ -236,15 +221,21 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
   // (G v)_r for v spread one entry per lane
   auto matvec = [&](double v) {
     double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < N; ++j) s += grow[j] * __shfl(v, j, kAdmmLanes);
+    static_for<0, N>([&](auto J) { s += grow[J] * bcast16<J>(v); });
     return s;
   };
-  // this lane's component of the cone projection of v (one entry per lane)
+  // the three entries of this lane's cone (lanes of other cones: their own cone's)
   auto triplet = [&](double v, double (&o)[3]) {
-    o[0] = __shfl(v, base, kAdmmLanes);
-    o[1] = __shfl(v, base + 1, kAdmmLanes);
-    o[2] = __shfl(v, base + 2, kAdmmLanes);
+    o[0] = o[1] = o[2] = 0.0;  // (rows past the problem)
+    static_for<0, NC>([&](auto C) {
+      constexpr int c = C;
+      const double t0 = bcast16<3 * c>(v), t1 = bcast16<3 * c + 1>(v), t2 = bcast16<3 * c + 2>(v);
+      if (base == 3 * c) {
+        o[0] = t0;
+        o[1] = t1;
+        o[2] = t2;
+      }
+    });
   };
   auto projK = [&](double v) {
     double tv[3], o[3];
@@ -265,16 +256,16 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
   };
   // largest eigenvalue of G (power iteration from 1 / sqrt(n), relative tolerance 1e-8)
   double pv = row ? 1.0 / sqrt(double(N)) : 0.0;
-  double est = sqrt(gsum16(pv * pv));
+  double est = sqrt(sum16(pv * pv));
   for (int k = 0; k < P.max_it_eig; ++k) {
     const double prev = est;
     pv /= est;
     pv = matvec(pv);
-    est = sqrt(gsum16(pv * pv));
+    est = sqrt(sum16(pv * pv));
     if (fabs(prev - est) <= 1e-8 * fmax(fabs(prev), fabs(est))) break;
   }
   const double L = est;
-  const double muR = gmin16(row ? R : 1.0e300);
+  const double muR = -max16(row ? -R : -1.0e300);
   const double m = P.mu_prox + muR;
   const double cond = L / m;
   const double rhoInc = pow(cond, P.rho_power_factor);
@@ -290,12 +281,12 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
     if (!row) rid = 1.0;
     static_for<0, N>([&](auto D) {
       constexpr int d = D;
-      const double rd = 1.0 / sqrt(__shfl(a[d], d, kAdmmLanes));
+      const double rd = 1.0 / sqrt(bcast16<d>(a[d]));
       if (r == d) rid = rd;
       if (r > d) a[d] *= rd;
       static_for<d + 1, N>([&](auto E) {
         constexpr int e = E;
-        const double le = __shfl(a[d], e, kAdmmLanes);
+        const double le = bcast16<e>(a[d]);
         if (r >= e) a[e] -= a[d] * le;
         if (r == d) lc[e] = le;
       });
@@ -305,7 +296,7 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
     double s = bv, y = 0.0;
     static_for<0, N>([&](auto I) {
       constexpr int i = I;
-      const double yi = __shfl(s * rid, i, kAdmmLanes);
+      const double yi = bcast16<i>(s * rid);
       if (r == i) y = yi;
       else if (r > i) s -= a[i] * yi;
     });
@@ -313,7 +304,7 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
     double x = 0.0;
     static_for<0, N>([&](auto I) {
       constexpr int i = N - 1 - I;
-      const double xi = __shfl(s * rid, i, kAdmmLanes);
+      const double xi = bcast16<i>(s * rid);
       if (r == i) x = xi;
       else if (r < i) s -= lc[i] * xi;
     });
@@ -344,13 +335,13 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
     y = projK(x - z / (P.tau * rho));
     z -= (P.tau * rho) * (x - y);
     const double dy = y - yp;
-    primal = gmax16(fabs(x - y));
-    prox = gmax16(fabs(dy));
-    dual = gmax16(fabs((P.tau * rho) * dy + P.mu_prox * (x - xp)));
+    primal = max16(fabs(x - y));
+    prox = max16(fabs(dy));
+    dual = max16(fabs((P.tau * rho) * dy + P.mu_prox * (x - xp)));
     double zy[3], yy[3];
     triplet(z, zy);
     triplet(y, yy);
-    compl_ = gmax16(row ? fabs(zy[0] * yy[0] + zy[1] * yy[1] + zy[2] * yy[2]) : 0.0);
+    compl_ = max16(row ? fabs(zy[0] * yy[0] + zy[1] * yy[1] + zy[2] * yy[2]) : 0.0);
     absR = compl_ <= P.abs_prec && dual <= P.abs_prec && primal <= P.abs_prec;
     if (absR) break;
     bool refactor = false;
