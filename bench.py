@@ -192,6 +192,15 @@ def contact_bench(torch, pa, ctx, eng, Qc, B, dev, time_call):
                                          "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
                                          "converged_frac": float(admm.stats[:, 3].mean()),
                                          "absolute_precision": 1e-8}
+    # the same solver held to exactly 100 iterations (precisions 0): the cost of an
+    # iteration without the spread of iteration counts within a wave
+    fixed = pa.ADMMContactSolver(ctx, 12)
+    fixed.setAbsolutePrecision(0.0)
+    fixed.setRelativePrecision(0.0)
+    fixed.setMaxIterations(100)
+    t = time_call(lambda: fixed.solve(Gp, gv, mus, async_=True), reps=3)
+    others["admm_contact_humanoid36"]["fixed_100_iterations_ms"] = t * 1e3
+    others["admm_contact_humanoid36"]["ns_per_problem_iteration_fixed"] = t / (B * 100) * 1e9
     pgs = pa.PGSContactSolver(ctx, 12)
     pgs.setAbsolutePrecision(1e-8)
     pgs.setMaxIterations(500)

@@ -9,6 +9,8 @@
 //                   each step combines a lane with a partner holding a disjoint set, and
 //                   the combinations are commutative, so every lane ends with the same bits
 //   sum4 / max4     the same over a quad (the first two steps)
+//   triple3(x, k)   the three values of lanes i - k .. i - k + 2 (k = 0, 1, 2: the lane's
+//                   place in a group of three within its row), by row shifts
 //
 // The whole row (quad) must be active: callers keep control flow uniform per problem.
 #pragma once
@@ -24,6 +26,8 @@ constexpr int kQuadXor2 = 0x4E;       // quad_perm [2, 3, 0, 1]
 constexpr int kRowMirror = 0x140;
 constexpr int kRowHalfMirror = 0x141;
 constexpr int kRowNewBcast = 0x150;   // + lane within the row
+constexpr int kRowShl1 = 0x101, kRowShl2 = 0x102;  // lane i reads lane i + 1, i + 2 of its row
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112;  // lane i reads lane i - 1, i - 2
 constexpr int quadBcast(int e) { return e | (e << 2) | (e << 4) | (e << 6); }
 
 template<int CTRL> __device__ __forceinline__ double mov(double x) {
@@ -49,6 +53,15 @@ __device__ __forceinline__ double sum4(double v) {
 __device__ __forceinline__ double max4(double v) {
   v = fmax(v, dpp::mov<dpp::kQuadXor1>(v));
   return fmax(v, dpp::mov<dpp::kQuadXor2>(v));
+}
+// o = (x of lane i - k, i - k + 1, i - k + 2): the lane's own triple of consecutive lanes
+// whose first lane sits k before it (the groups must not cross a row boundary)
+__device__ __forceinline__ void triple3(double x, int k, double (&o)[3]) {
+  const double l1 = dpp::mov<dpp::kRowShl1>(x), l2 = dpp::mov<dpp::kRowShl2>(x);
+  const double r1 = dpp::mov<dpp::kRowShr1>(x), r2 = dpp::mov<dpp::kRowShr2>(x);
+  o[0] = k == 0 ? x : (k == 1 ? r1 : r2);
+  o[1] = k == 0 ? l1 : (k == 1 ? x : r1);
+  o[2] = k == 0 ? l2 : (k == 1 ? l1 : x);
 }
 __device__ __forceinline__ double sum16(double v) {
   v = sum4(v);

@@ -240,34 +240,35 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
     static_for<0, NC>([&](auto D) {
       constexpr int d = D;
       const double rd = 1.0 / sqrt(bcast16<d>(a[d]));
-      if (r == d) rid = rd;
-      if (r > d) a[d] *= rd;
+      rid = r == d ? rd : rid;
+      a[d] = r > d ? a[d] * rd : a[d];
       static_for<d + 1, NC>([&](auto E) {
         constexpr int e = E;
         const double le = bcast16<e>(a[d]);
-        if (r >= e) a[e] -= a[d] * le;
-        if (r == d) lc[e] = le;
+        a[e] = r >= e ? a[e] - a[d] * le : a[e];
+        lc[e] = r == d ? le : lc[e];
       });
     });
+    // the diagonal lives in rid; zero there, row r of a is strictly lower and column r in lc
+    // strictly below the diagonal, so each sweep step updates every lane alike
+#pragma unroll
+    for (int e = 0; e < NC; ++e) a[e] = e == r ? 0.0 : a[e];
     {
-      // G^-1 b for b spread one row per lane: L y = b, then L^T x = y
+      // G^-1 b for b spread one row per lane: L y = b, then L^T x = y (a lane's value stops
+      // changing once the sweep has passed its row: y_r and x_r are its final value
+      // times 1 / L_rr)
       auto solve = [&](double bv) {
-        double s = bv, y = 0.0;
+        double s = bv;
         static_for<0, NC>([&](auto I) {
           constexpr int i = I;
-          const double yi = bcast16<i>(s * rid);
-          if (r == i) y = yi;
-          else if (r > i) s -= a[i] * yi;
+          s -= a[i] * bcast16<i>(s * rid);
         });
-        s = y;
-        double x = 0.0;
+        s *= rid;
         static_for<0, NC>([&](auto I) {
           constexpr int i = NC - 1 - I;
-          const double xi = bcast16<i>(s * rid);
-          if (r == i) x = xi;
-          else if (r < i) s -= lc[i] * xi;
+          s -= lc[i] * bcast16<i>(s * rid);
         });
-        return x;
+        return s * rid;
       };
       if (delassus) {
         // column j of the output: the Delassus column (mu = 0), or (I + mu G)^-1 of it

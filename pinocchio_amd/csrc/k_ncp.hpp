@@ -225,18 +225,7 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
     return s;
   };
   // the three entries of this lane's cone (lanes of other cones: their own cone's)
-  auto triplet = [&](double v, double (&o)[3]) {
-    o[0] = o[1] = o[2] = 0.0;  // (rows past the problem)
-    static_for<0, NC>([&](auto C) {
-      constexpr int c = C;
-      const double t0 = bcast16<3 * c>(v), t1 = bcast16<3 * c + 1>(v), t2 = bcast16<3 * c + 2>(v);
-      if (base == 3 * c) {
-        o[0] = t0;
-        o[1] = t1;
-        o[2] = t2;
-      }
-    });
-  };
+  auto triplet = [&](double v, double (&o)[3]) { triple3(v, comp, o); };
   auto projK = [&](double v) {
     double tv[3], o[3];
     triplet(v, tv);
@@ -279,36 +268,39 @@ __global__ __launch_bounds__(256) void k_admm(int64_t B, CView<double> gG, CView
       lc[e] = 0.0;
     }
     if (!row) rid = 1.0;
+    // (selects, not branches: every lane runs every step)
     static_for<0, N>([&](auto D) {
       constexpr int d = D;
       const double rd = 1.0 / sqrt(bcast16<d>(a[d]));
-      if (r == d) rid = rd;
-      if (r > d) a[d] *= rd;
+      rid = r == d ? rd : rid;
+      a[d] = r > d ? a[d] * rd : a[d];
       static_for<d + 1, N>([&](auto E) {
         constexpr int e = E;
         const double le = bcast16<e>(a[d]);
-        if (r >= e) a[e] -= a[d] * le;
-        if (r == d) lc[e] = le;
+        a[e] = r >= e ? a[e] - a[d] * le : a[e];
+        lc[e] = r == d ? le : lc[e];
       });
     });
+    // the diagonal lives in rid: a zero there lets the sweeps update every lane alike
+#pragma unroll
+    for (int e = 0; e < N; ++e) a[e] = e == r ? 0.0 : a[e];
   };
+  // (a: strictly lower row, zero from the diagonal on; lc: strictly lower column, zero
+  // down to the diagonal -- so each step updates every lane with one multiply-add, and a
+  // lane's value stops changing once the sweep has passed its row: y_r and x_r are its
+  // final value times 1 / L_rr)
   auto solve = [&](double bv) {
-    double s = bv, y = 0.0;
+    double s = bv;
     static_for<0, N>([&](auto I) {
       constexpr int i = I;
-      const double yi = bcast16<i>(s * rid);
-      if (r == i) y = yi;
-      else if (r > i) s -= a[i] * yi;
+      s -= a[i] * bcast16<i>(s * rid);
     });
-    s = y;
-    double x = 0.0;
+    s *= rid;
     static_for<0, N>([&](auto I) {
       constexpr int i = N - 1 - I;
-      const double xi = bcast16<i>(s * rid);
-      if (r == i) x = xi;
-      else if (r < i) s -= lc[i] * xi;
+      s -= lc[i] * bcast16<i>(s * rid);
     });
-    return row ? x : 0.0;
+    return row ? s * rid : 0.0;
   };
   double proxValue = P.mu_prox + P.tau * rho;
   factor(proxValue);
