@@ -22,6 +22,12 @@
  *                         getFrameJacobian          algorithm/frames.hxx:156-175
  *   rbd_constraint_dynamics constraintDynamics      algorithm/constrained-dynamics.hxx:193-500
  *   rbd_delassus          computeDelassusMatrix     algorithm/delassus.hpp:41-60, delassus.hxx:118-300
+ *   rbd_damped_delassus_inverse
+ *                         computeDampedDelassusMatrixInverse
+ *                                                   algorithm/delassus.hpp:84-102, delassus.hxx:800-836
+ *   rbd_contact_cholesky_solve
+ *                         ContactCholeskyDecomposition compute + solve
+ *                                                   algorithm/contact-cholesky.hpp:292-400
  *   rbd_pgs_solve         PGSContactSolverTpl::solve algorithm/pgs-solver.hpp:16-59, pgs-solver.hxx:19-141
  *   rbd_admm_solve        ADMMContactSolverTpl::solve algorithm/admm-solver.hpp:120-316, admm-solver.hxx:22-335
  *   rbd_impulse_dynamics  impulseDynamics(q, v_before, J, r_coeff, inv_damping)
@@ -302,6 +308,24 @@ int rbd_constraint_dynamics(rbd_ctx* ctx, const rbd_model* model, int64_t batch,
  * the factor and the products are FP64.  Prebuilt topologies only. */
 int rbd_delassus(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
                  int ncontacts, const rbd_rigid_constraint* contacts, double mu, void* G, int64_t ldG, uint32_t flags);
+
+/* computeDampedDelassusMatrixInverse(model, data, q, contact_models, contact_datas, Ginv, mu,
+ * scaled) (delassus.hpp:84-102, delassus.hxx:800-836) per configuration: (J M^-1 J^T + mu I)^-1
+ * (mu > 0), times mu when `scaled` is nonzero.  Layout, precision and constraints as for
+ * rbd_delassus. */
+int rbd_damped_delassus_inverse(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                                int64_t ldq, int ncontacts, const rbd_rigid_constraint* contacts, double mu, int scaled,
+                                void* Ginv, int64_t ldG, uint32_t flags);
+
+/* ContactCholeskyDecomposition::compute(model, data, contact_models, contact_datas, mu) then
+ * solve(rhs) (contact-cholesky.hpp:292-400) per configuration: x = K^-1 rhs with the KKT
+ * matrix K = [[-mu I, J], [J^T, M]] at q (J the constraints' Jacobian as in
+ * rbd_constraint_dynamics, M from CRBA).  rhs and x hold nrows + nv rows per configuration,
+ * the constraint rows first.  Solved through the Schur complement J M^-1 J^T + mu I (the
+ * reference factors K itself); the factor and the solve are FP64. */
+int rbd_contact_cholesky_solve(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                               int64_t ldq, int ncontacts, const rbd_rigid_constraint* contacts, double mu,
+                               const void* rhs, int64_t ldrhs, void* x, int64_t ldx, uint32_t flags);
 
 /* Frictional contact problems over Coulomb friction cones (constraints/coulomb-friction-
  * cone.hpp:20-290), batched: `batch` independent problems of `ncones` cones (1..4, problem

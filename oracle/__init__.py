@@ -428,3 +428,18 @@ def admm_solve(G, g, mu, R=None, primal_guess=None, dual_guess=None, mu_prox=1e-
     return dict(y=y, z=z, s=s, it=it, absolute_residual=stats[:, 0], relative_residual=stats[:, 1], rho=stats[:, 2],
                 rho_power=stats[:, 3], largest_eigen_value=stats[:, 4], cholesky_update_count=chol,
                 converged=conv.astype(bool))
+
+
+def contact_cholesky_solve(model, q, constraints, mu, rhs, scalar="f64", nthreads=0):
+    """ContactCholeskyDecomposition compute + solve per column (contact-cholesky.hpp:292-400):
+    x = K^-1 rhs with K = [[-mu I, J], [J^T, M]]; rhs (nrows + nv, B).  Returns x (same shape)."""
+    q = _cols(q, model.nq)
+    B = q.shape[1]
+    arr = constraint_table(constraints)
+    nrows = sum(6 if c["type"] == CONTACT_6D else 3 for c in constraints)
+    r = _cols(rhs, nrows + model.nv)
+    x = np.zeros((nrows + model.nv, B), order="F")
+    d = model.cstruct()
+    _check(lib().orc_contact_cholesky_solve(ctypes.byref(d), ctypes.c_int64(B), _p(q), len(constraints), arr,
+                                            ctypes.c_double(mu), _p(r), _p(x), SCALARS[scalar], nthreads))
+    return np.ascontiguousarray(x)

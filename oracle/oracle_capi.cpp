@@ -807,3 +807,44 @@ extern "C" int orc_admm_solve(int64_t B, int nc, const double* G, const double* 
     return 1;
   }
 }
+
+// ContactCholeskyDecomposition compute + solve per configuration (dense restatement): rhs and
+// x hold nrows + nv entries per configuration, back to back
+namespace {
+template<class S>
+int kktT(const ModelDesc* d, int64_t B, const double* q, int ncon, const OrcConstraint* oc, double mu,
+         const double* rhs, double* x, int nt) {
+  int nrows = 0;
+  const std::vector<RigidConstraint<S>> cons = toConstraints<S>(ncon, oc, nrows);
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    for (const auto& c : cons)
+      if (c.joint1 < 0 || c.joint1 >= m.njoints || c.joint2 < 0 || c.joint2 >= m.njoints)
+        throw std::invalid_argument("constraint joint out of range");
+    const int n = nrows + m.nv;
+    static thread_local Buf<S> bq, br;
+    static thread_local std::vector<S> xs;
+    xs.resize(n);
+    contactKktSolve(m, data, bq.load(q + b * m.nq, m.nq), ncon, cons.data(), S(mu), br.load(rhs + b * n, n),
+                    xs.data());
+    store(xs.data(), x + b * n, n);
+  });
+}
+}  // namespace
+
+extern "C" int orc_contact_cholesky_solve(const ModelDesc* d, int64_t B, const double* q, int ncon,
+                                          const OrcConstraint* cons, double mu, const double* rhs, double* x,
+                                          int scalar, int nthreads) {
+  try {
+    if (ncon < 1 || !cons) throw std::invalid_argument("at least one constraint");
+    switch (scalar) {
+      case 0: return kktT<double>(d, B, q, ncon, cons, mu, rhs, x, nthreads);
+      case 1: return kktT<float>(d, B, q, ncon, cons, mu, rhs, x, nthreads);
+      case 2: return kktT<long double>(d, B, q, ncon, cons, mu, rhs, x, nthreads);
+    }
+    g_err = "bad scalar";
+    return 1;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}

@@ -1335,6 +1335,51 @@ inline void delassusMatrix(const Model<S>& model, Data<S>& data, const S* q, int
   }
 }
 
+// A solve with the KKT matrix of the contact Cholesky (contact-cholesky.hpp:292-400),
+// K = [[-mu I, J], [J^T, M]] at q: x = K^-1 rhs, rhs and x of nrows + nv entries (the
+// constraint rows first).  Restated densely -- K assembled from the constraint Jacobian and
+// CRBA's M, then Gaussian elimination with partial pivoting -- where the reference factors K
+// with its sparsity and the GPU path goes through the Schur complement.
+template<class S>
+inline void contactKktSolve(const Model<S>& model, Data<S>& data, const S* q, int ncon, const RigidConstraint<S>* cons,
+                            S mu, const S* rhs, S* x) {
+  using std::abs;
+  const int nv = model.nv;
+  int nrows = 0;
+  for (int c = 0; c < ncon; ++c) nrows += cons[c].size();
+  const int n = nrows + nv;
+  std::vector<S> Jc((size_t)nrows * nv), zero(nv, S(0)), K((size_t)n * n, S(0)), b(rhs, rhs + n);
+  constraintForward(model, data, q, zero.data(), ncon, cons, nrows, (S*)nullptr, Jc.data());
+  crbaWorld(model, data, q);
+  auto at = [&](int i, int j) -> S& { return K[(size_t)i * n + j]; };  // row major
+  for (int i = 0; i < nrows; ++i) {
+    at(i, i) = -mu;
+    for (int k = 0; k < nv; ++k) at(i, nrows + k) = at(nrows + k, i) = Jc[i + (size_t)k * nrows];
+  }
+  for (int i = 0; i < nv; ++i)
+    for (int j = 0; j < nv; ++j) at(nrows + i, nrows + j) = i <= j ? data.M(i, j) : data.M(j, i);
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (abs(at(r, c)) > abs(at(p, c))) p = r;
+    if (p != c) {
+      for (int j = 0; j < n; ++j) std::swap(at(c, j), at(p, j));
+      std::swap(b[c], b[p]);
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const S f = at(r, c) / at(c, c);
+      if (f == S(0)) continue;
+      for (int j = c; j < n; ++j) at(r, j) -= f * at(c, j);
+      b[r] -= f * b[c];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    S s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= at(i, j) * x[j];
+    x[i] = s / at(i, i);
+  }
+}
+
 template<class S>
 inline void constraintDynamics(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* tau, int ncon,
                                const RigidConstraint<S>* cons, const ProximalSettings<S>& settings, S* ddq, S* lambda,

@@ -494,6 +494,38 @@ class Engine:
                                   ctypes.c_uint32(self._flags(async_))))
         return out
 
+    def damped_delassus_inverse(self, q, constraints, mu, scaled=False, out=None, async_=False):
+        """computeDampedDelassusMatrixInverse(model, data, q, contact_models, contact_datas, Ginv,
+        mu, scaled) per configuration (delassus.hxx:800-836): (J M^-1 J^T + mu I)^-1, times mu
+        when scaled; layout as delassus()."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        table = _constraint_table(constraints)
+        nrows = sum(c.size() for c in constraints)
+        out = self._new_vec(q, nrows * nrows, B) if out is None else out
+        _, ldG = self._vec(out, nrows * nrows, "Ginv")
+        _check(lib().rbd_damped_delassus_inverse(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                                 len(constraints), table, ctypes.c_double(mu), int(bool(scaled)),
+                                                 _ptr(out), _i64(ldG), ctypes.c_uint32(self._flags(async_))))
+        return out
+
+    def contact_cholesky_solve(self, q, constraints, mu, rhs, out=None, async_=False):
+        """ContactCholeskyDecomposition compute(model, data, contact_models, contact_datas, mu)
+        then solve(rhs) per configuration (contact-cholesky.hpp:292-400): x = K^-1 rhs with
+        K = [[-mu I, J], [J^T, M]]; rhs and x have nrows + nv rows per configuration, the
+        constraint rows first."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        table = _constraint_table(constraints)
+        nrows = sum(c.size() for c in constraints)
+        _, ldr = self._vec(rhs, nrows + m.nv, "rhs")
+        out = self._new_vec(q, nrows + m.nv, B) if out is None else out
+        _, ldx = self._vec(out, nrows + m.nv, "x")
+        _check(lib().rbd_contact_cholesky_solve(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                                len(constraints), table, ctypes.c_double(mu), _ptr(rhs), _i64(ldr),
+                                                _ptr(out), _i64(ldx), ctypes.c_uint32(self._flags(async_))))
+        return out
+
     @staticmethod
     def mats_to_numpy(t, layout, nv):
         """Device matrix output -> (B, nv, nv) numpy array with [b] the configuration's matrix."""

@@ -63,6 +63,14 @@ struct KktProx {
 // 118-300, with damping = its mu): J (M + mu J^T J)^-1 J^T = (I + mu D0)^-1 D0 with
 // D0 = J M^-1 J^T, written to glam (element (i, j) at row i + j nc); gtau, gnle, gg and
 // gddq unused.
+// mode 3: a solve with the KKT matrix [[-mu I, J], [J^T, M]] (the matrix of the contact
+// Cholesky, contact-cholesky.hpp:376-400, mu = damping): gg holds the constraint rows b1
+// of the right-hand side and gtau its joint rows b2 (gnle zero); glam receives x1 =
+// (D0 + mu I)^-1 (J M^-1 b2 - b1) and gddq x2 = M^-1 (b2 - J^T x1) -- mode 0 with
+// gamma = -b1 and lambda = -x1.
+// mode 4: the damped Delassus inverse (computeDampedDelassusMatrixInverse, delassus.hxx:
+// 800-836): (D0 + mu I)^-1, times rcoef (1, or mu for the reference's `scaled`), into glam
+// as in mode 2.
 template<class TP, class S>
 __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<double> gU, CView<double> gD, int nc,
                                                             CView<S> gJ, CView<S> gtau, CView<S> gnle, CView<S> gg,
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
   const int L = t & 63, w = t >> 6;
   const int c = L % T;
   const int j = w * K::RPW + L / T;
-  const bool active = (j < nc || (j == nc && mode != 2)) && c < nb;
+  const bool active = (j < nc || (j == nc && mode != 2 && mode != 4)) && c < nb;
   const int64_t b = b0 + c;
   double x[NV];
   double p[NC];
@@ -222,19 +230,20 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
     const int r = t & 15, cg = t >> 4;
     const bool row = r < nc;
     const int64_t bt = b0 + cg;
-    const bool delassus = mode == 2;
+    const bool delassus = mode == 2 || mode == 4;  // matrices out, no right-hand side
     double a[NC], pr[NC], lc[NC];
     double rhs = 0.0, gam = 0.0, rid = 1.0;
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       pr[e] = row && e < nc ? sP[(r + e * NC) * T + cg] : 0.0;
-      const double ge = delassus ? (e == r ? 1.0 : 0.0) + damping * pr[e] : pr[e] + (e == r ? damping : 0.0);
+      const double ge = mode == 2 ? (e == r ? 1.0 : 0.0) + damping * pr[e] : pr[e] + (e == r ? damping : 0.0);
       a[e] = row ? (e <= r ? ge : 0.0) : (e == r ? 1.0 : 0.0);
       lc[e] = 0.0;
     }
     if (row && !delassus) {
+      const double grow = mode == 1 ? 0.0 : double(gg.p[(int64_t)r * gg.rs + bt * gg.bs]);
       rhs = mode == 1 ? -(1.0 + rcoef) * sP[(r + nc * NC) * T + cg]
-                      : -sP[(r + nc * NC) * T + cg] - double(gg.p[(int64_t)r * gg.rs + bt * gg.bs]);
+                      : -sP[(r + nc * NC) * T + cg] + (mode == 3 ? grow : -grow);
       if (prox.maxIter > 1) gam = double(gg.p[(int64_t)r * gg.rs + bt * gg.bs]);
     }
     static_for<0, NC>([&](auto D) {
@@ -271,12 +280,14 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
         return s * rid;
       };
       if (delassus) {
-        // column j of the output: the Delassus column (mu = 0), or (I + mu G)^-1 of it
+        // column j of the output: the Delassus column (mu = 0) or (I + mu G)^-1 of it
+        // (mode 2); column j of (G + mu I)^-1, scaled (mode 4)
         for (int j = 0; j < nc; ++j) {
           double gj = 0.0;
 #pragma unroll
           for (int e = 0; e < NC; ++e) gj = e == j ? pr[e] : gj;
-          const double xj = damping > 0.0 ? solve(gj) : gj;
+          const double bj = mode == 4 ? (r == j ? 1.0 : 0.0) : gj;
+          const double xj = mode == 4 ? rcoef * solve(bj) : (damping > 0.0 ? solve(bj) : bj);
           if (row) stOut(&glam.p[(int64_t)(r + j * nc) * glam.rs + bt * glam.bs], S(xj));
         }
       }
@@ -297,13 +308,13 @@ __global__ __launch_bounds__(64 * KktShape<S>::W) void k_kkt(int64_t B, CView<do
       }
       if (row && !delassus) {
         sL[r * T + cg] = lam;
-        stOut(&glam.p[(int64_t)r * glam.rs + bt * glam.bs], S(lam));
+        stOut(&glam.p[(int64_t)r * glam.rs + bt * glam.bs], S(mode == 3 ? -lam : lam));
       }
       if (r == 0 && prox.iters) prox.iters[bt] = it;
     }
   }
   __syncthreads();
-  if (mode == 2) return;  // (uniform over the workgroup)
+  if (mode == 2 || mode == 4) return;  // (uniform over the workgroup)
 
   // ---- phase 5: ddq = r + Y lambda, fixed order
   for (int e = t; e < NV * T; e += NT) {

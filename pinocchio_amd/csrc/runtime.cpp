@@ -1172,25 +1172,33 @@ void cdCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t 
   finishCall(ctx, flags, st);
 }
 
-// computeDelassusMatrix: kinematics at zero velocity, the constraint rows, CRBA and the FP64
-// tree factor, then the KKT kernel in its Delassus mode with mu as its damping.
+// The constraints' matrices at q: kinematics at zero velocity, the constraint rows, CRBA and
+// the FP64 tree factor, then the KKT kernel in mode 2 (computeDelassusMatrix: out = G,
+// nrows^2 rows), 4 (computeDampedDelassusMatrixInverse: out = scale (G + mu I)^-1) or 3 (a
+// solve with the contact Cholesky's KKT matrix [[-mu I, J], [J^T, M]]: rhs and out hold
+// nrows + nv rows, the constraint rows first).
 template<class S>
-void delassusCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, int ncon,
-                  const rbd_rigid_constraint* cons, double mu, void* G, int64_t ldG, uint32_t flags) {
+void constraintMatrixCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, int ncon,
+                          const rbd_rigid_constraint* cons, int mode, double mu, double scale, const void* rhs,
+                          int64_t ldrhs, void* out, int64_t ldout, uint32_t flags) {
   constexpr bool f32 = std::is_same_v<S, float>;
   const int nv = m->nv, nj = m->njoints, nnz = m->ops->nnzU;
   std::vector<RigidConstraintDev> tab;
   std::vector<uint8_t> sup;
   const int nrows = constraintTable(m, ncon, cons, tab, sup);
   if (nrows > kKktMaxConstraints) throw Error(RBD_EINVAL, "the constraints' total dimension must be at most 12");
+  const int outRows = mode == 3 ? nrows + nv : nrows * nrows;
   auto vq = argView<S>(m->nq, B, ldq, flags, "q");
-  auto vG = argView<S>(nrows * nrows, B, ldG, flags, "G");
+  auto vo = argView<S>(outRows, B, ldout, flags, mode == 3 ? "x" : "G");
+  ArgView<S> vr{};
+  if (mode == 3) vr = argView<S>(nrows + nv, B, ldrhs, flags, "rhs");
   if (B == 0) return;
   setDevice(ctx);
   Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
   const bool batchLayout = (flags & RBD_LAYOUT_BATCH) != 0;
   const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
-  S* dG = (S*)st.out(G, vG.extent * sizeof(S), ldG != nrows * nrows || batchLayout);
+  const S* dr = mode == 3 ? (const S*)st.in(rhs, vr.extent * sizeof(S)) : nullptr;
+  S* dout = (S*)st.out(out, vo.extent * sizeof(S), ldout != outRows || batchLayout);
   auto table = [&](const void* h, size_t bytes) {
     void* d = ctx->stageBuf(st.slot++, bytes);
     HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -1231,13 +1239,38 @@ void delassusCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, in
     o.chol_f64(ctx->stream, m->gen.get(), B, CView<S>{wM, B, 1}, View<double>{wU, B, 1}, View<double>{wD, B, 1});
   }
   const auto kkt = f32 ? (LaunchKkt<S>)o.kkt_f32 : (LaunchKkt<S>)o.kkt_f64;
-  kkt(ctx->stream, m->gen.get(), B, CView<double>{wU, B, 1}, CView<double>{wD, B, 1}, nrows, CView<S>{wJc, B, 1},
-      CView<S>{}, CView<S>{}, CView<S>{}, mu, 2, 0.0, KktProx{1, 0.0, 0.0, nullptr}, View<S>{},
-      View<S>{dG, vG.rs, vG.bs});
+  const CView<double> cU{wU, B, 1}, cD{wD, B, 1};
+  if (mode == 3)
+    kkt(ctx->stream, m->gen.get(), B, cU, cD, nrows, CView<S>{wJc, B, 1},
+        CView<S>{dr + (int64_t)nrows * vr.rs, vr.rs, vr.bs}, zero, CView<S>{dr, vr.rs, vr.bs}, mu, 3, 0.0,
+        KktProx{1, 0.0, 0.0, nullptr}, View<S>{dout + (int64_t)nrows * vo.rs, vo.rs, vo.bs},
+        View<S>{dout, vo.rs, vo.bs});
+  else
+    kkt(ctx->stream, m->gen.get(), B, cU, cD, nrows, CView<S>{wJc, B, 1}, CView<S>{}, CView<S>{}, CView<S>{}, mu,
+        mode, scale, KktProx{1, 0.0, 0.0, nullptr}, View<S>{}, View<S>{dout, vo.rs, vo.bs});
   finishCall(ctx, flags, st);
 }
 
 static_assert(kNcpStats == RBD_NCP_STATS, "per-problem statistics rows");
+
+// checks shared by the constraint-matrix entry points
+void checkConstraintSet(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, int ncontacts,
+                        const rbd_rigid_constraint* contacts, double mu, const char* what) {
+  checkCommon(ctx, m, B, dtype);
+  RBD_CHECK_ARG(ncontacts >= 1 && ncontacts <= kKktMaxConstraints, "between 1 and 12 constraints");
+  RBD_CHECK_ARG(contacts != nullptr, "null constraint table");
+  for (int c = 0; c < ncontacts; ++c) {
+    const rbd_rigid_constraint& k = contacts[c];
+    RBD_CHECK_ARG(k.type == RBD_CONTACT_3D || k.type == RBD_CONTACT_6D, "constraint type must be RBD_CONTACT_3D or RBD_CONTACT_6D");
+    RBD_CHECK_ARG(k.joint1_id >= 0 && k.joint1_id < m->njoints && k.joint2_id >= 0 && k.joint2_id < m->njoints,
+                  "The index of the Joint is outside the bounds.");
+    RBD_CHECK_ARG(k.reference_frame == RBD_LOCAL || k.reference_frame == RBD_LOCAL_WORLD_ALIGNED,
+                  "the constraint rows are taken in LOCAL or LOCAL_WORLD_ALIGNED frames");
+  }
+  RBD_CHECK_ARG(mu >= 0.0, "mu has to be positive");
+  if (!m->compact || !m->ops->kkt_f32 || !m->ops->crba_f32 || !m->ops->kin_f32)
+    throw Error(RBD_EUNSUPPORTED, std::string(what) + ": no kernels for this topology");
+}
 
 // the cone count of a contact problem and its checks
 void checkNcp(rbd_ctx* ctx, int64_t B, int ncones) {
@@ -1324,25 +1357,41 @@ int rbd_constraint_dynamics(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dty
 int rbd_delassus(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, int ncontacts,
                  const rbd_rigid_constraint* contacts, double mu, void* G, int64_t ldG, uint32_t flags) {
   return guarded([&] {
-    checkCommon(ctx, m, B, dtype);
-    RBD_CHECK_ARG(ncontacts >= 1 && ncontacts <= kKktMaxConstraints, "between 1 and 12 constraints");
-    RBD_CHECK_ARG(contacts != nullptr, "null constraint table");
-    for (int c = 0; c < ncontacts; ++c) {
-      const rbd_rigid_constraint& k = contacts[c];
-      RBD_CHECK_ARG(k.type == RBD_CONTACT_3D || k.type == RBD_CONTACT_6D, "constraint type must be RBD_CONTACT_3D or RBD_CONTACT_6D");
-      RBD_CHECK_ARG(k.joint1_id >= 0 && k.joint1_id < m->njoints && k.joint2_id >= 0 && k.joint2_id < m->njoints,
-                    "The index of the Joint is outside the bounds.");
-      RBD_CHECK_ARG(k.reference_frame == RBD_LOCAL || k.reference_frame == RBD_LOCAL_WORLD_ALIGNED,
-                    "the constraint rows are taken in LOCAL or LOCAL_WORLD_ALIGNED frames");
-    }
-    RBD_CHECK_ARG(mu >= 0.0, "mu has to be positive");
+    checkConstraintSet(ctx, m, B, dtype, ncontacts, contacts, mu, "Delassus matrix");
     RBD_CHECK_ARG(B == 0 || (q && G), "null array argument");
-    if (!m->compact || !m->ops->kkt_f32 || !m->ops->crba_f32 || !m->ops->kin_f32)
-      throw Error(RBD_EUNSUPPORTED, "Delassus matrix: no kernels for this topology");
     if (dtype == RBD_F32)
-      delassusCall<float>(ctx, m, B, q, ldq, ncontacts, contacts, mu, G, ldG, flags);
+      constraintMatrixCall<float>(ctx, m, B, q, ldq, ncontacts, contacts, 2, mu, 1.0, nullptr, 0, G, ldG, flags);
     else
-      delassusCall<double>(ctx, m, B, q, ldq, ncontacts, contacts, mu, G, ldG, flags);
+      constraintMatrixCall<double>(ctx, m, B, q, ldq, ncontacts, contacts, 2, mu, 1.0, nullptr, 0, G, ldG, flags);
+  });
+}
+
+int rbd_damped_delassus_inverse(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                                int ncontacts, const rbd_rigid_constraint* contacts, double mu, int scaled,
+                                void* Ginv, int64_t ldG, uint32_t flags) {
+  return guarded([&] {
+    checkConstraintSet(ctx, m, B, dtype, ncontacts, contacts, mu, "damped Delassus inverse");
+    RBD_CHECK_ARG(mu > 0.0, "mu must be strictly positive");
+    RBD_CHECK_ARG(B == 0 || (q && Ginv), "null array argument");
+    const double scale = scaled ? mu : 1.0;
+    if (dtype == RBD_F32)
+      constraintMatrixCall<float>(ctx, m, B, q, ldq, ncontacts, contacts, 4, mu, scale, nullptr, 0, Ginv, ldG, flags);
+    else
+      constraintMatrixCall<double>(ctx, m, B, q, ldq, ncontacts, contacts, 4, mu, scale, nullptr, 0, Ginv, ldG,
+                                   flags);
+  });
+}
+
+int rbd_contact_cholesky_solve(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                               int ncontacts, const rbd_rigid_constraint* contacts, double mu, const void* rhs,
+                               int64_t ldrhs, void* x, int64_t ldx, uint32_t flags) {
+  return guarded([&] {
+    checkConstraintSet(ctx, m, B, dtype, ncontacts, contacts, mu, "contact Cholesky solve");
+    RBD_CHECK_ARG(B == 0 || (q && rhs && x), "null array argument");
+    if (dtype == RBD_F32)
+      constraintMatrixCall<float>(ctx, m, B, q, ldq, ncontacts, contacts, 3, mu, 1.0, rhs, ldrhs, x, ldx, flags);
+    else
+      constraintMatrixCall<double>(ctx, m, B, q, ldq, ncontacts, contacts, 3, mu, 1.0, rhs, ldrhs, x, ldx, flags);
   });
 }
 

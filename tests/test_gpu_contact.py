@@ -108,6 +108,51 @@ def test_delassus_errors(gpu_ctx):
         eng.delassus(Q, to_models(SETS["frames_local"] * 2))  # 24 rows
 
 
+@pytest.mark.parametrize("scaled", [False, True])
+@pytest.mark.parametrize("layout", ["col", "batch"])
+def test_damped_delassus_inverse(gpu_ctx, layout, scaled):
+    """computeDampedDelassusMatrixInverse against numpy's inverse of the pinned Delassus"""
+    desc = M.humanoid36()
+    eng = pa.Engine(pa.Model(gpu_ctx, desc), layout)
+    q = inputs(desc, B, k=92)[0]
+    cons = SETS["points_lwa_hands"]
+    nr, mu = 6, 0.05
+    for dtype, tol in ((torch.float64, 1e-10), (torch.float32, 2e-4)):
+        qq = f32_round(q)[0] if dtype == torch.float32 else q
+        D = oracle.delassus(desc, qq, cons)
+        want = np.linalg.inv(D + mu * np.eye(nr)) * (mu if scaled else 1.0)
+        Q = torch.tensor(np.ascontiguousarray(qq.T if layout == "col" else qq), dtype=dtype, device="cuda")
+        G = eng.damped_delassus_inverse(Q, to_models(cons), mu, scaled=scaled).double().cpu().numpy()
+        G = G.reshape(B, nr * nr) if layout == "col" else G.T
+        got = G.reshape(B, nr, nr).transpose(0, 2, 1)
+        err = np.abs(got - want).max(axis=(1, 2)) / np.abs(want).max(axis=(1, 2))
+        print(f"damped inverse {layout} scaled={scaled} {dtype}: {err.max():.2e}")
+        assert err.max() < tol
+    with pytest.raises(ValueError):
+        eng.damped_delassus_inverse(Q, to_models(cons), 0.0)
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.01])
+@pytest.mark.parametrize("layout", ["col", "batch"])
+def test_contact_cholesky_solve(gpu_ctx, layout, mu):
+    """x = K^-1 rhs, K = [[-mu I, J], [J^T, M]], against the oracle's dense elimination of K"""
+    desc = M.humanoid36()
+    eng = pa.Engine(pa.Model(gpu_ctx, desc), layout)
+    q = inputs(desc, B, k=93)[0]
+    cons = SETS["points_local"] + [dict(type=1, joint1=22, reference_frame=2)]
+    n = 12 + desc.nv
+    rhs = np.random.default_rng(5).normal(size=(n, B))
+    for dtype, tol in ((torch.float64, 1e-9), (torch.float32, 2e-4)):
+        qq, rr = f32_round(q, rhs) if dtype == torch.float32 else (q, rhs)
+        want = oracle.contact_cholesky_solve(desc, qq, cons, mu, rr)
+        dev_ = lambda x: torch.tensor(np.ascontiguousarray(x.T if layout == "col" else x), dtype=dtype, device="cuda")
+        x = eng.contact_cholesky_solve(dev_(qq), to_models(cons), mu, dev_(rr)).double().cpu().numpy()
+        x = x.T if layout == "col" else x
+        err = np.abs(x - want).max(axis=0) / np.abs(want).max(axis=0)
+        print(f"KKT solve {layout} mu={mu} {dtype}: {err.max():.2e}")
+        assert err.max() < tol
+
+
 # ---------------------------------------------------------------------------------------
 # PGS
 # ---------------------------------------------------------------------------------------

@@ -216,3 +216,37 @@ def test_delassus_is_the_schur_complement_of_constraint_dynamics():
     for b in range(4):
         r = (G[b] + 1e-2 * np.eye(6)) @ l1[:, b] - (G[b] + 5e-2 * np.eye(6)) @ l2[:, b]
         assert np.abs(r).max() < 1e-9 * max(1.0, np.abs(l1[:, b]).max())
+
+
+# ---------------------------------------------------------------------------------------
+# the contact Cholesky's KKT solve
+# ---------------------------------------------------------------------------------------
+KKT_CONS = [dict(type=0, joint1=7, reference_frame=1, placement1=np.concatenate([np.eye(3).ravel(), [0.03, 0, -0.08]])),
+            dict(type=1, joint1=13, reference_frame=2)]
+
+
+def test_kkt_solve_constraint_rows_are_the_damped_delassus_solve():
+    """b = [b1; 0]: x1 = -(J M^-1 J^T + mu I)^-1 b1, with the Delassus matrix pinned above"""
+    desc = M.humanoid36()
+    q = M.sample_inputs(desc, 3, M.INPUT_SEED_BASE + 903)[0]
+    nr = 9
+    b1 = np.random.default_rng(4).normal(size=(nr, 3))
+    for mu in (0.0, 0.01):
+        x = oracle.contact_cholesky_solve(desc, q, KKT_CONS, mu, np.vstack([b1, np.zeros((desc.nv, 3))]))
+        G = oracle.delassus(desc, q, KKT_CONS)
+        for b in range(3):
+            want = -np.linalg.solve(G[b] + mu * np.eye(nr), b1[:, b])
+            assert np.abs(x[:nr, b] - want).max() < 1e-10 * np.abs(want).max()
+
+
+def test_kkt_solve_is_constraint_dynamics_at_rest():
+    """at v = 0 with no correction the constraint drift vanishes, so K [-lambda; ddq] =
+    [0; tau - g(q)] is constraintDynamics' own system"""
+    desc = M.humanoid36()
+    q, _, _, tau = M.sample_inputs(desc, 3, M.INPUT_SEED_BASE + 904)
+    v0 = np.zeros((desc.nv, 3))
+    ddq, lam, _ = oracle.constraint_dynamics(desc, q, v0, tau, KKT_CONS)
+    g = oracle.rnea(desc, q, v0, v0)
+    x = oracle.contact_cholesky_solve(desc, q, KKT_CONS, 0.0, np.vstack([np.zeros((9, 3)), tau - g]))
+    np.testing.assert_allclose(x[:9], -lam, rtol=0, atol=1e-9 * np.abs(lam).max())
+    np.testing.assert_allclose(x[9:], ddq, rtol=0, atol=1e-9 * np.abs(ddq).max())
