@@ -13,6 +13,7 @@
  *   rbd_crba              crba                      algorithm/crba.hxx:498-548, 602-617
  *   rbd_cholesky_decompose cholesky::decompose      algorithm/cholesky.hxx:17-62
  *   rbd_cholesky_solve    cholesky::solve           algorithm/cholesky.hxx:466-520
+ *   rbd_cholesky_apply    cholesky::Uv/Utv/Uiv/Utiv/Mv algorithm/cholesky.hxx:64-465
  *   rbd_rnea_derivatives  computeRNEADerivatives    algorithm/rnea-derivatives.hxx:472-541, 756-766
  *   rbd_aba_derivatives   computeABADerivatives     algorithm/aba-derivatives.hxx:380-453, 900-923
  *   rbd_compute_minverse  computeMinverse           algorithm/aba.hxx:608-902
@@ -164,6 +165,13 @@ int rbd_model_cholesky_pattern(const rbd_model* model, int32_t* nnz, int32_t* ro
 /* x <- M^{-1} x with a factor from rbd_cholesky_decompose (x has dtype `dtype`). */
 int rbd_cholesky_solve(rbd_ctx* ctx, const rbd_model* model, int64_t batch, const double* U, int64_t ldU,
                        const double* D, int64_t ldD, int dtype, void* x, int64_t ldx, uint32_t flags);
+
+/* The factor's products and triangular solves, in place on x (nv rows, FP64), with U and D
+ * from rbd_cholesky_decompose: cholesky::Uv (x <- U x), Utv (U^T x), Uiv (U^-1 x), Utiv
+ * (U^-T x) and Mv (M x = U D U^T x) (cholesky.hpp:86-150, cholesky.hxx:64-465). */
+enum { RBD_CHOL_UV = 0, RBD_CHOL_UTV = 1, RBD_CHOL_UIV = 2, RBD_CHOL_UTIV = 3, RBD_CHOL_MV = 4 };
+int rbd_cholesky_apply(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int op, const double* U, int64_t ldU,
+                       const double* D, int64_t ldD, double* x, int64_t ldx, uint32_t flags);
 
 /* computeRNEADerivatives: dtau_dq, dtau_dv, dtau_da (= M) per configuration; any output
  * may be NULL.  Structurally zero blocks are written as zeros.  Requires gravity with a

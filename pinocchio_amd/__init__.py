@@ -310,6 +310,20 @@ class Engine:
                                         _dtype_code(x), _ptr(x), _i64(ldx), ctypes.c_uint32(self._flags(async_))))
         return x
 
+    def cholesky_apply(self, op, U, D, x, async_=False):
+        """In place on x (float64) with a factor from cholesky_decompose: op "Uv" (U x), "Utv"
+        (U^T x), "Uiv" (U^-1 x), "Utiv" (U^-T x) or "Mv" (M x = U D U^T x) -- cholesky::Uv,
+        Utv, Uiv, Utiv, Mv (cholesky.hxx:64-465)."""
+        codes = {"Uv": 0, "Utv": 1, "Uiv": 2, "Utiv": 3, "Mv": 4}
+        if op not in codes:
+            raise ValueError(f"op must be one of {sorted(codes)}")
+        m = self.model
+        B, ldx = self._vec(x, m.nv, "x")
+        _check(lib().rbd_cholesky_apply(m.ctx.handle, m.handle, _i64(B), codes[op], _ptr(U), _i64(U.stride(0)),
+                                        _ptr(D), _i64(D.stride(0)), _ptr(x), _i64(ldx),
+                                        ctypes.c_uint32(self._flags(async_))))
+        return x
+
     def rnea_derivatives(self, q, v, a, async_=False):
         """(dtau_dq, dtau_dv, M) per configuration (computeRNEADerivatives)."""
         m = self.model

@@ -43,6 +43,11 @@ GenericModel* buildGenericModel(int nj, const int* parents, const int* types);
 // Launchers of the run-time-topology kernels; every entry is set (name "generic").
 const TopoOps* genericOps();
 
+// In place on x: the products and triangular solves of the packed tree factor (cholops.cpp;
+// op: RBD_CHOL_UV, _UTV, _UIV, _UTIV, _MV).
+void launchCholApply(hipStream_t st, const GenericModel* g, int op, int64_t B, CView<double> U, CView<double> D,
+                     View<double> x);
+
 // Grows the lane memory of a model so that its RNEA and ABA launches over `count`
 // configurations allocate nothing (called before a HIP graph capture).
 void genericReserveDyn(const GenericModel* g, int64_t count);

@@ -348,6 +348,29 @@ int rbd_graph_destroy(rbd_graph* graph) {
   });
 }
 
+int rbd_cholesky_apply(rbd_ctx* ctx, const rbd_model* m, int64_t B, int op, const double* U, int64_t ldU,
+                       const double* D, int64_t ldD, double* x, int64_t ldx, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, RBD_F64);
+    RBD_CHECK_ARG(op >= RBD_CHOL_UV && op <= RBD_CHOL_MV, "op must be RBD_CHOL_UV, _UTV, _UIV, _UTIV or _MV");
+    RBD_CHECK_ARG(B == 0 || (U && x && (D || op != RBD_CHOL_MV)), "null array argument");
+    if (!m->compact) throw Error(RBD_EUNSUPPORTED, "cholesky: the model is not DFS-compact");
+    const int nv = m->nv, nnz = (int)m->patRows.size();
+    auto vU = argView<double>(nnz > 0 ? nnz : 1, B, ldU, flags, "U");
+    auto vD = argView<double>(nv, B, op == RBD_CHOL_MV ? ldD : (flags & RBD_LAYOUT_BATCH ? B : nv), flags, "D");
+    auto vx = argView<double>(nv, B, ldx, flags, "x");
+    if (B == 0) return;
+    setDevice(ctx);
+    Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+    const double* dU = (const double*)st.in(U, vU.extent * 8);
+    const double* dD = op == RBD_CHOL_MV ? (const double*)st.in(D, vD.extent * 8) : nullptr;
+    double* dx = (double*)st.out(x, vx.extent * 8, true);
+    launchCholApply(ctx->stream, m->gen.get(), op, B, CView<double>{dU, vU.rs, vU.bs}, CView<double>{dD, vD.rs, vD.bs},
+                    View<double>{dx, vx.rs, vx.bs});
+    finishCall(ctx, flags, st);
+  });
+}
+
 }  // extern "C"
 
 namespace {

@@ -17,7 +17,7 @@ import torch
 import oracle
 import pinocchio_amd as pa
 from pinocchio_amd import models as M
-from helpers import f32_round, inputs, rel_err_mats
+from helpers import f32_round, inputs, rel_err_cols, rel_err_mats
 
 pytestmark = pytest.mark.gpu
 
@@ -101,6 +101,41 @@ def test_cholesky_fp64(gpu_ctx, name, layout):
     x = to_dev(tau, layout, torch.float64)
     eng.cholesky_solve(U, D, x)
     np.testing.assert_allclose(to_host(x, layout), x_ref, rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("name", ["arm7", "humanoid36", "random_tree"])
+def test_cholesky_products(gpu_ctx, name, layout):
+    """cholesky::Uv / Utv / Uiv / Utiv / Mv with the GPU factor, against dense products of
+    the oracle's U and D -- and Mv against M x with CRBA's M (the factor closes the loop)"""
+    if name == "random_tree":  # a tree without prebuilt kernels: the factor from the
+        desc = M.random_tree(20, M.MODEL_SEED_BASE + 107, free_flyer=True, compact=True)  # run-time kernels
+        model = pa.Model(gpu_ctx, desc)
+    else:
+        desc, model = gpu_model(gpu_ctx, name)
+    eng = pa.Engine(model, layout)
+    B = 257
+    q, _, _, tau = inputs(desc, B, k=15)
+    Mref = oracle.crba(desc, q)
+    if layout == "col":
+        Md = torch.tensor(np.ascontiguousarray(Mref.transpose(0, 2, 1)), dtype=torch.float64, device="cuda")
+    else:
+        Md = torch.tensor(np.ascontiguousarray(Mref.transpose(2, 1, 0).reshape(desc.nv * desc.nv, -1)),
+                          dtype=torch.float64, device="cuda")
+    U, D = eng.cholesky_decompose(Md)
+    Uo, Do, _, _ = oracle.cholesky(desc, Mref)
+    want = {"Uv": np.einsum("bij,jb->ib", Uo, tau), "Utv": np.einsum("bji,jb->ib", Uo, tau),
+            "Uiv": np.stack([np.linalg.solve(Uo[b], tau[:, b]) for b in range(B)], 1),
+            "Utiv": np.stack([np.linalg.solve(Uo[b].T, tau[:, b]) for b in range(B)], 1),
+            "Mv": np.einsum("bij,jb->ib", Mref, tau)}
+    for op, w in want.items():
+        x = to_dev(tau, layout, torch.float64)
+        eng.cholesky_apply(op, U, D, x)
+        e = rel_err_cols(to_host(x, layout), w)
+        print(f"{name} {layout} {op}: {e.max():.2e}")
+        assert e.max() < 1e-11, op
+    with pytest.raises(ValueError):
+        eng.cholesky_apply("Dv", U, D, to_dev(tau, layout, torch.float64))
 
 
 @pytest.mark.parametrize("layout", ["col", "batch"])
