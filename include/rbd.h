@@ -10,6 +10,8 @@
  *                         (per column: rnea         algorithm/rnea.hxx:117-161, 773)
  *   rbd_aba               abaInParallel             algorithm/parallel/aba.hpp:40-84
  *                         (per column: aba WORLD    algorithm/aba.hxx:242-293, 926-948)
+ *   rbd_rnea_fext, rbd_aba_fext
+ *                         rnea / aba with fext      algorithm/rnea.hpp:60-79, aba.hpp:63-85
  *   rbd_crba              crba                      algorithm/crba.hxx:498-548, 602-617
  *   rbd_cholesky_decompose cholesky::decompose      algorithm/cholesky.hxx:17-62
  *   rbd_cholesky_solve    cholesky::solve           algorithm/cholesky.hxx:466-520
@@ -165,6 +167,17 @@ int rbd_model_cholesky_pattern(const rbd_model* model, int32_t* nnz, int32_t* ro
 /* x <- M^{-1} x with a factor from rbd_cholesky_decompose (x has dtype `dtype`). */
 int rbd_cholesky_solve(rbd_ctx* ctx, const rbd_model* model, int64_t batch, const double* U, int64_t ldU,
                        const double* D, int64_t ldD, int dtype, void* x, int64_t ldx, uint32_t flags);
+
+/* rnea(model, data, q, v, a, fext) and aba(model, data, q, v, tau, fext) (rnea.hpp:60-79,
+ * aba.hpp:63-85) per configuration: fext holds 6 (njoints - 1) rows per configuration, the
+ * external force on each joint 1.. as [linear; angular] in that joint's frame.  Run on the
+ * run-time-topology kernels for every tree (the prebuilt kernels take no forces). */
+int rbd_rnea_fext(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                  const void* v, int64_t ldv, const void* a, int64_t lda, const void* fext, int64_t ldfext, void* tau,
+                  int64_t ldtau, uint32_t flags);
+int rbd_aba_fext(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                 const void* v, int64_t ldv, const void* tau, int64_t ldtau, const void* fext, int64_t ldfext,
+                 void* ddq, int64_t ldddq, uint32_t flags);
 
 /* The factor's products and triangular solves, in place on x (nv rows, FP64), with U and D
  * from rbd_cholesky_decompose: cholesky::Uv (x <- U x), Utv (U^T x), Uiv (U^-1 x), Utiv

@@ -443,3 +443,25 @@ def contact_cholesky_solve(model, q, constraints, mu, rhs, scalar="f64", nthread
     _check(lib().orc_contact_cholesky_solve(ctypes.byref(d), ctypes.c_int64(B), _p(q), len(constraints), arr,
                                             ctypes.c_double(mu), _p(r), _p(x), SCALARS[scalar], nthreads))
     return np.ascontiguousarray(x)
+
+
+def _fext(model, alg, q, v, x, fext, scalar, nthreads):
+    q, v, x = _cols(q, model.nq), _cols(v, model.nv), _cols(x, model.nv)
+    B = q.shape[1]
+    f = _cols(fext, 6 * (model.njoints - 1))
+    out = np.zeros((model.nv, B), order="F")
+    d = model.cstruct()
+    _check(lib().orc_fext(ctypes.byref(d), ctypes.c_int64(B), alg, _p(q), _p(v), _p(x), _p(f), _p(out),
+                          SCALARS[scalar], nthreads))
+    return np.ascontiguousarray(out)
+
+
+def rnea_fext(model, q, v, a, fext, scalar="f64", nthreads=0):
+    """rnea(model, data, q, v, a, fext) per column (rnea.hpp:60-79): fext (6 (njoints - 1), B),
+    each joint's external force [linear; angular] in its local frame."""
+    return _fext(model, 0, q, v, a, fext, scalar, nthreads)
+
+
+def aba_fext(model, q, v, tau, fext, scalar="f64", nthreads=0):
+    """aba(model, data, q, v, tau, fext) per column (aba.hpp:63-85), fext as in rnea_fext."""
+    return _fext(model, 1, q, v, tau, fext, scalar, nthreads)

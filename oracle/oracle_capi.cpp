@@ -848,3 +848,36 @@ extern "C" int orc_contact_cholesky_solve(const ModelDesc* d, int64_t B, const d
     return 1;
   }
 }
+
+// RNEA and ABA (LOCAL) with external forces: fext holds 6 (njoints - 1) numbers per
+// configuration, back to back (joint 1.., [linear; angular] in the joint's frame)
+namespace {
+template<class S>
+int fextT(const ModelDesc* d, int64_t B, int alg, const double* q, const double* v, const double* x,
+          const double* fext, double* out, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv, bx, bf;
+    static thread_local std::vector<S> o;
+    o.resize(m.nv);
+    const int nf = 6 * (m.njoints - 1);
+    const S* qq = bq.load(q + b * m.nq, m.nq);
+    const S* vv = bv.load(v + b * m.nv, m.nv);
+    const S* xx = bx.load(x + b * m.nv, m.nv);
+    const S* ff = bf.load(fext + b * nf, nf);
+    if (alg == 0) rnea(m, data, qq, vv, xx, o.data(), ff);
+    else abaLocal(m, data, qq, vv, xx, o.data(), ff);
+    store(o.data(), out + b * m.nv, m.nv);
+  });
+}
+}  // namespace
+
+extern "C" int orc_fext(const ModelDesc* d, int64_t B, int alg, const double* q, const double* v, const double* x,
+                        const double* fext, double* out, int scalar, int nthreads) {
+  switch (scalar) {
+    case 0: return fextT<double>(d, B, alg, q, v, x, fext, out, nthreads);
+    case 1: return fextT<float>(d, B, alg, q, v, x, fext, out, nthreads);
+    case 2: return fextT<long double>(d, B, alg, q, v, x, fext, out, nthreads);
+  }
+  g_err = "bad scalar";
+  return 1;
+}

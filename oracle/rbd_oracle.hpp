@@ -627,8 +627,11 @@ template<class S> inline void computeJointJacobians(const Model<S>& model, Data<
 // ----------------------------------------------------------------------------------------
 // RNEA, rnea.hxx:17-161 (LOCAL frames)
 // ----------------------------------------------------------------------------------------
+// fext (optional): an external force on each joint 1.., [linear; angular] in the joint's
+// local frame, 6 per joint (rnea.hpp:60-79: f_i -= fext_i in the forward step)
 template<class S>
-inline void rnea(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* a, S* tau_out) {
+inline void rnea(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* a, S* tau_out,
+                 const S* fext = nullptr) {
   std::fill(data.tau.begin(), data.tau.end(), S(0));
   data.v[0] = zeroMotion<S>();
   data.a_gf[0] = S(-1) * model.gravity;
@@ -644,6 +647,10 @@ inline void rnea(const Model<S>& model, Data<S>& data, const S* q, const S* v, c
     data.a_gf[i] = ag;
     data.h[i] = model.inertias[i] * data.v[i];
     data.f[i] = model.inertias[i] * data.a_gf[i] + crossMF(data.v[i], data.h[i]);
+    if (fext) {
+      const S* fe = fext + (size_t)(i - 1) * 6;
+      data.f[i] = Force<S>{data.f[i].lin - v3(fe[0], fe[1], fe[2]), data.f[i].ang - v3(fe[3], fe[4], fe[5])};
+    }
     data.jc[i] = jc;
   }
   for (int i = model.njoints - 1; i > 0; --i) {  // RneaBackwardStep, rnea.hxx:91-107
@@ -757,8 +764,10 @@ inline void abaWorld(const Model<S>& model, Data<S>& data, const S* q, const S* 
 // ----------------------------------------------------------------------------------------
 // ABA, LOCAL convention (aba.hxx:354-540), the single-configuration default
 // ----------------------------------------------------------------------------------------
+// fext as in rnea (aba.hpp:63-85: the bias force f_i = v_i x* I_i v_i - fext_i)
 template<class S>
-inline void abaLocal(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* tau, S* ddq_out) {
+inline void abaLocal(const Model<S>& model, Data<S>& data, const S* q, const S* v, const S* tau, S* ddq_out,
+                     const S* fext = nullptr) {
   const int nj = model.njoints;
   data.v[0] = zeroMotion<S>();
   data.a_gf[0] = S(-1) * model.gravity;
@@ -775,6 +784,10 @@ inline void abaLocal(const Model<S>& model, Data<S>& data, const S* q, const S* 
     data.Yaba[i] = model.inertias[i].matrix();
     data.h[i] = model.inertias[i] * data.v[i];
     data.f[i] = crossMF(data.v[i], data.h[i]);
+    if (fext) {
+      const S* fe = fext + (size_t)(i - 1) * 6;
+      data.f[i] = Force<S>{data.f[i].lin - v3(fe[0], fe[1], fe[2]), data.f[i].ang - v3(fe[3], fe[4], fe[5])};
+    }
   }
   for (int i = nj - 1; i > 0; --i) {
     const int p = model.parents[i];

@@ -260,6 +260,28 @@ class Engine:
                              _ptr(tau), _i64(ldt), _ptr(out), _i64(ldo), ctypes.c_uint32(self._flags(async_))))
         return out
 
+    def rnea_fext(self, q, v, a, fext, out=None, async_=False):
+        """rnea(model, data, q, v, a, fext) per configuration (rnea.hpp:60-79): fext has
+        6 (njoints - 1) rows, each joint's external force [linear; angular] in its frame."""
+        return self._fext("rbd_rnea_fext", q, v, a, fext, out, async_)
+
+    def aba_fext(self, q, v, tau, fext, out=None, async_=False):
+        """aba(model, data, q, v, tau, fext) per configuration (aba.hpp:63-85)."""
+        return self._fext("rbd_aba_fext", q, v, tau, fext, out, async_)
+
+    def _fext(self, fn, q, v, x, fext, out, async_):
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        _, ldx = self._vec(x, m.nv, "a/tau")
+        _, ldf = self._vec(fext, 6 * (m.njoints - 1), "fext")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        _, ldo = self._vec(out, m.nv, "out")
+        _check(getattr(lib(), fn)(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v),
+                                  _i64(ldv), _ptr(x), _i64(ldx), _ptr(fext), _i64(ldf), _ptr(out), _i64(ldo),
+                                  ctypes.c_uint32(self._flags(async_))))
+        return out
+
     def _mat_arg(self, t, B, name):
         nv = self.model.nv
         if self.layout == "col":

@@ -186,11 +186,11 @@ template<class S> size_t kinLane(GM g) { return (size_t)g->nlev * kGKinLvl * kTi
 
 template<class S>
 void gRnea(hipStream_t st, GM g, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a, View<S> out) {
-  RBD_G_LAUNCH(g_rnea, S, rneaLane<S>(g), B, progOf(g), mc, B, q, v, a, out);
+  RBD_G_LAUNCH(g_rnea, S, rneaLane<S>(g), B, progOf(g), mc, B, q, v, a, CView<S>{}, out);
 }
 template<class S>
 void gAba(hipStream_t st, GM g, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> tau, View<S> out) {
-  RBD_G_LAUNCH(g_aba, S, abaLane<S>(g), B, progOf(g), mc, B, q, v, tau, out);
+  RBD_G_LAUNCH(g_aba, S, abaLane<S>(g), B, progOf(g), mc, B, q, v, tau, CView<S>{}, out);
 }
 template<class S> void gCrba(hipStream_t st, GM g, const S* mc, int64_t B, CView<S> q, View<S> M, int upper) {
   RBD_G_LAUNCH(g_crba, S, crbaLane<S>(g), B, progOf(g), mc, B, q, M, upper);
@@ -282,5 +282,25 @@ const TopoOps* genericOps() {
   }();
   return &ops;
 }
+
+// RNEA / ABA with external forces (run-time topology for every tree)
+template<class S>
+void launchRneaFext(hipStream_t st, const GenericModel* g, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a,
+                    CView<S> fext, View<S> out) {
+  RBD_G_LAUNCH(g_rnea, S, rneaLane<S>(g), B, progOf(g), mc, B, q, v, a, fext, out);
+}
+template<class S>
+void launchAbaFext(hipStream_t st, const GenericModel* g, const S* mc, int64_t B, CView<S> q, CView<S> v,
+                   CView<S> tau, CView<S> fext, View<S> out) {
+  RBD_G_LAUNCH(g_aba, S, abaLane<S>(g), B, progOf(g), mc, B, q, v, tau, fext, out);
+}
+template void launchRneaFext<float>(hipStream_t, const GenericModel*, const float*, int64_t, CView<float>, CView<float>,
+                                    CView<float>, CView<float>, View<float>);
+template void launchRneaFext<double>(hipStream_t, const GenericModel*, const double*, int64_t, CView<double>,
+                                     CView<double>, CView<double>, CView<double>, View<double>);
+template void launchAbaFext<float>(hipStream_t, const GenericModel*, const float*, int64_t, CView<float>, CView<float>,
+                                   CView<float>, CView<float>, View<float>);
+template void launchAbaFext<double>(hipStream_t, const GenericModel*, const double*, int64_t, CView<double>,
+                                    CView<double>, CView<double>, CView<double>, View<double>);
 
 }  // namespace rbd

@@ -43,6 +43,15 @@ GenericModel* buildGenericModel(int nj, const int* parents, const int* types);
 // Launchers of the run-time-topology kernels; every entry is set (name "generic").
 const TopoOps* genericOps();
 
+// RNEA / ABA with external forces (fext: 6 rows per joint 1.., [linear; angular] in the
+// joint's frame), on the run-time-topology kernels for every tree
+template<class S>
+void launchRneaFext(hipStream_t st, const GenericModel* g, const S* mc, int64_t B, CView<S> q, CView<S> v, CView<S> a,
+                    CView<S> fext, View<S> out);
+template<class S>
+void launchAbaFext(hipStream_t st, const GenericModel* g, const S* mc, int64_t B, CView<S> q, CView<S> v,
+                   CView<S> tau, CView<S> fext, View<S> out);
+
 // In place on x: the products and triangular solves of the packed tree factor (cholops.cpp;
 // op: RBD_CHOL_UV, _UTV, _UIV, _UTIV, _MV).
 void launchCholApply(hipStream_t st, const GenericModel* g, int op, int64_t B, CView<double> U, CView<double> D,

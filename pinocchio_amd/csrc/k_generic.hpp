@@ -150,11 +150,17 @@ __device__ __forceinline__ bool gNz(const GProg& g, int r, int c) {
   for (int64_t tile_ = blockIdx.x; tile_ < ntiles_; tile_ += gridDim.x)
 
 // ---------------------------------------------------------------------------------------
-// RNEA (rnea.hxx:17-161, local frames): tau = M(q) a + C(q, v) v + g(q) + armature * a
+// RNEA (rnea.hxx:17-161, local frames): tau = M(q) a + C(q, v) v + g(q) + armature * a,
+// minus the external forces' generalized forces when gfext.p is set (rnea.hpp:60-79:
+// 6 rows per joint 1.., [linear; angular] in the joint's frame)
 // ---------------------------------------------------------------------------------------
+template<class S> __device__ __forceinline__ Frc<S> gFext(const CView<S>& gf, int i, int64_t b) {
+  const int r = (i - 1) * 6;
+  return {{gin(gf, r, b), gin(gf, r + 1, b), gin(gf, r + 2, b)}, {gin(gf, r + 3, b), gin(gf, r + 4, b), gin(gf, r + 5, b)}};
+}
 template<class S, bool kLds>
 __global__ __launch_bounds__(64) void g_rnea(GProg g, const S* __restrict__ mc, int64_t B, CView<S> gq, CView<S> gv,
-                                             CView<S> ga, View<S> gtau, S* __restrict__ ws) {
+                                             CView<S> ga, CView<S> gfext, View<S> gtau, S* __restrict__ ws) {
   const int L = laneId();
   const Lane<S> m = laneMem<S, kLds>(ws, (int64_t)g.nlev * kGRneaLvl * kTile);
   const Mot<S> a0 = mcGravityNeg(mc, g.nj * MC_REC + g.nv);
@@ -184,7 +190,9 @@ __global__ __launch_bounds__(64) void g_rnea(GProg g, const S* __restrict__ mc, 
         const Body<S> Y = mcBody(mc, i);
         m.put(s0, vi);
         m.put(s0 + 6, ai);
-        m.put(s0 + 12, inertiaMul(Y, ai) + crossMF(vi, inertiaMul(Y, vi)));
+        Frc<S> fi = inertiaMul(Y, ai) + crossMF(vi, inertiaMul(Y, vi));
+        if (gfext.p) fi = fi - gFext(gfext, i, b);
+        m.put(s0 + 12, fi);
       } else {
         const Frc<S> f = m.template get<Frc<S>>(s0 + 12);
         if (live) {
@@ -203,11 +211,12 @@ __global__ __launch_bounds__(64) void g_rnea(GProg g, const S* __restrict__ mc, 
 
 // ---------------------------------------------------------------------------------------
 // ABA (aba.hxx:79-293 computed in local frames, bias re-associated as in k_dyn.hpp):
-//   passes 1 + 2 over the events; pass 3 (accelerations) over the PRE events.
+//   passes 1 + 2 over the events; pass 3 (accelerations) over the PRE events.  External
+//   forces (aba.hpp:63-85) enter the bias force, p_i = v_i x* I_i v_i - fext_i.
 // ---------------------------------------------------------------------------------------
 template<class S, bool kLds>
 __global__ __launch_bounds__(64) void g_aba(GProg g, const S* __restrict__ mc, int64_t B, CView<S> gq, CView<S> gv,
-                                            CView<S> gt, View<S> gddq, S* __restrict__ ws) {
+                                            CView<S> gt, CView<S> gfext, View<S> gddq, S* __restrict__ ws) {
   const int L = laneId();
   const int jbase = g.nlev * kGAbaLvl;
   const Lane<S> m = laneMem<S, kLds>(ws, (int64_t)(jbase + g.nj * kGAbaJnt) * kTile);
@@ -235,7 +244,9 @@ __global__ __launch_bounds__(64) void g_aba(GProg g, const S* __restrict__ mc, i
         m.put(s0, vi);
         m.put(s0 + 6, chat);
         m.put(s0 + 12, mcSym6(mc, i));
-        m.put(s0 + 33, crossMF(vi, inertiaMul(mcBody(mc, i), vi)));
+        Frc<S> pi = crossMF(vi, inertiaMul(mcBody(mc, i), vi));
+        if (gfext.p) pi = pi - gFext(gfext, i, b);
+        m.put(s0 + 33, pi);
       } else {
         Sym6<S> Ia = m.template get<Sym6<S>>(s0 + 12);
         const Frc<S> pA = m.template get<Frc<S>>(s0 + 33);

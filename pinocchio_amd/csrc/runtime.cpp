@@ -225,6 +225,36 @@ int vec3Op(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* x0, int64_t 
 
 }  // namespace
 
+// rnea / aba with external forces (run-time-topology kernels for every tree)
+namespace {
+template<class S>
+void fextCall(rbd_ctx* ctx, const rbd_model* m, int alg, int64_t B, const void* q, int64_t ldq, const void* v,
+              int64_t ldv, const void* x, int64_t ldx, const void* fext, int64_t ldf, void* out, int64_t ldo,
+              uint32_t flags) {
+  const int nv = m->nv;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v");
+  auto vx = argView<S>(nv, B, ldx, flags, alg == 0 ? "a" : "tau");
+  auto vf = argView<S>(6 * (m->njoints - 1), B, ldf, flags, "fext");
+  auto vo = argView<S>(nv, B, ldo, flags, alg == 0 ? "tau" : "ddq");
+  if (B == 0) return;
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* dx = (const S*)st.in(x, vx.extent * sizeof(S));
+  const S* df = (const S*)st.in(fext, vf.extent * sizeof(S));
+  S* dout = (S*)st.out(out, vo.extent * sizeof(S), ldo != nv || (flags & RBD_LAYOUT_BATCH));
+  const S* mc = std::is_same<S, float>::value ? (const S*)m->mc32 : (const S*)m->mc64;
+  const CView<S> cq{dq, vq.rs, vq.bs}, cv{dv, vv.rs, vv.bs}, cx{dx, vx.rs, vx.bs}, cf{df, vf.rs, vf.bs};
+  if (alg == 0)
+    launchRneaFext<S>(ctx->stream, m->gen.get(), mc, B, cq, cv, cx, cf, View<S>{dout, vo.rs, vo.bs});
+  else
+    launchAbaFext<S>(ctx->stream, m->gen.get(), mc, B, cq, cv, cx, cf, View<S>{dout, vo.rs, vo.bs});
+  finishCall(ctx, flags, st);
+}
+}  // namespace
+
 extern "C" {
 
 const char* rbd_last_error(void) { return g_err.c_str(); }
@@ -345,6 +375,29 @@ int rbd_graph_destroy(rbd_graph* graph) {
     if (graph->graph) (void)hipGraphDestroy(graph->graph);
     --graph->ctx->graphs;
     delete graph;
+  });
+}
+
+
+int rbd_rnea_fext(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, const void* v,
+                  int64_t ldv, const void* a, int64_t lda, const void* fext, int64_t ldf, void* tau, int64_t ldtau,
+                  uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && v && a && fext && tau), "null array argument");
+    if (dtype == RBD_F32) fextCall<float>(ctx, m, 0, B, q, ldq, v, ldv, a, lda, fext, ldf, tau, ldtau, flags);
+    else fextCall<double>(ctx, m, 0, B, q, ldq, v, ldv, a, lda, fext, ldf, tau, ldtau, flags);
+  });
+}
+
+int rbd_aba_fext(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, const void* v,
+                 int64_t ldv, const void* tau, int64_t ldtau, const void* fext, int64_t ldf, void* ddq, int64_t ldddq,
+                 uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && v && tau && fext && ddq), "null array argument");
+    if (dtype == RBD_F32) fextCall<float>(ctx, m, 1, B, q, ldq, v, ldv, tau, ldtau, fext, ldf, ddq, ldddq, flags);
+    else fextCall<double>(ctx, m, 1, B, q, ldq, v, ldv, tau, ldtau, fext, ldf, ddq, ldddq, flags);
   });
 }
 
