@@ -151,6 +151,7 @@ def surrogate_bench(torch, pa, M, ctx, args):
 
     t_aba = tm(lambda: eng.aba(Q, V, T, out=out, async_=True))
     t_pred = tm(lambda: sur.predict(Q, V, T, out=out, async_=True))
+    t_pred_layered = tm(lambda: sur.predict(Q, V, T, out=out, async_=True, layered=True))
     t_fd = tm(lambda: sur.forward_dynamics(Q, V, T, tol=1e-3, out=out))
     _, nfb, acc, res = sur.forward_dynamics(Q, V, T, tol=1e-3, out=out, want_mask=True)
     r = res.cpu().numpy()
@@ -164,6 +165,7 @@ def surrogate_bench(torch, pa, M, ctx, args):
         "aba_configs_per_s": n / t_aba, "surrogate_with_fallback_configs_per_s": n / t_fd,
         "surrogate_only_configs_per_s": n / t_pred, "speedup_vs_aba": t_aba / t_fd,
         "mlp_forward_TFLOP_s": fwd_flop * n / t_pred / 1e12, "mlp_frac_of_bf16_peak": fwd_flop * n / t_pred / 2.5e15,
+        "mlp_inference_ms": {"fused": t_pred * 1e3, "layer_by_layer": t_pred_layered * 1e3},
     }
 
 
@@ -265,10 +267,19 @@ def main():
         n = 262144
         Qm, Vm, Tm = S.sample_inputs_device(mq, n, M.INPUT_SEED_BASE + 2000, tau_sigma=20.0)
         out = torch.empty((dq.nv, n), dtype=torch.float32, device=dev)
-        for _ in range(args.warmup + args.steps):
-            sur.predict(Qm, Vm, Tm, out=out, async_=True)
-        ctx.synchronize()
-        print(json.dumps({"only": "mlp", "batch": n}))
+        fwd_flop = 2 * sum(o * i for o, i in sur.layer_shapes())
+        res = {"only": "mlp", "batch": n, "forward_flop_per_config": fwd_flop}
+        for name, layered in (("fused", False), ("layer_by_layer", True)):
+            for _ in range(args.warmup):
+                sur.predict(Qm, Vm, Tm, out=out, async_=True, layered=layered)
+            ctx.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                sur.predict(Qm, Vm, Tm, out=out, async_=True, layered=layered)
+            ctx.synchronize()
+            ms = (time.perf_counter() - t) / args.steps * 1e3
+            res[name] = {"ms": ms, "TFLOP_s": fwd_flop * n / ms / 1e9, "frac_of_bf16_peak": fwd_flop * n / ms / 2.5e12}
+        print(json.dumps(res))
         return
 
     if args.only == "surrogate":

@@ -454,7 +454,10 @@ int rbd_mlp_set_params(rbd_mlp* mlp, const float* params, int64_t nparams);
 /* raw (unnormalised) features, device array [batch][nin] (row per configuration) */
 int rbd_mlp_features(rbd_ctx* ctx, rbd_mlp* mlp, int64_t batch, const float* q, int64_t ldq, const float* v,
                      int64_t ldv, const float* tau, int64_t ldtau, float* features, uint32_t flags);
-/* ddq = surrogate(q, v, tau), no check */
+/* ddq = surrogate(q, v, tau), no check.  Networks of hidden width 128, 256 or 512 with at
+ * most 64 outputs run as one fused kernel (activations kept on chip); RBD_MLP_LAYERED
+ * evaluates them layer by layer instead, with the training path's kernels. */
+#define RBD_MLP_LAYERED 0x40u
 int rbd_mlp_predict(rbd_ctx* ctx, rbd_mlp* mlp, int64_t batch, const float* q, int64_t ldq, const float* v,
                     int64_t ldv, const float* tau, int64_t ldtau, float* ddq, int64_t ldddq, uint32_t flags);
 /* one Adam step (lr; beta 0.9, 0.999; eps 1e-8) on the minibatch with targets ddq; the

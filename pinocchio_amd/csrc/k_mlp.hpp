@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernels_common.hpp"
+
 namespace rbd {
 namespace mlp {
 
@@ -50,7 +52,9 @@ struct GemmArgs {
   int64_t zstride;  // EPI_PARTIAL: distance between the partials of consecutive splits
 };
 
-__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function with the hardware reciprocal (1 ulp; an IEEE division costs about ten
+// more instructions per activation)
+__device__ __forceinline__ float sigmoidf(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // Stage ROWS x kBK operand values (row r, k) into s[r * kLdk + k].
 template<int ROWS, bool KCONTIG>
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(64) void k_features(FeatMap fm, int64_t B, const fl
     int f = 0;
     for (int r = 0; r < fm.nrev; ++r) {
       float sn, cs;
-      sincosf(q[fm.revq[r] * ldq + b], &sn, &cs);
+      sincosT(q[fm.revq[r] * ldq + b], &sn, &cs);
       put(r, sn);
       put(fm.nrev + r, cs);
     }
@@ -352,6 +356,275 @@ __global__ __launch_bounds__(64) void k_features(FeatMap fm, int64_t B, const fl
     const int r = c / half, k = c - r * half;
     dst[c] = src[r * (half + 1) + k];
   }
+}
+
+// ---- fused inference --------------------------------------------------------------------
+// The whole network for 128 configurations per workgroup: input features, every hidden
+// layer and the output layer, with the activations resident in LDS (row per configuration,
+// bf16, pitch max(kin, H) + 8) and the weights streamed from L2 straight into the matrix
+// cores' operand registers -- no activation leaves the chip.  Products are computed
+// transposed, C(n, b) = sum_k W(n, k) X(b, k) (v_mfma_f32_32x32x16_bf16, A = 32 weight
+// rows, B = 32 configurations), so a lane holds four consecutive outputs of one
+// configuration: the SiLU epilogue writes them back to LDS as one 8-byte store, and the
+// output layer's stores run along b, coalesced in the batch-fastest ddq.  Eight waves; in
+// a hidden layer wave w owns 64 outputs (column group w % (H / 64)) of 128 / WR
+// configurations (WR = 8 / (H / 64) row groups).  k runs in 16-steps from 0 to K, the
+// bias and SiLU are those of the layer-by-layer kernels.  LDS: the activation rows
+// (128 x pitch bf16), then the hidden layers' biases (L x H fp32).
+constexpr int kInfRows = 128, kInfThreads = 512, kInfMaxOut = 64, kInfMaxLayers = 9;
+constexpr int kFragElems = 512;  // one 32 x 16 weight fragment
+
+// Fragment-major copy of a layer's weights W[N][K] (N % 32 == 0, K % 16 == 0): fragment
+// (n / 32, k / 16), fragments of one 32-row block consecutive along k, holds its 512 values
+// in the order the matrix cores take them: lane (n % 32) + 32 ((k % 16) / 8), then k % 8.
+__global__ void k_pack_fragments(const bf16* __restrict__ W, bf16* __restrict__ F, int N, int K) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * K) return;
+  const int n = (int)(i / K), k = (int)(i - (int64_t)n * K);
+  const int64_t frag = (int64_t)(n / 32) * (K / 16) + k / 16;
+  F[frag * kFragElems + ((n % 32) + 32 * ((k % 16) / 8)) * 8 + (k % 8)] = W[i];
+}
+
+struct InferArgs {
+  FeatMap fm;
+  int64_t B;
+  const float* q;
+  int64_t ldq;
+  const float* v;
+  int64_t ldv;
+  const float* tau;
+  int64_t ldt;
+  const float *xmean, *xinv, *ymean, *ystd;
+  const bf16* Wb;  // fragment-major bf16 weights (k_pack_fragments), layer l at offW[l]
+  const float* P;  // fp32 parameter block (biases)
+  int64_t offW[kInfMaxLayers], offB[kInfMaxLayers];
+  int L, kin, kout, nout, pitch;
+  float* ddq;
+  int64_t ldd;
+};
+
+// features of the tile's configurations into X[s * pitch + f] (bf16, normalised; padding
+// features zero, configurations past the batch all zero).  Four groups of 128 threads
+// (two waves each): sin / cos of the revolute joints, the free flyer's 13 and the zero
+// padding, v, tau.
+__device__ __forceinline__ void inferFeatures(const InferArgs& a, bf16* __restrict__ X, int64_t b0) {
+  const FeatMap& fm = a.fm;
+  const int tid = threadIdx.x, s = tid & (kInfRows - 1);
+  const int part = __builtin_amdgcn_readfirstlane(tid >> 7);
+  const int64_t b = b0 + s;
+  const bool live = b < a.B;
+  bf16* row = X + s * a.pitch;
+  auto put = [&](int f, float raw) { row[f] = live ? (bf16)((raw - a.xmean[f]) * a.xinv[f]) : (bf16)0.f; };
+  const int fv = 2 * fm.nrev + (fm.hasFF ? 13 : 0);
+  // inputs are read eight at a time, all eight requests in flight before the first is used
+  if (part == 0) {
+    for (int r0 = 0; r0 < fm.nrev; r0 += 8) {
+      float x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = (live && r0 + j < fm.nrev) ? a.q[fm.revq[r0 + j] * a.ldq + b] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (r0 + j < fm.nrev) {
+          float sn, cs;
+          sincosT(x[j], &sn, &cs);
+          put(r0 + j, sn);
+          put(fm.nrev + r0 + j, cs);
+        }
+      }
+    }
+  } else if (part == 1) {
+    if (fm.hasFF) {
+      float qx = 0.f, qy = 0.f, qz = 0.f, qw = 1.f;
+      if (live) {
+        qx = a.q[3 * a.ldq + b];
+        qy = a.q[4 * a.ldq + b];
+        qz = a.q[5 * a.ldq + b];
+        qw = a.q[6 * a.ldq + b];
+      }
+      float r9[9];
+      quatRot9(qx, qy, qz, qw, r9);
+      const int f = 2 * fm.nrev;
+      put(f, qx);
+      put(f + 1, qy);
+      put(f + 2, qz);
+      put(f + 3, qw);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) put(f + 4 + k, r9[k]);
+    }
+    for (int f = fm.nfeat; f < a.kin; ++f) row[f] = (bf16)0.f;
+  } else {
+    const float* src = part == 2 ? a.v : a.tau;
+    const int64_t ld = part == 2 ? a.ldv : a.ldt;
+    const int f0 = part == 2 ? fv : fv + fm.nv;
+    for (int k0 = 0; k0 < fm.nv; k0 += 8) {
+      float x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = (live && k0 + j < fm.nv) ? src[(k0 + j) * ld + b] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (k0 + j < fm.nv) put(f0 + k0 + j, x[j]);
+    }
+  }
+}
+
+// acc[j][r] += W(n0 + 32 j + i, k) X(r0 + 32 r + c, k) over k in [0, K): A operand from
+// the fragment-major weights in global memory (one load = one contiguous 1 KB fragment),
+// B operand from the LDS activations.  K is a compile-time constant so that the k loop
+// unrolls completely: the weights of the next 64-k stage are requested before the current
+// stage's products (nothing loop-carried for the compiler to sink back to the point of use).
+template<int RB, int K>
+__device__ __forceinline__ void inferProduct(f32x16 (&acc)[2][RB], const bf16* __restrict__ X, int pitch,
+                                             const bf16* __restrict__ W, int n0, int r0, int lane) {
+  static_assert(K % 64 == 0, "whole 64-k stages");
+  constexpr int NST = K / 64, KS = K / 16;
+  const int lr = lane & 31, lk = (lane >> 5) * 8;
+  const bf16* w0 = W + (int64_t)(n0 / 32) * KS * kFragElems + lane * 8;
+  const bf16* w1 = w0 + (int64_t)KS * kFragElems;
+  const bf16* x = X + (r0 + lr) * pitch + lk;
+  bf16x8 wr[NST][4][2];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    wr[0][kk][0] = *reinterpret_cast<const bf16x8*>(w0 + kk * kFragElems);
+    wr[0][kk][1] = *reinterpret_cast<const bf16x8*>(w1 + kk * kFragElems);
+  }
+#pragma unroll
+  for (int st = 0; st < NST; ++st) {
+    if (st + 1 < NST) {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        wr[st + 1][kk][0] = *reinterpret_cast<const bf16x8*>(w0 + ((st + 1) * 4 + kk) * kFragElems);
+        wr[st + 1][kk][1] = *reinterpret_cast<const bf16x8*>(w1 + ((st + 1) * 4 + kk) * kFragElems);
+      }
+    }
+    // keep those requests ahead of this stage's products (the scheduler would otherwise
+    // move them next to their use and expose the L2 latency at every step)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = st * 64 + kk * 16;
+      bf16x8 xb[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) xb[r] = *reinterpret_cast<const bf16x8*>(x + r * 32 * pitch + k);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        acc[0][r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[st][kk][0], xb[r], acc[0][r], 0, 0, 0);
+        acc[1][r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[st][kk][1], xb[r], acc[1][r], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// one hidden layer: products, then (after every wave has read the input) bias + SiLU back
+// into the activation rows
+template<int H, int K>
+__device__ __forceinline__ void inferLayer(bf16* __restrict__ X, int pitch, const bf16* __restrict__ W,
+                                           const float* __restrict__ bias, int lane, int wave) {
+  constexpr int WC = H / 64, WR = 8 / WC, RB = kInfRows / WR / 32;
+  const int n0 = (wave % WC) * 64, r0 = (wave / WC) * (kInfRows / WR);
+  const int lr = lane & 31, lh = (lane >> 5) * 4;
+  f32x16 acc[2][RB];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[j][r][e] = 0.f;
+  inferProduct<RB, K>(acc, X, pitch, W, n0, r0, lane);
+  __syncthreads();  // every wave has read the layer's input
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int n = n0 + j * 32 + g * 8 + lh;  // this lane's outputs n .. n + 3
+      const float4 bv = *reinterpret_cast<const float4*>(bias + n);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        bf16 o[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float z = acc[j][r][4 * g + t] + (t == 0 ? bv.x : t == 1 ? bv.y : t == 2 ? bv.z : bv.w);
+          o[t] = (bf16)(z * sigmoidf(z));
+        }
+        *reinterpret_cast<uint2*>(X + (r0 + r * 32 + lr) * pitch + n) = *reinterpret_cast<const uint2*>(o);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// output layer: block (nb, rb) of 32 outputs x 32 configurations per wave (nb < kout / 32),
+// then ddq = (acc + bias) * std + mean stored along b.  Every weight of the block is
+// requested before the first product, and the k steps feed four independent accumulator
+// chains (summed at the end), so the block costs a few MFMA latencies, not 32.
+template<int H>
+__device__ __forceinline__ void inferOutput(const InferArgs& a, const bf16* __restrict__ X, int64_t b0, int lane,
+                                            int wave) {
+  constexpr int NS = H / 16;
+  const int nb = wave >> 2, rb = wave & 3;
+  if (nb * 32 >= a.kout) return;
+  const int lr = lane & 31, lh = (lane >> 5) * 4;
+  const bf16* W = a.Wb + a.offW[a.L] + (int64_t)nb * NS * kFragElems + lane * 8;
+  const bf16* x = X + (rb * 32 + lr) * a.pitch + (lane >> 5) * 8;
+  bf16x8 wf[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) wf[k] = *reinterpret_cast<const bf16x8*>(W + k * kFragElems);
+  const float* bias = a.P + a.offB[a.L];
+  float cb[16], cs[16], cm[16];  // bias, std and mean of this lane's outputs (n < kout: in range)
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int n = nb * 32 + (e >> 2) * 8 + lh + (e & 3);
+    cb[e] = bias[n];
+    cs[e] = a.ystd[n];
+    cm[e] = a.ymean[n];
+  }
+  f32x16 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+    acc[k & 3] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[k], *reinterpret_cast<const bf16x8*>(x + k * 16),
+                                                         acc[k & 3], 0, 0, 0);
+  const int64_t b = b0 + rb * 32 + lr;
+  if (b < a.B) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int n = nb * 32 + (e >> 2) * 8 + lh + (e & 3);
+      if (n < a.nout) {
+        const float o = ((acc[0][e] + acc[1][e]) + (acc[2][e] + acc[3][e])) + cb[e];
+        a.ddq[n * a.ldd + b] = o * cs[e] + cm[e];
+      }
+    }
+  }
+}
+
+template<int H>
+__global__ __launch_bounds__(kInfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_infer(InferArgs a) {
+  static_assert(H == 128 || H == 256 || H == 512, "hidden width of the fused kernel");
+  extern __shared__ __attribute__((aligned(16))) bf16 X[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t b0 = (int64_t)blockIdx.x * kInfRows;
+  const int pitch = a.pitch;
+  inferFeatures(a, X, b0);
+  // hidden-layer biases into LDS behind the activations (their reads inside the products
+  // would otherwise wait behind the in-flight weight loads)
+  float* sbias = reinterpret_cast<float*>(X + kInfRows * pitch);
+  for (int i = tid; i < a.L * H; i += kInfThreads) sbias[i] = a.P[a.offB[i / H] + i % H];
+  __syncthreads();
+
+  // first layer over the padded input width, then H -> H
+  const bf16* W0 = a.Wb + a.offW[0];
+  switch (a.kin) {
+    case 64: inferLayer<H, 64>(X, pitch, W0, sbias, lane, wave); break;
+    case 128: inferLayer<H, 128>(X, pitch, W0, sbias, lane, wave); break;
+    case 192: inferLayer<H, 192>(X, pitch, W0, sbias, lane, wave); break;
+    default: inferLayer<H, 256>(X, pitch, W0, sbias, lane, wave); break;
+  }
+  for (int l = 1; l < a.L; ++l) inferLayer<H, H>(X, pitch, a.Wb + a.offW[l], sbias + l * H, lane, wave);
+
+  inferOutput<H>(a, X, b0, lane, wave);
 }
 
 // zero rows [B, Bpad) of a bf16 matrix (padding configurations of the last tile)

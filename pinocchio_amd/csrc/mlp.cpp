@@ -73,7 +73,8 @@ struct rbd_mlp {
   int64_t nparams = 0;                          // real parameter count
   int64_t step = 0;                             // Adam steps taken
   DevBuf<float> P, m1, m2, G, norm, lossAcc;    // norm: xmean, xinv, ymean, ystd, yinv
-  DevBuf<bf16> Pb;
+  DevBuf<bf16> Pb, Pf;  // bf16 copy; its fragment-major copy for the fused inference kernel
+  bool fragStale = true; // Pf lags Pb
   // activations / workspace (grown with the batch)
   DevBuf<bf16> X, act, dOut, dZ;
   DevBuf<float> Z, out, Y, tauChk, cq, cv, ct, cdd, rel;
@@ -84,7 +85,7 @@ struct rbd_mlp {
   ~rbd_mlp() {
     for (DevBuf<float>* b : {&P, &m1, &m2, &G, &norm, &lossAcc, &Z, &out, &Y, &tauChk, &cq, &cv, &ct, &cdd, &rel})
       b->release();
-    for (DevBuf<bf16>* b : {&Pb, &X, &act, &dOut, &dZ}) b->release();
+    for (DevBuf<bf16>* b : {&Pb, &Pf, &X, &act, &dOut, &dZ}) b->release();
     count.release();
     idx.release();
   }
@@ -157,6 +158,66 @@ void features(rbd_mlp* m, hipStream_t st, int64_t B, int64_t Bp, const float* q,
                        m->fm, B, q, ldq, v, ldv, tau, ldt, m->xmean(), m->xinv(), m->X.p, Xraw);
   if (Bp > B) hipLaunchKernelGGL(k_zero_rows, dim3(blocksFor((Bp - B) * m->kin)), dim3(256), 0, st, m->X.p,
                                  (int64_t)m->kin, B, Bp);
+}
+
+// The fused inference kernel takes hidden widths 128, 256 and 512 (the activations of its
+// 128 configurations stay in LDS) and at most 64 padded outputs.
+bool fusedInference(const rbd_mlp* m) {
+  return (m->H == 128 || m->H == 256 || m->H == 512) && m->kout <= kInfMaxOut && m->L + 1 <= kInfMaxLayers;
+}
+
+template<int H> void launchInfer(hipStream_t st, const InferArgs& a, size_t lds, unsigned grid) {
+  // dynamic LDS above the default 64 KB per workgroup needs the kernel attribute (set once)
+  static const hipError_t attr =
+      hipFuncSetAttribute((const void*)&k_mlp_infer<H>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (attr != hipSuccess) throw Error(RBD_EDEVICE, "fused inference: the device refuses 160 KB of LDS per workgroup");
+  hipLaunchKernelGGL(k_mlp_infer<H>, dim3(grid), dim3(kInfThreads), lds, st, a);
+}
+
+// ddq of B configurations through the fused kernel (no workspace)
+void infer(rbd_mlp* m, hipStream_t st, int64_t B, const float* q, int64_t ldq, const float* v, int64_t ldv,
+           const float* tau, int64_t ldt, float* ddq, int64_t ldd) {
+  InferArgs a{};
+  a.fm = m->fm;
+  a.B = B;
+  a.q = q;
+  a.ldq = ldq;
+  a.v = v;
+  a.ldv = ldv;
+  a.tau = tau;
+  a.ldt = ldt;
+  a.xmean = m->xmean();
+  a.xinv = m->xinv();
+  a.ymean = m->ymean();
+  a.ystd = m->ystd();
+  if (m->fragStale) {  // refresh the fragment-major weights (stream-ordered after the update)
+    bf16* F = m->Pf.get(m->total);
+    for (int l = 0; l <= m->L; ++l) {
+      const int64_t n = (int64_t)m->outP[l] * m->inP[l];
+      hipLaunchKernelGGL(k_pack_fragments, dim3(blocksFor(n)), dim3(256), 0, st, m->Pb.p + m->offW[l], F + m->offW[l],
+                         m->outP[l], m->inP[l]);
+    }
+    m->fragStale = false;
+  }
+  a.Wb = m->Pf.p;
+  a.P = m->P.p;
+  for (int l = 0; l <= m->L; ++l) {
+    a.offW[l] = m->offW[l];
+    a.offB[l] = m->offB[l];
+  }
+  a.L = m->L;
+  a.kin = m->kin;
+  a.kout = m->kout;
+  a.nout = m->nout;
+  a.pitch = std::max(m->kin, m->H) + 8;
+  a.ddq = ddq;
+  a.ldd = ldd;
+  const size_t lds = (size_t)kInfRows * a.pitch * sizeof(bf16) + (size_t)m->L * m->H * sizeof(float);
+  if (lds > 160 * 1024 || m->kin % 64 || m->kin > kMaxLdx) throw Error(RBD_EINVAL, "internal: network does not fit the fused kernel");
+  const unsigned grid = (unsigned)((B + kInfRows - 1) / kInfRows);
+  if (m->H == 512) launchInfer<512>(st, a, lds, grid);
+  else if (m->H == 256) launchInfer<256>(st, a, lds, grid);
+  else launchInfer<128>(st, a, lds, grid);
 }
 
 // forward pass over Bp rows of m->X; pre-activations kept when training
@@ -239,6 +300,7 @@ void lossGrad(rbd_mlp* m, hipStream_t st, int64_t B, int64_t Bp, const float* dd
 
 void uploadParamsBf16(rbd_mlp* m, hipStream_t st) {
   hipLaunchKernelGGL(k_to_bf16, dim3(blocksFor(m->total)), dim3(256), 0, st, m->total, m->P.p, m->Pb.p);
+  m->fragStale = true;
 }
 
 // packed (unpadded, per layer W row-major then b) <-> padded block
@@ -458,12 +520,16 @@ int rbd_mlp_predict(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q, int64_t
     checkVec(ddq, ldd, B, "ddq");
     if (B == 0) return;
     setDev(ctx);
-    const int64_t Bp = roundUp(B, kPadRows);
-    ensure(m, Bp, false);
-    features(m, ctx->stream, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
-    forward(m, ctx->stream, Bp, false);
-    hipLaunchKernelGGL(k_outputs, dim3(blocksFor(B * m->nout)), dim3(256), 0, ctx->stream, m->nout, m->kout, B,
-                       m->out.p, m->ymean(), m->ystd(), ddq, ldd);
+    if (fusedInference(m) && !(flags & RBD_MLP_LAYERED)) {
+      infer(m, ctx->stream, B, q, ldq, v, ldv, tau, ldt, ddq, ldd);
+    } else {
+      const int64_t Bp = roundUp(B, kPadRows);
+      ensure(m, Bp, false);
+      features(m, ctx->stream, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
+      forward(m, ctx->stream, Bp, false);
+      hipLaunchKernelGGL(k_outputs, dim3(blocksFor(B * m->nout)), dim3(256), 0, ctx->stream, m->nout, m->kout, B,
+                         m->out.p, m->ymean(), m->ystd(), ddq, ldd);
+    }
     HIP_CHECK(hipGetLastError());
     if (!(flags & RBD_ASYNC)) HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
@@ -495,6 +561,7 @@ int rbd_mlp_train_step(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q, int6
     const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)m->step)));
     hipLaunchKernelGGL(k_adam, dim3(blocksFor(m->total)), dim3(256), 0, st, m->total, m->P.p, m->m1.p, m->m2.p, m->G.p,
                        S, m->total, lr, b1, b2, eps, c1, c2, m->Pb.p);
+    m->fragStale = true;
     HIP_CHECK(hipGetLastError());
     if (!(flags & RBD_ASYNC)) HIP_CHECK(hipStreamSynchronize(st));
   });
@@ -558,12 +625,16 @@ int rbd_mlp_forward_dynamics(rbd_ctx* ctx, rbd_mlp* m, int64_t B, const float* q
     hipStream_t st = ctx->stream;
     const rbd_model* md = m->model;
     const int nq = md->nq, nv = md->nv;
-    const int64_t Bp = roundUp(B, kPadRows);
-    ensure(m, Bp, false);
-    features(m, st, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
-    forward(m, st, Bp, false);
-    hipLaunchKernelGGL(k_outputs, dim3(blocksFor(B * nv)), dim3(256), 0, st, nv, m->kout, B, m->out.p, m->ymean(),
-                       m->ystd(), ddq, ldd);
+    if (fusedInference(m)) {
+      infer(m, st, B, q, ldq, v, ldv, tau, ldt, ddq, ldd);
+    } else {
+      const int64_t Bp = roundUp(B, kPadRows);
+      ensure(m, Bp, false);
+      features(m, st, B, Bp, q, ldq, v, ldv, tau, ldt, nullptr);
+      forward(m, st, Bp, false);
+      hipLaunchKernelGGL(k_outputs, dim3(blocksFor(B * nv)), dim3(256), 0, st, nv, m->kout, B, m->out.p, m->ymean(),
+                         m->ystd(), ddq, ldd);
+    }
     float* tc = m->tauChk.get((size_t)nv * B);
     md->ops->rnea_f32(st, md->gen.get(), md->mc32, B, CView<float>{q, ldq, 1}, CView<float>{v, ldv, 1}, CView<float>{ddq, ldd, 1},
                       View<float>{tc, B, 1});

@@ -22,6 +22,7 @@ import numpy as np
 from . import Context, Engine, Model, _check, _i64, lib
 
 RBD_ASYNC = 0x2
+RBD_MLP_LAYERED = 0x40
 
 
 def _p(t):
@@ -113,13 +114,16 @@ class Surrogate:
         ys = Y.std(1).clamp_min(1e-6)
         self.set_normalization(X.mean(0).cpu().numpy(), xs.cpu().numpy(), Y.mean(1).cpu().numpy(), ys.cpu().numpy())
 
-    def predict(self, q, v, tau, out=None, async_=False):
+    def predict(self, q, v, tau, out=None, async_=False, layered=False):
+        """ddq = network(q, v, tau).  Hidden widths 128, 256 and 512 run as one fused kernel;
+        layered=True evaluates layer by layer with the training path's kernels."""
         import torch
         B = q.shape[1]
         out = torch.empty((self.nout, B), dtype=torch.float32, device=q.device) if out is None else out
+        flags = (RBD_ASYNC if async_ else 0) | (RBD_MLP_LAYERED if layered else 0)
         _check(lib().rbd_mlp_predict(self.ctx.handle, self.handle, _i64(B), _p(q), _ld(q, self.model.nq, "q"), _p(v),
                                      _ld(v, self.model.nv, "v"), _p(tau), _ld(tau, self.model.nv, "tau"), _p(out),
-                                     _ld(out, self.nout, "ddq"), ctypes.c_uint32(RBD_ASYNC if async_ else 0)))
+                                     _ld(out, self.nout, "ddq"), ctypes.c_uint32(flags)))
         return out
 
     def train_step(self, q, v, tau, ddq, lr, loss_acc=None, async_=True):

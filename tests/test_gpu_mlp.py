@@ -88,6 +88,52 @@ def test_forward_matches_oracle(quad, B):
         assert err < bar, (bf16, err)
 
 
+@pytest.mark.parametrize("hidden,nhidden,B", [(512, 3, 1000), (256, 2, 333), (128, 4, 129), (512, 1, 40000)])
+def test_fused_inference_matches_layered(quad, hidden, nhidden, B):
+    """The fused inference kernel (one launch, activations kept in LDS) against the
+    layer-by-layer kernels of the training path on the same inputs.  The hidden layers take
+    the same bf16 roundings over the same k order, so only the output layer's FP32
+    summation order may differ: max |difference| within 1e-5 of the output's magnitude.
+    Batches that are not whole 128-configuration tiles exercise the last tile."""
+    desc, model = quad
+    sur = S.Surrogate(model, hidden=hidden, nhidden=nhidden, seed=21)
+    sur.set_normalization(*random_norm(sur, 2))
+    q, v, tau = host_inputs(desc, B, 5)
+    Q, V, T = dev(q), dev(v), dev(tau)
+    fused = sur.predict(Q, V, T).cpu().numpy()
+    layered = sur.predict(Q, V, T, layered=True).cpu().numpy()
+    assert np.isfinite(fused).all()
+    err = np.abs(fused - layered).max() / np.abs(layered).max()
+    assert err < 1e-5, err
+
+
+def test_fused_inference_humanoid_strided(gpu_ctx):
+    """A free-flyer tree with more input features than hidden units (humanoid36: 145
+    features padded to 192 against H = 128) and two output blocks (36 outputs padded to
+    64), into a strided output whose extra columns must stay untouched; checked against
+    the layer-by-layer kernels and the bf16-emulating oracle."""
+    desc = M.humanoid36()
+    model = pa.Model(gpu_ctx, desc)
+    sur = S.Surrogate(model, hidden=128, nhidden=2, seed=8)
+    norm = random_norm(sur, 3)
+    sur.set_normalization(*norm)
+    B = 300
+    q, v, tau = host_inputs(desc, B, 6)
+    Q, V, T = dev(q), dev(v), dev(tau)
+    wide = torch.full((sur.nout, B + 9), 7.0, dtype=torch.float32, device="cuda")
+    sur.predict(Q, V, T, out=wide[:, :B])
+    fused = wide[:, :B].cpu().numpy()
+    assert (wide[:, B:] == 7.0).all()
+    layered = sur.predict(Q, V, T, layered=True).cpu().numpy()
+    assert np.abs(fused - layered).max() / np.abs(layered).max() < 1e-5
+    layers = mo.unpack(sur.params().astype(np.float64), sur.layer_shapes())
+    X = mo.features(desc, *(x.astype(np.float64) for x in (q, v, tau)))
+    Xn = mo.normalise_inputs(X, norm[0].astype(np.float32), norm[1].astype(np.float32), bf16=True)
+    o, _, _ = mo.forward(layers, Xn, bf16=True)
+    ref = o * norm[3].astype(np.float32) + norm[2].astype(np.float32)
+    assert np.abs(fused.T - ref).max() / np.abs(ref).max() < 2e-2
+
+
 @pytest.mark.parametrize("B", [256, 200])
 def test_gradients_match_oracle(quad, B):
     desc, model = quad
