@@ -371,8 +371,14 @@ __global__ __launch_bounds__(64) void k_features(FeatMap fm, int64_t B, const fl
 // configurations (WR = 8 / (H / 64) row groups).  k runs in 16-steps from 0 to K, the
 // bias and SiLU are those of the layer-by-layer kernels.  LDS: the activation rows
 // (128 x pitch bf16), then the hidden layers' biases (L x H fp32).
-constexpr int kInfRows = 128, kInfThreads = 512, kInfMaxOut = 64, kInfMaxLayers = 9;
+constexpr int kInfThreads = 512, kInfMaxOut = 64, kInfMaxLayers = 9;
 constexpr int kFragElems = 512;  // one 32 x 16 weight fragment
+// configurations per workgroup: 64 for hidden widths 256 and 512 (two workgroups per CU,
+// so that one's VALU phases -- features, SiLU epilogues, output -- overlap the other's
+// products; four waves per SIMD, 128 registers), 128 for width 128 (32-row blocks)
+constexpr int inferRows(int H) { return H >= 256 ? 64 : 128; }
+// weight prefetch distance in 16-k steps: 64 k with 256 registers, 32 k with 128
+template<int R> constexpr int kInfStageSteps = R == 128 ? 4 : 2;
 
 // Fragment-major copy of a layer's weights W[N][K] (N % 32 == 0, K % 16 == 0): fragment
 // (n / 32, k / 16), fragments of one 32-row block consecutive along k, holds its 512 values
@@ -403,28 +409,38 @@ struct InferArgs {
   int64_t ldd;
 };
 
-// features of the tile's configurations into X[s * pitch + f] (bf16, normalised; padding
-// features zero, configurations past the batch all zero).  Four groups of 128 threads
-// (two waves each): sin / cos of the revolute joints, the free flyer's 13 and the zero
-// padding, v, tau.
+// features of the tile's R configurations into X[s * pitch + f] (bf16, normalised;
+// padding features zero, configurations past the batch all zero).  The 512 threads form
+// 512 / R groups of R (one configuration each), cycling through four roles -- sin / cos of
+// the revolute joints, the free flyer's 13 and the zero padding, v, tau -- with a role's
+// inputs split between its groups when there are eight.
+template<int R>
 __device__ __forceinline__ void inferFeatures(const InferArgs& a, bf16* __restrict__ X, int64_t b0) {
+  constexpr int NREP = kInfThreads / R / 4;  // groups per role
   const FeatMap& fm = a.fm;
-  const int tid = threadIdx.x, s = tid & (kInfRows - 1);
-  const int part = __builtin_amdgcn_readfirstlane(tid >> 7);
+  const int tid = threadIdx.x, s = tid & (R - 1);
+  const int part = __builtin_amdgcn_readfirstlane(tid / R);
+  const int role = part & 3, rep = part >> 2;
   const int64_t b = b0 + s;
   const bool live = b < a.B;
   bf16* row = X + s * a.pitch;
   auto put = [&](int f, float raw) { row[f] = live ? (bf16)((raw - a.xmean[f]) * a.xinv[f]) : (bf16)0.f; };
+  auto split = [&](int n, int& lo, int& hi) {
+    lo = n * rep / NREP;
+    hi = n * (rep + 1) / NREP;
+  };
   const int fv = 2 * fm.nrev + (fm.hasFF ? 13 : 0);
   // inputs are read eight at a time, all eight requests in flight before the first is used
-  if (part == 0) {
-    for (int r0 = 0; r0 < fm.nrev; r0 += 8) {
+  if (role == 0) {
+    int lo, hi;
+    split(fm.nrev, lo, hi);
+    for (int r0 = lo; r0 < hi; r0 += 8) {
       float x[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) x[j] = (live && r0 + j < fm.nrev) ? a.q[fm.revq[r0 + j] * a.ldq + b] : 0.f;
+      for (int j = 0; j < 8; ++j) x[j] = (live && r0 + j < hi) ? a.q[fm.revq[r0 + j] * a.ldq + b] : 0.f;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        if (r0 + j < fm.nrev) {
+        if (r0 + j < hi) {
           float sn, cs;
           sincosT(x[j], &sn, &cs);
           put(r0 + j, sn);
@@ -432,8 +448,8 @@ __device__ __forceinline__ void inferFeatures(const InferArgs& a, bf16* __restri
         }
       }
     }
-  } else if (part == 1) {
-    if (fm.hasFF) {
+  } else if (role == 1) {
+    if (fm.hasFF && rep == 0) {
       float qx = 0.f, qy = 0.f, qz = 0.f, qw = 1.f;
       if (live) {
         qx = a.q[3 * a.ldq + b];
@@ -451,18 +467,21 @@ __device__ __forceinline__ void inferFeatures(const InferArgs& a, bf16* __restri
 #pragma unroll
       for (int k = 0; k < 9; ++k) put(f + 4 + k, r9[k]);
     }
-    for (int f = fm.nfeat; f < a.kin; ++f) row[f] = (bf16)0.f;
+    if (rep == NREP - 1)
+      for (int f = fm.nfeat; f < a.kin; ++f) row[f] = (bf16)0.f;
   } else {
-    const float* src = part == 2 ? a.v : a.tau;
-    const int64_t ld = part == 2 ? a.ldv : a.ldt;
-    const int f0 = part == 2 ? fv : fv + fm.nv;
-    for (int k0 = 0; k0 < fm.nv; k0 += 8) {
+    const float* src = role == 2 ? a.v : a.tau;
+    const int64_t ld = role == 2 ? a.ldv : a.ldt;
+    const int f0 = role == 2 ? fv : fv + fm.nv;
+    int lo, hi;
+    split(fm.nv, lo, hi);
+    for (int k0 = lo; k0 < hi; k0 += 8) {
       float x[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) x[j] = (live && k0 + j < fm.nv) ? src[(k0 + j) * ld + b] : 0.f;
+      for (int j = 0; j < 8; ++j) x[j] = (live && k0 + j < hi) ? src[(k0 + j) * ld + b] : 0.f;
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        if (k0 + j < fm.nv) put(f0 + k0 + j, x[j]);
+        if (k0 + j < hi) put(f0 + k0 + j, x[j]);
     }
   }
 }
@@ -472,18 +491,18 @@ __device__ __forceinline__ void inferFeatures(const InferArgs& a, bf16* __restri
 // B operand from the LDS activations.  K is a compile-time constant so that the k loop
 // unrolls completely: the weights of the next 64-k stage are requested before the current
 // stage's products (nothing loop-carried for the compiler to sink back to the point of use).
-template<int RB, int K>
+template<int RB, int K, int SK>
 __device__ __forceinline__ void inferProduct(f32x16 (&acc)[2][RB], const bf16* __restrict__ X, int pitch,
                                              const bf16* __restrict__ W, int n0, int r0, int lane) {
-  static_assert(K % 64 == 0, "whole 64-k stages");
-  constexpr int NST = K / 64, KS = K / 16;
+  static_assert(K % (16 * SK) == 0, "whole stages of SK k-steps");
+  constexpr int NST = K / (16 * SK), KS = K / 16;
   const int lr = lane & 31, lk = (lane >> 5) * 8;
   const bf16* w0 = W + (int64_t)(n0 / 32) * KS * kFragElems + lane * 8;
   const bf16* w1 = w0 + (int64_t)KS * kFragElems;
   const bf16* x = X + (r0 + lr) * pitch + lk;
-  bf16x8 wr[NST][4][2];
+  bf16x8 wr[NST][SK][2];
 #pragma unroll
-  for (int kk = 0; kk < 4; ++kk) {
+  for (int kk = 0; kk < SK; ++kk) {
     wr[0][kk][0] = *reinterpret_cast<const bf16x8*>(w0 + kk * kFragElems);
     wr[0][kk][1] = *reinterpret_cast<const bf16x8*>(w1 + kk * kFragElems);
   }
@@ -491,17 +510,17 @@ __device__ __forceinline__ void inferProduct(f32x16 (&acc)[2][RB], const bf16* _
   for (int st = 0; st < NST; ++st) {
     if (st + 1 < NST) {
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        wr[st + 1][kk][0] = *reinterpret_cast<const bf16x8*>(w0 + ((st + 1) * 4 + kk) * kFragElems);
-        wr[st + 1][kk][1] = *reinterpret_cast<const bf16x8*>(w1 + ((st + 1) * 4 + kk) * kFragElems);
+      for (int kk = 0; kk < SK; ++kk) {
+        wr[st + 1][kk][0] = *reinterpret_cast<const bf16x8*>(w0 + ((st + 1) * SK + kk) * kFragElems);
+        wr[st + 1][kk][1] = *reinterpret_cast<const bf16x8*>(w1 + ((st + 1) * SK + kk) * kFragElems);
       }
     }
     // keep those requests ahead of this stage's products (the scheduler would otherwise
     // move them next to their use and expose the L2 latency at every step)
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      const int k = st * 64 + kk * 16;
+    for (int kk = 0; kk < SK; ++kk) {
+      const int k = (st * SK + kk) * 16;
       bf16x8 xb[RB];
 #pragma unroll
       for (int r = 0; r < RB; ++r) xb[r] = *reinterpret_cast<const bf16x8*>(x + r * 32 * pitch + k);
@@ -516,11 +535,12 @@ __device__ __forceinline__ void inferProduct(f32x16 (&acc)[2][RB], const bf16* _
 
 // one hidden layer: products, then (after every wave has read the input) bias + SiLU back
 // into the activation rows
-template<int H, int K>
+template<int H, int K, int R>
 __device__ __forceinline__ void inferLayer(bf16* __restrict__ X, int pitch, const bf16* __restrict__ W,
                                            const float* __restrict__ bias, int lane, int wave) {
-  constexpr int WC = H / 64, WR = 8 / WC, RB = kInfRows / WR / 32;
-  const int n0 = (wave % WC) * 64, r0 = (wave / WC) * (kInfRows / WR);
+  constexpr int WC = H / 64, WR = 8 / WC, RB = R / WR / 32;
+  static_assert(RB >= 1, "32-row blocks");
+  const int n0 = (wave % WC) * 64, r0 = (wave / WC) * (R / WR);
   const int lr = lane & 31, lh = (lane >> 5) * 4;
   f32x16 acc[2][RB];
 #pragma unroll
@@ -529,7 +549,7 @@ __device__ __forceinline__ void inferLayer(bf16* __restrict__ X, int pitch, cons
     for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[j][r][e] = 0.f;
-  inferProduct<RB, K>(acc, X, pitch, W, n0, r0, lane);
+  inferProduct<RB, K, kInfStageSteps<R>>(acc, X, pitch, W, n0, r0, lane);
   __syncthreads();  // every wave has read the layer's input
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -553,21 +573,18 @@ __device__ __forceinline__ void inferLayer(bf16* __restrict__ X, int pitch, cons
 }
 
 // output layer: block (nb, rb) of 32 outputs x 32 configurations per wave (nb < kout / 32),
-// then ddq = (acc + bias) * std + mean stored along b.  Every weight of the block is
-// requested before the first product, and the k steps feed four independent accumulator
-// chains (summed at the end), so the block costs a few MFMA latencies, not 32.
-template<int H>
+// then ddq = (acc + bias) * std + mean stored along b.  The block's weights are requested
+// eight fragments at a time ahead of their products, and the k steps feed four
+// independent accumulator chains (summed at the end): a few MFMA latencies, not H / 16.
+template<int H, int R>
 __device__ __forceinline__ void inferOutput(const InferArgs& a, const bf16* __restrict__ X, int64_t b0, int lane,
                                             int wave) {
-  constexpr int NS = H / 16;
-  const int nb = wave >> 2, rb = wave & 3;
+  constexpr int NS = H / 16, NRB = R / 32;
+  const int nb = wave / NRB, rb = wave % NRB;
   if (nb * 32 >= a.kout) return;
   const int lr = lane & 31, lh = (lane >> 5) * 4;
   const bf16* W = a.Wb + a.offW[a.L] + (int64_t)nb * NS * kFragElems + lane * 8;
   const bf16* x = X + (rb * 32 + lr) * a.pitch + (lane >> 5) * 8;
-  bf16x8 wf[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) wf[k] = *reinterpret_cast<const bf16x8*>(W + k * kFragElems);
   const float* bias = a.P + a.offB[a.L];
   float cb[16], cs[16], cm[16];  // bias, std and mean of this lane's outputs (n < kout: in range)
 #pragma unroll
@@ -583,9 +600,15 @@ __device__ __forceinline__ void inferOutput(const InferArgs& a, const bf16* __re
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
 #pragma unroll
-  for (int k = 0; k < NS; ++k)
-    acc[k & 3] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[k], *reinterpret_cast<const bf16x8*>(x + k * 16),
-                                                         acc[k & 3], 0, 0, 0);
+  for (int k0 = 0; k0 < NS; k0 += 8) {  // eight weight fragments in flight at a time
+    bf16x8 wf[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) wf[k] = *reinterpret_cast<const bf16x8*>(W + (k0 + k) * kFragElems);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      acc[k & 3] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[k], *reinterpret_cast<const bf16x8*>(x + (k0 + k) * 16),
+                                                           acc[k & 3], 0, 0, 0);
+  }
   const int64_t b = b0 + rb * 32 + lr;
   if (b < a.B) {
 #pragma unroll
@@ -599,32 +622,33 @@ __device__ __forceinline__ void inferOutput(const InferArgs& a, const bf16* __re
   }
 }
 
-template<int H>
-__global__ __launch_bounds__(kInfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_infer(InferArgs a) {
-  static_assert(H == 128 || H == 256 || H == 512, "hidden width of the fused kernel");
+template<int H, int R>
+__global__ __launch_bounds__(kInfThreads) __attribute__((amdgpu_waves_per_eu(256 / R, 256 / R))) void k_mlp_infer(
+    InferArgs a) {
+  static_assert((H == 128 && R == 128) || ((H == 256 || H == 512) && R == 64), "fused kernel shapes");
   extern __shared__ __attribute__((aligned(16))) bf16 X[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int64_t b0 = (int64_t)blockIdx.x * kInfRows;
+  const int64_t b0 = (int64_t)blockIdx.x * R;
   const int pitch = a.pitch;
-  inferFeatures(a, X, b0);
+  inferFeatures<R>(a, X, b0);
   // hidden-layer biases into LDS behind the activations (their reads inside the products
   // would otherwise wait behind the in-flight weight loads)
-  float* sbias = reinterpret_cast<float*>(X + kInfRows * pitch);
+  float* sbias = reinterpret_cast<float*>(X + R * pitch);
   for (int i = tid; i < a.L * H; i += kInfThreads) sbias[i] = a.P[a.offB[i / H] + i % H];
   __syncthreads();
 
   // first layer over the padded input width, then H -> H
   const bf16* W0 = a.Wb + a.offW[0];
   switch (a.kin) {
-    case 64: inferLayer<H, 64>(X, pitch, W0, sbias, lane, wave); break;
-    case 128: inferLayer<H, 128>(X, pitch, W0, sbias, lane, wave); break;
-    case 192: inferLayer<H, 192>(X, pitch, W0, sbias, lane, wave); break;
-    default: inferLayer<H, 256>(X, pitch, W0, sbias, lane, wave); break;
+    case 64: inferLayer<H, 64, R>(X, pitch, W0, sbias, lane, wave); break;
+    case 128: inferLayer<H, 128, R>(X, pitch, W0, sbias, lane, wave); break;
+    case 192: inferLayer<H, 192, R>(X, pitch, W0, sbias, lane, wave); break;
+    default: inferLayer<H, 256, R>(X, pitch, W0, sbias, lane, wave); break;
   }
-  for (int l = 1; l < a.L; ++l) inferLayer<H, H>(X, pitch, a.Wb + a.offW[l], sbias + l * H, lane, wave);
+  for (int l = 1; l < a.L; ++l) inferLayer<H, H, R>(X, pitch, a.Wb + a.offW[l], sbias + l * H, lane, wave);
 
-  inferOutput<H>(a, X, b0, lane, wave);
+  inferOutput<H, R>(a, X, b0, lane, wave);
 }
 
 // zero rows [B, Bpad) of a bf16 matrix (padding configurations of the last tile)

@@ -167,11 +167,12 @@ bool fusedInference(const rbd_mlp* m) {
 }
 
 template<int H> void launchInfer(hipStream_t st, const InferArgs& a, size_t lds, unsigned grid) {
+  constexpr int R = inferRows(H);
   // dynamic LDS above the default 64 KB per workgroup needs the kernel attribute (set once)
   static const hipError_t attr =
-      hipFuncSetAttribute((const void*)&k_mlp_infer<H>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      hipFuncSetAttribute((const void*)&k_mlp_infer<H, R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (attr != hipSuccess) throw Error(RBD_EDEVICE, "fused inference: the device refuses 160 KB of LDS per workgroup");
-  hipLaunchKernelGGL(k_mlp_infer<H>, dim3(grid), dim3(kInfThreads), lds, st, a);
+  hipLaunchKernelGGL((k_mlp_infer<H, R>), dim3(grid), dim3(kInfThreads), lds, st, a);
 }
 
 // ddq of B configurations through the fused kernel (no workspace)
@@ -212,9 +213,10 @@ void infer(rbd_mlp* m, hipStream_t st, int64_t B, const float* q, int64_t ldq, c
   a.pitch = std::max(m->kin, m->H) + 8;
   a.ddq = ddq;
   a.ldd = ldd;
-  const size_t lds = (size_t)kInfRows * a.pitch * sizeof(bf16) + (size_t)m->L * m->H * sizeof(float);
+  const int R = inferRows(m->H);
+  const size_t lds = (size_t)R * a.pitch * sizeof(bf16) + (size_t)m->L * m->H * sizeof(float);
   if (lds > 160 * 1024 || m->kin % 64 || m->kin > kMaxLdx) throw Error(RBD_EINVAL, "internal: network does not fit the fused kernel");
-  const unsigned grid = (unsigned)((B + kInfRows - 1) / kInfRows);
+  const unsigned grid = (unsigned)((B + R - 1) / R);
   if (m->H == 512) launchInfer<512>(st, a, lds, grid);
   else if (m->H == 256) launchInfer<256>(st, a, lds, grid);
   else launchInfer<128>(st, a, lds, grid);
