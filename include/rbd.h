@@ -246,6 +246,18 @@ int rbd_frames(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, c
                int nframes, const int32_t* frame_joint, const double* frame_placement, int reference_frame,
                void* oMf, int64_t ldoMf, void* J, int64_t ldJ, uint32_t flags);
 
+/* Frame motions after forwardKinematics(q, v[, a]) for the same frame tables as rbd_frames:
+ * getFrameVelocity (frames.hxx:71-99) into vf, getFrameAcceleration (frames.hxx:102-130)
+ * into af and getFrameClassicalAcceleration (frames.hxx:133-149) into acf, each 6 rows per
+ * frame ([linear; angular], element 6 f + k) in the requested reference frame (RBD_LOCAL,
+ * RBD_WORLD or RBD_LOCAL_WORLD_ALIGNED).  a, af and acf may be NULL (af / acf need a);
+ * vf, af and acf share the leading dimension ldm.  Frame tables are host arrays (the call
+ * copies them and always synchronises). */
+int rbd_frame_motions(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                      const void* v, int64_t ldv, const void* a, int64_t lda, int nframes, const int32_t* frame_joint,
+                      const double* frame_placement, int reference_frame, void* vf, void* af, void* acf, int64_t ldm,
+                      uint32_t flags);
+
 /* forwardDynamics with acceleration constraints J ddq + gamma = 0 (contact-dynamics.hpp:
  * 66-75, contact-dynamics.hxx:90-167): M(q) and nle(q, v) as computeAllTerms gives them,
  * then ddq = M^-1 (tau - nle + J^T lambda) with

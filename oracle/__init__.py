@@ -131,6 +131,26 @@ def frames(model, q, parents, placements, reference_frame="world", scalar="f64",
     return oMf, J.transpose(0, 1, 3, 2).copy()
 
 
+def frame_motions(model, q, v, parents, placements, a=None, reference_frame="local", scalar="f64", nthreads=0):
+    """getFrameVelocity / getFrameAcceleration / getFrameClassicalAcceleration after
+    forwardKinematics(q, v[, a]) for frames given as (parent joint, placement [R row-major 9,
+    p 3]).  Returns vf, af, acf (B, nf, 6) [linear, angular]; af and acf are None without a."""
+    q, v = _cols(q, model.nq), _cols(v, model.nv)
+    aa = _cols(a, model.nv) if a is not None else None
+    B = q.shape[1]
+    par = np.ascontiguousarray(parents, dtype=np.int32)
+    pl = np.ascontiguousarray(placements, dtype=np.float64).reshape(len(par), 12)
+    nf = len(par)
+    vf = np.zeros((B, nf, 6))
+    af = np.zeros((B, nf, 6)) if a is not None else None
+    acf = np.zeros((B, nf, 6)) if a is not None else None
+    d = model.cstruct()
+    _check(lib().orc_frame_motions(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(v), _p(aa), int(nf), _p(par), _p(pl),
+                                   REFERENCE_FRAMES[reference_frame], _p(vf), _p(af), _p(acf), SCALARS[scalar],
+                                   nthreads))
+    return vf, af, acf
+
+
 def aba(model, q, v, tau, convention="world", scalar="f64", nthreads=0):
     q, v, tau = _cols(q, model.nq), _cols(v, model.nv), _cols(tau, model.nv)
     B = q.shape[1]

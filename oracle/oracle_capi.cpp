@@ -213,6 +213,32 @@ int framesT(const ModelDesc* d, int64_t B, const double* q, int nf, const int32_
     }
   });
 }
+template<class S>
+int frameMotionsT(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a, int nf,
+                  const int32_t* parent, const double* placement, int rf, double* vf, double* af, double* acf, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv, ba;
+    const S* aa = a ? ba.load(a + b * m.nv, m.nv) : nullptr;
+    forwardKinematics(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * m.nv, m.nv), aa);
+    for (int f = 0; f < nf; ++f) {
+      if (parent[f] < 0 || parent[f] >= m.njoints) throw std::invalid_argument("frame parent out of range");
+      SE3<S> X;
+      for (int k = 0; k < 9; ++k) X.R.m[k] = S(placement[12 * f + k]);
+      for (int k = 0; k < 3; ++k) X.p[k] = S(placement[12 * f + 9 + k]);
+      Motion<S> vel, acc, cl;
+      frameMotions(data, parent[f], X, rf, a != nullptr, vel, acc, cl);
+      S tmp[6];
+      toVec(vel, tmp);
+      store(tmp, vf + (b * nf + f) * 6, 6);
+      if (a) {
+        toVec(acc, tmp);
+        store(tmp, af + (b * nf + f) * 6, 6);
+        toVec(cl, tmp);
+        store(tmp, acf + (b * nf + f) * 6, 6);
+      }
+    }
+  });
+}
 }  // namespace
 
 extern "C" {
@@ -282,6 +308,20 @@ int orc_forward_kinematics(const ModelDesc* d, int64_t B, const double* q, const
 int orc_frames(const ModelDesc* d, int64_t B, const double* q, int nf, const int32_t* parent,
                const double* placement, int rf, double* oMf, double* J, int scalar, int nthreads) {
 #define CALL(S) framesT<S>(d, B, q, nf, parent, placement, rf, oMf, J, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_frame_motions(const ModelDesc* d, int64_t B, const double* q, const double* v, const double* a, int nf,
+                      const int32_t* parent, const double* placement, int rf, double* vf, double* af, double* acf,
+                      int scalar, int nthreads) {
+#define CALL(S) frameMotionsT<S>(d, B, q, v, a, nf, parent, placement, rf, vf, af, acf, nthreads)
   switch (scalar) {
     case 0: return CALL(double);
     case 1: return CALL(float);

@@ -964,6 +964,36 @@ inline void frameJacobian(const Model<S>& model, const Data<S>& data, int parent
 }
 
 // ----------------------------------------------------------------------------------------
+// Frame motions after forwardKinematics(q, v[, a]): getFrameVelocity (frames.hxx:71-99),
+// getFrameAcceleration (frames.hxx:102-130), getFrameClassicalAcceleration
+// (frames.hxx:133-149), for the frame (parent joint, placement):
+//   LOCAL                  placement^-1 . m_parent
+//   WORLD                  oMi[parent] . m_parent
+//   LOCAL_WORLD_ALIGNED    (R_i (m.lin + m.ang x placement.p), R_i m.ang)
+// with m = data.v / data.a of the parent joint; classical: acc.lin += vel.ang x vel.lin
+// (both in the requested frame).
+// ----------------------------------------------------------------------------------------
+template<class S>
+inline Motion<S> frameMotion(const Data<S>& data, int parent, const SE3<S>& placement, int rf, const Motion<S>& m) {
+  const SE3<S>& oMi = data.oMi[parent];
+  switch (rf) {
+    case RF_LOCAL: return actInv(placement, m);
+    case RF_WORLD: return act(oMi, m);
+    case RF_LOCAL_WORLD_ALIGNED: return Motion<S>{oMi.R * (m.lin + cross(m.ang, placement.p)), oMi.R * m.ang};
+  }
+  throw std::invalid_argument("Bad reference frame.");
+}
+template<class S>
+inline void frameMotions(const Data<S>& data, int parent, const SE3<S>& placement, int rf, bool withAcc, Motion<S>& vel,
+                         Motion<S>& acc, Motion<S>& classical) {
+  vel = frameMotion(data, parent, placement, rf, data.v[parent]);
+  if (!withAcc) return;
+  acc = frameMotion(data, parent, placement, rf, data.a[parent]);
+  classical = acc;
+  classical.lin = classical.lin + cross(vel.ang, vel.lin);
+}
+
+// ----------------------------------------------------------------------------------------
 // forwardDynamics with acceleration constraints J ddq + gamma = 0
 // (contact-dynamics.hxx:90-167): computeAllTerms gives M (CRBA) and nle (RNEA at a = 0),
 // then cholesky::decompose and

@@ -429,6 +429,30 @@ class Engine:
                                 ctypes.c_uint32(self._flags(False))))
         return oMf, J
 
+    def frame_motions(self, q, v, frame_joint, frame_placement, a=None, reference_frame="local"):
+        """getFrameVelocity, getFrameAcceleration and getFrameClassicalAcceleration after
+        forwardKinematics(q, v[, a]) for frames given by parent joint and placement (12 numbers
+        each): (vf, af, acf), 6 rows per frame ([linear; angular], row 6 f + k); af and acf
+        are None without a.  Synchronous."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        lda = self._vec(a, m.nv, "a")[1] if a is not None else 0
+        fj = np.ascontiguousarray(frame_joint, dtype=np.int32).ravel()
+        nf = len(fj)
+        fp = np.ascontiguousarray(frame_placement, dtype=np.float64).reshape(nf, 12)
+        vf = self._new_vec(q, 6 * nf, B)
+        af = self._new_vec(q, 6 * nf, B) if a is not None else None
+        acf = self._new_vec(q, 6 * nf, B) if a is not None else None
+        ldm = self._vec(vf, 6 * nf, "vf")[1]
+        _check(lib().rbd_frame_motions(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq), _ptr(v),
+                                       _i64(ldv), _ptr(a) if a is not None else None, _i64(lda), int(nf),
+                                       fj.ctypes.data_as(ctypes.c_void_p), fp.ctypes.data_as(ctypes.c_void_p),
+                                       int(self.REFERENCE_FRAMES[reference_frame]), _ptr(vf),
+                                       _ptr(af) if af is not None else None, _ptr(acf) if acf is not None else None,
+                                       _i64(ldm), ctypes.c_uint32(self._flags(False))))
+        return vf, af, acf
+
     def compute_minverse(self, q, out=None, async_=False):
         """M(q)^-1 per configuration (computeMinverse), FP32 or FP64 like q."""
         m = self.model

@@ -88,4 +88,87 @@ __global__ __launch_bounds__(256) void k_frames(int64_t B, int nv, const int32_t
   }
 }
 
+// Frame motions (getFrameVelocity / getFrameAcceleration / getFrameClassicalAcceleration,
+// frames.hxx:71-149) from the joint placements oMi and the joints' LOCAL velocities and
+// accelerations that k_kin writes (6 (j - 1) + k rows, [linear; angular]):
+//   LOCAL                placement^-1 . m
+//   WORLD                oMi . m
+//   LOCAL_WORLD_ALIGNED  (R_i (m.lin + m.ang x placement.p), R_i m.ang)
+// classical: acc.lin += vel.ang x vel.lin in the requested frame.  A frame on the universe
+// (parent 0) does not move.  One thread per (configuration, frame) as in k_frames; aj, af
+// and acf may be absent.
+template<class S>
+__device__ __forceinline__ void frameMotion(const S* R, const S* ti, const S* Xp, const S* Xt, int rf, const S* m,
+                                            S* o) {
+  if (rf == kRfLocal) {  // placement^-1 . m: (R_p^T (lin - t_p x ang), R_p^T ang)
+    const S l0 = m[0] - (Xt[1] * m[5] - Xt[2] * m[4]);
+    const S l1 = m[1] - (Xt[2] * m[3] - Xt[0] * m[5]);
+    const S l2 = m[2] - (Xt[0] * m[4] - Xt[1] * m[3]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      o[k] = Xp[k] * l0 + Xp[3 + k] * l1 + Xp[6 + k] * l2;
+      o[3 + k] = Xp[k] * m[3] + Xp[3 + k] * m[4] + Xp[6 + k] * m[5];
+    }
+    return;
+  }
+  S l[3] = {m[0], m[1], m[2]};
+  if (rf == kRfLocalWorldAligned) {  // lin + ang x t_p
+    l[0] += m[4] * Xt[2] - m[5] * Xt[1];
+    l[1] += m[5] * Xt[0] - m[3] * Xt[2];
+    l[2] += m[3] * Xt[1] - m[4] * Xt[0];
+  }
+  S w[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    w[r] = R[3 * r] * m[3] + R[3 * r + 1] * m[4] + R[3 * r + 2] * m[5];
+    o[r] = R[3 * r] * l[0] + R[3 * r + 1] * l[1] + R[3 * r + 2] * l[2];
+    o[3 + r] = w[r];
+  }
+  if (rf == kRfWorld) {  // oMi . m: R lin + t_i x (R ang)
+    o[0] += ti[1] * w[2] - ti[2] * w[1];
+    o[1] += ti[2] * w[0] - ti[0] * w[2];
+    o[2] += ti[0] * w[1] - ti[1] * w[0];
+  }
+}
+
+template<class S>
+__global__ __launch_bounds__(256) void k_frame_motions(int64_t B, const int32_t* __restrict__ fpar,
+                                                       const S* __restrict__ fpl, CView<S> oMi, CView<S> vj, CView<S> aj,
+                                                       int rf, View<S> vf, View<S> af, View<S> acf) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (b >= B) return;
+  const int p = fpar[f];
+  const S* X = fpl + 12 * f;
+  S vo[6] = {S(0), S(0), S(0), S(0), S(0), S(0)}, ao[6] = {S(0), S(0), S(0), S(0), S(0), S(0)};
+  if (p > 0) {
+    S R[9], ti[3], m[6];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = oMi.p[(int64_t)(12 * (p - 1) + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ti[k] = oMi.p[(int64_t)(12 * (p - 1) + 9 + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) m[k] = vj.p[(int64_t)(6 * (p - 1) + k) * vj.rs + b * vj.bs];
+    frameMotion(R, ti, X, X + 9, rf, m, vo);
+    if (aj.p) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) m[k] = aj.p[(int64_t)(6 * (p - 1) + k) * aj.rs + b * aj.bs];
+      frameMotion(R, ti, X, X + 9, rf, m, ao);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) vf.p[(int64_t)(6 * f + k) * vf.rs + b * vf.bs] = vo[k];
+  if (af.p) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) af.p[(int64_t)(6 * f + k) * af.rs + b * af.bs] = ao[k];
+  }
+  if (acf.p) {  // + vel.ang x vel.lin
+    ao[0] += vo[4] * vo[2] - vo[5] * vo[1];
+    ao[1] += vo[5] * vo[0] - vo[3] * vo[2];
+    ao[2] += vo[3] * vo[1] - vo[4] * vo[0];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acf.p[(int64_t)(6 * f + k) * acf.rs + b * acf.bs] = ao[k];
+  }
+}
+
 }  // namespace rbd

@@ -1028,6 +1028,58 @@ void framesCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int6
   finishCall(ctx, flags & ~(uint32_t)RBD_ASYNC, st);
 }
 
+// Frame motions: k_kin (oMi, LOCAL joint velocities and accelerations into a workspace),
+// then k_frame_motions.  Frame tables are host arrays, copied during the call (which
+// therefore synchronises).
+template<class S>
+void frameMotionsCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v,
+                      int64_t ldv, const void* a, int64_t lda, int nf, const int32_t* fjoint, const double* fplace,
+                      int rf, void* vf, void* af, void* acf, int64_t ldm, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const auto launch = f32 ? (void*)m->ops->kin_f32 : (void*)m->ops->kin_f64;
+  if (!launch) throw Error(RBD_EUNSUPPORTED, "frame motions: no kinematics kernel for this topology");
+  const int nj = m->njoints;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(m->nv, B, ldv, flags, "v");
+  ArgView<S> va{};
+  if (a) va = argView<S>(m->nv, B, lda, flags, "a");
+  auto vo = argView<S>(6 * nf, B, ldm, flags, "vf");
+  if (B == 0 || nf == 0) return;
+  std::vector<S> pl((size_t)12 * nf);
+  for (size_t k = 0; k < pl.size(); ++k) pl[k] = S(fplace[k]);
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* da = a ? (const S*)st.in(a, va.extent * sizeof(S)) : nullptr;
+  const bool pre = (flags & RBD_LAYOUT_BATCH) != 0 || ldm != 6 * nf;
+  S* dvf = (S*)st.out(vf, vo.extent * sizeof(S), pre);
+  S* daf = af ? (S*)st.out(af, vo.extent * sizeof(S), pre) : nullptr;
+  S* dacf = acf ? (S*)st.out(acf, vo.extent * sizeof(S), pre) : nullptr;
+  auto table = [&](const void* h, size_t bytes) {
+    void* d = ctx->stageBuf(st.slot++, bytes);
+    HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return d;
+  };
+  const int32_t* dpar = (const int32_t*)table(fjoint, sizeof(int32_t) * nf);
+  const S* dpl = (const S*)table(pl.data(), sizeof(S) * pl.size());
+  // workspace, batch-fastest: oMi (12 (nj - 1) rows), joint velocities, accelerations (6 (nj - 1) each)
+  const size_t nO = (size_t)12 * (nj - 1) * B, nV = (size_t)6 * (nj - 1) * B;
+  S* wO = (S*)ctx->workspace((nO + (a ? 2 : 1) * nV) * sizeof(S));
+  S* wV = wO + nO;
+  S* wA = a ? wV + nV : nullptr;
+  const S* mc;
+  if constexpr (f32) mc = m->mc32;
+  else mc = m->mc64;
+  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, m->gen.get(), mc, B, CView<S>{dq, vq.rs, vq.bs},
+                                        CView<S>{dv, vv.rs, vv.bs}, CView<S>{da, va.rs, va.bs}, View<S>{wO, B, 1},
+                                        View<S>{wV, B, 1}, View<S>{wA, B, 1}, View<S>{});
+  hipLaunchKernelGGL(k_frame_motions<S>, dim3((unsigned)((B + 255) / 256), (unsigned)nf), dim3(256), 0, ctx->stream,
+                     B, dpar, dpl, CView<S>{wO, B, 1}, CView<S>{wV, B, 1}, CView<S>{wA, B, 1}, rf,
+                     View<S>{dvf, vo.rs, vo.bs}, View<S>{daf, vo.rs, vo.bs}, View<S>{dacf, vo.rs, vo.bs});
+  finishCall(ctx, flags & ~(uint32_t)RBD_ASYNC, st);
+}
+
 template<class S> __global__ void k_fill_zero(S* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = S(0);
@@ -1376,6 +1428,28 @@ int rbd_frames(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const voi
     else
       framesCall<double>(ctx, m, B, q, ldq, nframes, frame_joint, frame_placement, reference_frame, oMf, ldoMf, J,
                          ldJ, flags);
+  });
+}
+
+int rbd_frame_motions(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                      const void* v, int64_t ldv, const void* a, int64_t lda, int nframes, const int32_t* frame_joint,
+                      const double* frame_placement, int reference_frame, void* vf, void* af, void* acf, int64_t ldm,
+                      uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(nframes >= 0 && nframes <= 65535, "the frame count must be between 0 and 65535");
+    RBD_CHECK_ARG(nframes == 0 || (frame_joint && frame_placement), "null frame table");
+    RBD_CHECK_ARG(reference_frame >= RBD_WORLD && reference_frame <= RBD_LOCAL_WORLD_ALIGNED, "Bad reference frame.");
+    for (int f = 0; f < nframes; ++f)
+      RBD_CHECK_ARG(frame_joint[f] >= 0 && frame_joint[f] < m->njoints, "The index of the Joint is outside the bounds.");
+    RBD_CHECK_ARG(B == 0 || nframes == 0 || (q && v && vf), "null array argument");
+    RBD_CHECK_ARG(a || (!af && !acf), "frame accelerations need the joint accelerations a");
+    if (dtype == RBD_F32)
+      frameMotionsCall<float>(ctx, m, B, q, ldq, v, ldv, a, lda, nframes, frame_joint, frame_placement, reference_frame,
+                              vf, af, acf, ldm, flags);
+    else
+      frameMotionsCall<double>(ctx, m, B, q, ldq, v, ldv, a, lda, nframes, frame_joint, frame_placement,
+                               reference_frame, vf, af, acf, ldm, flags);
   });
 }
 

@@ -101,3 +101,63 @@ def test_frames_host_pointers(gpu_ctx):
     Jg = J[:, :6 * desc.nv * nf].reshape(B, nf, desc.nv, 6).transpose(0, 1, 3, 2)
     assert rel(Jg, ref_J) < 1e-12
     assert (oMf[:, 12 * nf:] == 7.0).all() and (J[:, 6 * desc.nv * nf:] == 7.0).all()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("rf", ["world", "local", "local_world_aligned"])
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "arm7"])
+def test_frame_motions_parity(gpu_ctx, name, rf, layout, prec):
+    """getFrameVelocity / getFrameAcceleration / getFrameClassicalAcceleration
+    (frames.hxx:71-149) after forwardKinematics(q, v, a) against the oracle, same bars."""
+    desc, model = gpu_model(gpu_ctx, name)
+    B, nf = 257, 5
+    q, v, a, _ = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 881)
+    par, pl = random_frames(desc, nf, 7)
+    if prec == "f32":
+        q, v, a = f32_round(q, v, a)
+        pl = np.asarray(pl, dtype=np.float32).astype(np.float64)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    eng = pa.Engine(model, layout)
+    dev = lambda x: torch.tensor(np.ascontiguousarray(x.T if layout == "col" else x), dtype=dt, device="cuda")
+    outs = eng.frame_motions(dev(q), dev(v), par, pl, a=dev(a), reference_frame=rf)
+    refs = oracle.frame_motions(desc, q, v, par, pl, a=a, reference_frame=rf)
+    for what, o, r in zip(("v", "a", "a_classical"), outs, refs):
+        o = o.double().cpu().numpy()
+        o = (o.T if layout == "batch" else o).reshape(B, nf, 6)
+        e = rel(o, r)
+        print(f"{name} {rf} {layout} {prec} {what}: {e:.2e}")
+        assert e < TOL[prec], what
+    # velocity only (no accelerations requested)
+    vf, af, acf = eng.frame_motions(dev(q), dev(v), par, pl, reference_frame=rf)
+    assert af is None and acf is None
+    o = vf.double().cpu().numpy()
+    assert rel((o.T if layout == "batch" else o).reshape(B, nf, 6), refs[0]) < TOL[prec]
+
+
+def test_frame_motions_arguments(gpu_ctx):
+    desc, model = gpu_model(gpu_ctx, "arm7")
+    eng = pa.Engine(model, "batch")
+    Q = torch.zeros((desc.nq, 64), device="cuda")
+    V = torch.ones((desc.nv, 64), device="cuda")
+    pl = np.concatenate([np.eye(3).ravel(), [0.1, 0.2, 0.3]])[None]
+    with pytest.raises(ValueError, match="outside the bounds"):
+        eng.frame_motions(Q, V, [desc.njoints], pl)
+    with pytest.raises(ValueError, match="Bad reference frame"):
+        _check_raw_rf(eng, Q, V, pl)
+    vf, _, _ = eng.frame_motions(Q, V, [0], pl)  # on the universe
+    assert float(vf.abs().max()) == 0.0
+
+
+def _check_raw_rf(eng, Q, V, pl):
+    import ctypes
+    m = eng.model
+    fj = np.zeros(1, np.int32)
+    fp = np.ascontiguousarray(pl, dtype=np.float64)
+    out = torch.empty((6, Q.shape[1]), device="cuda")
+    pa._check(pa.lib().rbd_frame_motions(m.ctx.handle, m.handle, ctypes.c_int64(Q.shape[1]), 0,
+                                         ctypes.c_void_p(Q.data_ptr()), ctypes.c_int64(Q.shape[1]),
+                                         ctypes.c_void_p(V.data_ptr()), ctypes.c_int64(V.shape[1]), None,
+                                         ctypes.c_int64(0), 1, fj.ctypes.data_as(ctypes.c_void_p),
+                                         fp.ctypes.data_as(ctypes.c_void_p), 7, ctypes.c_void_p(out.data_ptr()), None,
+                                         None, ctypes.c_int64(Q.shape[1]), ctypes.c_uint32(pa.RBD_LAYOUT_BATCH)))

@@ -90,3 +90,93 @@ def test_jacobians_finite_differences(name):
         R = oMf[0, f, :9].reshape(3, 3)
         assert np.abs(Jl[0, f, :3] - R.T @ Jlwa[0, f, :3]).max() < 1e-12
         assert np.abs(Jl[0, f, 3:] - R.T @ Jlwa[0, f, 3:]).max() < 1e-12
+
+
+# ---- frame motions (getFrameVelocity / getFrameAcceleration / getFrameClassicalAcceleration,
+# frames.hxx:71-149), pinned by:
+#   * velocity = frame Jacobian x v in each reference frame (the Jacobians are pinned above);
+#   * LOCAL_WORLD_ALIGNED velocity = d/dt of the frame's world origin and orientation along
+#     q(t) = integrate(q, v t) (central differences);
+#   * classical linear and spatial angular acceleration in LOCAL_WORLD_ALIGNED = d/dt of that
+#     velocity along a trajectory whose joint velocities are v + a t (revolute coordinates
+#     q + v t + a t^2 / 2; a free flyer's body twist held constant, so its a is zero);
+#   * LOCAL = LOCAL_WORLD_ALIGNED seen in the frame's axes; WORLD linear = LWA linear plus
+#     the frame origin x angular.
+
+def _curve(desc, q, v, a, t):
+    """configuration and joint velocity at time t along the trajectory described above"""
+    dq = v * t + 0.5 * a * t * t
+    return oracle.integrate(desc, q, dq), v + a * t
+
+
+def _ff_zero(desc, a):
+    a = a.copy()
+    for j in range(1, desc.njoints):
+        if desc.nvs[j] == 6:
+            a[desc.idx_v[j]:desc.idx_v[j] + 6] = 0.0
+    return a
+
+
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "arm7"])
+def test_frame_velocity_is_jacobian_times_v(name):
+    d = getattr(M, name)()
+    q, v, _, _ = M.sample_inputs(d, 3, 90)
+    par, pl = random_frames(d, 6, 4)
+    for rf in ("local", "world", "local_world_aligned"):
+        vf, af, acf = oracle.frame_motions(d, q, v, par, pl, reference_frame=rf)
+        assert af is None and acf is None
+        _, J = oracle.frames(d, q, par, pl, rf)
+        for b in range(3):
+            ref = np.einsum("frc,c->fr", J[b], v[:, b])
+            assert np.abs(vf[b] - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), rf
+    assert np.abs(vf[:, 1]).max() == 0.0  # a frame on the universe does not move
+
+
+@pytest.mark.parametrize("name", ["humanoid36", "arm7", "quadruped18"])
+def test_frame_motions_finite_differences(name):
+    d = getattr(M, name)()
+    q, v, a, _ = M.sample_inputs(d, 1, 91)
+    a = _ff_zero(d, a)
+    par, pl = random_frames(d, 5, 5)
+    vf, af, acf = oracle.frame_motions(d, q, v, par, pl, a=a, reference_frame="local_world_aligned")
+    oMf, _ = oracle.frames(d, q, par, pl)
+    h = 1e-5
+    (qp, vp), (qm, vm) = _curve(d, q, v, a, h), _curve(d, q, v, a, -h)
+    Mp, _ = oracle.frames(d, qp, par, pl)
+    Mm, _ = oracle.frames(d, qm, par, pl)
+    vfp, _, _ = oracle.frame_motions(d, qp, vp, par, pl, reference_frame="local_world_aligned")
+    vfm, _, _ = oracle.frame_motions(d, qm, vm, par, pl, reference_frame="local_world_aligned")
+    for f in range(len(par)):
+        R0 = oMf[0, f, :9].reshape(3, 3)
+        dp = (Mp[0, f, 9:] - Mm[0, f, 9:]) / (2 * h)
+        W = ((Mp[0, f, :9] - Mm[0, f, :9]).reshape(3, 3) / (2 * h)) @ R0.T
+        fd_v = np.concatenate([dp, [W[2, 1], W[0, 2], W[1, 0]]])
+        scale = max(1.0, np.abs(vf[0, f]).max())
+        assert np.abs(fd_v - vf[0, f]).max() < 1e-7 * scale, (f, fd_v, vf[0, f])
+        fd_a = (vfp[0, f] - vfm[0, f]) / (2 * h)
+        scale = max(1.0, np.abs(acf[0, f]).max())
+        assert np.abs(fd_a[:3] - acf[0, f, :3]).max() < 1e-6 * scale, (f, fd_a, acf[0, f])
+        assert np.abs(fd_a[3:] - af[0, f, 3:]).max() < 1e-6 * scale
+    # the spatial and classical accelerations differ by exactly omega x v in the linear part
+    assert np.abs(acf[..., 3:] - af[..., 3:]).max() == 0.0
+
+
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18"])
+def test_frame_motions_between_reference_frames(name):
+    d = getattr(M, name)()
+    q, v, a, _ = M.sample_inputs(d, 2, 92)
+    par, pl = random_frames(d, 6, 6)
+    oMf, _ = oracle.frames(d, q, par, pl)
+    out = {rf: oracle.frame_motions(d, q, v, par, pl, a=a, reference_frame=rf)
+           for rf in ("local", "world", "local_world_aligned")}
+    for b in range(2):
+        for f in range(len(par)):
+            R, p = oMf[b, f, :9].reshape(3, 3), oMf[b, f, 9:]
+            for k in range(3):  # vf, af, acf
+                lwa, loc, wor = (out[rf][k][b, f] for rf in ("local_world_aligned", "local", "world"))
+                assert np.abs(loc[:3] - R.T @ lwa[:3]).max() < 1e-12 * max(1.0, np.abs(lwa).max())
+                assert np.abs(loc[3:] - R.T @ lwa[3:]).max() < 1e-12 * max(1.0, np.abs(lwa).max())
+                assert np.abs(wor[3:] - lwa[3:]).max() < 1e-12 * max(1.0, np.abs(lwa).max())
+                if k < 2:  # spatial motions: WORLD linear = LWA linear + p x angular
+                    assert np.abs(wor[:3] - (lwa[:3] + np.cross(p, lwa[3:]))).max() < 1e-12 * max(
+                        1.0, np.abs(lwa).max(), np.abs(wor).max())
