@@ -23,6 +23,9 @@
  *   rbd_joint_jacobians   computeJointJacobians     algorithm/jacobian.hxx:23-62
  *   rbd_frames            updateFramePlacements,    algorithm/frames.hxx:15-35,
  *                         getFrameJacobian          algorithm/frames.hxx:156-175
+ *   rbd_frame_motions     getFrameVelocity,         algorithm/frames.hxx:71-99,
+ *                         getFrameAcceleration,     algorithm/frames.hxx:102-130,
+ *                         getFrameClassicalAcceleration  algorithm/frames.hxx:133-149
  *   rbd_constraint_dynamics constraintDynamics      algorithm/constrained-dynamics.hxx:193-500
  *   rbd_delassus          computeDelassusMatrix     algorithm/delassus.hpp:41-60, delassus.hxx:118-300
  *   rbd_damped_delassus_inverse
