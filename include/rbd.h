@@ -21,6 +21,8 @@
  *   rbd_compute_minverse  computeMinverse           algorithm/aba.hxx:608-902
  *   rbd_forward_kinematics forwardKinematics        algorithm/kinematics.hxx:121-145, 209-238
  *   rbd_joint_jacobians   computeJointJacobians     algorithm/jacobian.hxx:23-62
+ *   rbd_joint_jacobians_time_variation
+ *                         computeJointJacobiansTimeVariation  algorithm/jacobian.hxx:673-740
  *   rbd_frames            updateFramePlacements,    algorithm/frames.hxx:15-35,
  *                         getFrameJacobian          algorithm/frames.hxx:156-175
  *   rbd_frame_motions     getFrameVelocity,         algorithm/frames.hxx:71-99,
@@ -228,6 +230,14 @@ int rbd_forward_kinematics(rbd_ctx* ctx, const rbd_model* model, int64_t batch, 
  * configuration, column of dof c = oMi.act(S_c); element index r + 6 c. */
 int rbd_joint_jacobians(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
                         void* J, int64_t ldJ, uint32_t flags);
+
+/* computeJointJacobiansTimeVariation (jacobian.hxx:673-740): the WORLD-frame Jacobian J
+ * (optional, as rbd_joint_jacobians writes it) and its time variation dJ, 6 x nv per
+ * configuration, element index r + 6 c; dJ column c of joint j = ov_j x J_c with ov_j the
+ * joint's spatial velocity in WORLD. */
+int rbd_joint_jacobians_time_variation(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                                       int64_t ldq, const void* v, int64_t ldv, void* J, int64_t ldJ, void* dJ,
+                                       int64_t lddJ, uint32_t flags);
 
 /* computeMinverse (aba.hxx:608-902; cholesky::computeMinv, cholesky.hxx:522-597): M(q)^-1
  * per configuration, written whole (symmetric; the reference fills the upper triangle).

@@ -112,6 +112,18 @@ def joint_jacobians(model, q, scalar="f64", nthreads=0):
     return J.transpose(0, 2, 1).copy()
 
 
+def joint_jacobians_time_variation(model, q, v, scalar="f64", nthreads=0):
+    """(J, dJ), each (B, 6, nv): computeJointJacobiansTimeVariation (WORLD frame)."""
+    q, v = _cols(q, model.nq), _cols(v, model.nv)
+    B = q.shape[1]
+    J = np.zeros((B, model.nv, 6))
+    dJ = np.zeros((B, model.nv, 6))
+    d = model.cstruct()
+    _check(lib().orc_joint_jacobians_time_variation(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(v), _p(J), _p(dJ),
+                                                    SCALARS[scalar], nthreads))
+    return J.transpose(0, 2, 1).copy(), dJ.transpose(0, 2, 1).copy()
+
+
 REFERENCE_FRAMES = {"world": 0, "local": 1, "local_world_aligned": 2}
 
 

@@ -189,6 +189,17 @@ template<class S> int jacT(const ModelDesc* d, int64_t B, const double* q, doubl
     storeMat(data.J, J + b * 6 * m.nv);
   });
 }
+template<class S> int jacDotT(const ModelDesc* d, int64_t B, const double* q, const double* v, double* J, double* dJ,
+                               int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv;
+    static thread_local Mat<S> dJm;
+    if (dJm.rows != 6 || dJm.cols != m.nv) dJm = Mat<S>(6, m.nv);
+    computeJointJacobiansTimeVariation(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * m.nv, m.nv), dJm);
+    storeMat(data.J, J + b * 6 * m.nv);
+    storeMat(dJm, dJ + b * 6 * m.nv);
+  });
+}
 template<class S>
 int framesT(const ModelDesc* d, int64_t B, const double* q, int nf, const int32_t* parent, const double* placement,
             int rf, double* oMf, double* J, int nt) {
@@ -322,6 +333,19 @@ int orc_frame_motions(const ModelDesc* d, int64_t B, const double* q, const doub
                       const int32_t* parent, const double* placement, int rf, double* vf, double* af, double* acf,
                       int scalar, int nthreads) {
 #define CALL(S) frameMotionsT<S>(d, B, q, v, a, nf, parent, placement, rf, vf, af, acf, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_joint_jacobians_time_variation(const ModelDesc* d, int64_t B, const double* q, const double* v, double* J,
+                                       double* dJ, int scalar, int nthreads) {
+#define CALL(S) jacDotT<S>(d, B, q, v, J, dJ, nthreads)
   switch (scalar) {
     case 0: return CALL(double);
     case 1: return CALL(float);

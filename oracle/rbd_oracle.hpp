@@ -624,6 +624,23 @@ template<class S> inline void computeJointJacobians(const Model<S>& model, Data<
   }
 }
 
+// computeJointJacobiansTimeVariation, jacobian.hxx:673-740 (JointJacobiansTimeVariation
+// ForwardStep): J as above, ov_i = oMi.act(v_i) the joint's spatial velocity in WORLD, and
+// each of joint i's dJ columns = ov_i x J_col (motion action).  dJ is 6 x nv like J.
+template<class S>
+inline void computeJointJacobiansTimeVariation(const Model<S>& model, Data<S>& data, const S* q, const S* v, Mat<S>& dJ) {
+  forwardKinematics(model, data, q, v, static_cast<const S*>(nullptr));
+  for (int i = 1; i < model.njoints; ++i) {
+    const JointCalc<S> jc = jointCalc(model, i, q, static_cast<const S*>(nullptr));
+    const Motion<S> ov = act(data.oMi[i], data.v[i]);
+    for (int k = 0; k < jc.nv; ++k) {
+      const Motion<S> Jc = act(data.oMi[i], jc.Scol[k]);
+      setCol(data.J, model.idx_v[i] + k, Jc);
+      setCol(dJ, model.idx_v[i] + k, crossMM(ov, Jc));
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------------------
 // RNEA, rnea.hxx:17-161 (LOCAL frames)
 // ----------------------------------------------------------------------------------------

@@ -407,6 +407,22 @@ class Engine:
                                          _i64(ldj), ctypes.c_uint32(self._flags(async_))))
         return J
 
+    def joint_jacobians_time_variation(self, q, v, jacobians=True):
+        """computeJointJacobiansTimeVariation: (J, dJ), WORLD-frame 6 x nv per configuration
+        (element r + 6 c); J is None when jacobians=False.  Synchronous."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        dJ = self._new_vec(q, 6 * m.nv, B)
+        J = self._new_vec(q, 6 * m.nv, B) if jacobians else None
+        _, ldd = self._vec(dJ, 6 * m.nv, "dJ")
+        ldj = self._vec(J, 6 * m.nv, "J")[1] if jacobians else 0
+        _check(lib().rbd_joint_jacobians_time_variation(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q),
+                                                        _i64(ldq), _ptr(v), _i64(ldv), _ptr(J) if jacobians else None,
+                                                        _i64(ldj), _ptr(dJ), _i64(ldd),
+                                                        ctypes.c_uint32(self._flags(False))))
+        return J, dJ
+
     REFERENCE_FRAMES = {"world": 0, "local": 1, "local_world_aligned": 2}
 
     def frames(self, q, frame_joint, frame_placement, reference_frame="world", placements=True, jacobians=True):

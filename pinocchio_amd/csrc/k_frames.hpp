@@ -171,4 +171,48 @@ __global__ __launch_bounds__(256) void k_frame_motions(int64_t B, const int32_t*
   }
 }
 
+// computeJointJacobiansTimeVariation (jacobian.hxx:673-740) from k_kin's oMi, LOCAL joint
+// velocities and WORLD Jacobian: column c of joint j = dofJoint[c] gets
+// dJ_c = ov_j x J_c, ov_j = oMi_j . v_j (motion action).  One thread per (configuration,
+// column); J is copied to Jout when that is present.
+template<class S>
+__global__ __launch_bounds__(256) void k_jacobian_dot(int64_t B, const int32_t* __restrict__ dofJoint, CView<S> oMi,
+                                                      CView<S> vj, CView<S> Jw, View<S> dJ, View<S> Jout) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (b >= B) return;
+  const int j = dofJoint[c];
+  S R[9], t[3], m[6], Jc[6];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = oMi.p[(int64_t)(12 * (j - 1) + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) t[k] = oMi.p[(int64_t)(12 * (j - 1) + 9 + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) m[k] = vj.p[(int64_t)(6 * (j - 1) + k) * vj.rs + b * vj.bs];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) Jc[k] = Jw.p[(int64_t)(6 * c + k) * Jw.rs + b * Jw.bs];
+  S w[3], l[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    w[r] = R[3 * r] * m[3] + R[3 * r + 1] * m[4] + R[3 * r + 2] * m[5];
+    l[r] = R[3 * r] * m[0] + R[3 * r + 1] * m[1] + R[3 * r + 2] * m[2];
+  }
+  l[0] += t[1] * w[2] - t[2] * w[1];
+  l[1] += t[2] * w[0] - t[0] * w[2];
+  l[2] += t[0] * w[1] - t[1] * w[0];
+  S o[6];  // (w x J.lin + l x J.ang, w x J.ang)
+  o[0] = w[1] * Jc[2] - w[2] * Jc[1] + l[1] * Jc[5] - l[2] * Jc[4];
+  o[1] = w[2] * Jc[0] - w[0] * Jc[2] + l[2] * Jc[3] - l[0] * Jc[5];
+  o[2] = w[0] * Jc[1] - w[1] * Jc[0] + l[0] * Jc[4] - l[1] * Jc[3];
+  o[3] = w[1] * Jc[5] - w[2] * Jc[4];
+  o[4] = w[2] * Jc[3] - w[0] * Jc[5];
+  o[5] = w[0] * Jc[4] - w[1] * Jc[3];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) dJ.p[(int64_t)(6 * c + k) * dJ.rs + b * dJ.bs] = o[k];
+  if (Jout.p) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Jout.p[(int64_t)(6 * c + k) * Jout.rs + b * Jout.bs] = Jc[k];
+  }
+}
+
 }  // namespace rbd

@@ -1080,6 +1080,52 @@ void frameMotionsCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q
   finishCall(ctx, flags & ~(uint32_t)RBD_ASYNC, st);
 }
 
+// Jacobian time variation: k_kin (oMi, LOCAL joint velocities, WORLD Jacobian into a
+// workspace), then k_jacobian_dot.
+template<class S>
+void jacobianDotCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v,
+                     int64_t ldv, void* J, int64_t ldJ, void* dJ, int64_t lddJ, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const auto launch = f32 ? (void*)m->ops->kin_f32 : (void*)m->ops->kin_f64;
+  if (!launch) throw Error(RBD_EUNSUPPORTED, "Jacobian time variation: no kinematics kernel for this topology");
+  const int nj = m->njoints, nv = m->nv;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v");
+  auto vd = argView<S>(6 * nv, B, lddJ, flags, "dJ");
+  ArgView<S> vJ{};
+  if (J) vJ = argView<S>(6 * nv, B, ldJ, flags, "J");
+  if (B == 0) return;
+  std::vector<int32_t> dofJoint(nv);
+  for (int j = 1; j < nj; ++j) {
+    const int nvj = m->jtype[j] == RBD_JOINT_FREEFLYER ? 6 : 1;
+    for (int k = 0; k < nvj; ++k) dofJoint[m->iv[j] + k] = j;
+  }
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  const bool bl = (flags & RBD_LAYOUT_BATCH) != 0;
+  S* ddJ = (S*)st.out(dJ, vd.extent * sizeof(S), bl || lddJ != 6 * nv);
+  S* dJo = J ? (S*)st.out(J, vJ.extent * sizeof(S), bl || ldJ != 6 * nv) : nullptr;
+  void* dtab = ctx->stageBuf(st.slot++, sizeof(int32_t) * nv);
+  HIP_CHECK(hipMemcpyAsync(dtab, dofJoint.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice, ctx->stream));
+  // workspace, batch-fastest: oMi (12 (nj - 1) rows), joint velocities (6 (nj - 1)), WORLD Jacobian (6 nv)
+  const size_t nO = (size_t)12 * (nj - 1) * B, nV = (size_t)6 * (nj - 1) * B, nJ = (size_t)6 * nv * B;
+  S* wO = (S*)ctx->workspace((nO + nV + nJ) * sizeof(S));
+  S* wV = wO + nO;
+  S* wJ = wV + nV;
+  const S* mc;
+  if constexpr (f32) mc = m->mc32;
+  else mc = m->mc64;
+  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, m->gen.get(), mc, B, CView<S>{dq, vq.rs, vq.bs},
+                                        CView<S>{dv, vv.rs, vv.bs}, CView<S>{}, View<S>{wO, B, 1}, View<S>{wV, B, 1},
+                                        View<S>{}, View<S>{wJ, B, 1});
+  hipLaunchKernelGGL(k_jacobian_dot<S>, dim3((unsigned)((B + 255) / 256), (unsigned)nv), dim3(256), 0, ctx->stream, B,
+                     (const int32_t*)dtab, CView<S>{wO, B, 1}, CView<S>{wV, B, 1}, CView<S>{wJ, B, 1},
+                     View<S>{ddJ, vd.rs, vd.bs}, View<S>{dJo, vJ.rs, vJ.bs});
+  finishCall(ctx, flags & ~(uint32_t)RBD_ASYNC, st);
+}
+
 template<class S> __global__ void k_fill_zero(S* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = S(0);
@@ -1428,6 +1474,17 @@ int rbd_frames(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const voi
     else
       framesCall<double>(ctx, m, B, q, ldq, nframes, frame_joint, frame_placement, reference_frame, oMf, ldoMf, J,
                          ldJ, flags);
+  });
+}
+
+int rbd_joint_jacobians_time_variation(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q,
+                                       int64_t ldq, const void* v, int64_t ldv, void* J, int64_t ldJ, void* dJ,
+                                       int64_t lddJ, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && v && dJ), "null array argument");
+    if (dtype == RBD_F32) jacobianDotCall<float>(ctx, m, B, q, ldq, v, ldv, J, ldJ, dJ, lddJ, flags);
+    else jacobianDotCall<double>(ctx, m, B, q, ldq, v, ldv, J, ldJ, dJ, lddJ, flags);
   });
 }
 

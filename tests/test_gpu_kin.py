@@ -77,3 +77,29 @@ def test_kinematics_empty_and_position_only(gpu_ctx):
     z = torch.zeros((desc.nq, 0), dtype=torch.float64, device="cuda")
     oM0, _, _ = eng.forward_kinematics(z)
     assert oM0.shape[1] == 0
+
+
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("name", MODELS)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_jacobian_time_variation(gpu_ctx, name, layout, dtype):
+    """computeJointJacobiansTimeVariation (jacobian.hxx:673-740) against the oracle (whose dJ
+    is pinned by central differences of J along integrate(q, v t)); bars as above, relative
+    to each configuration's largest |dJ| entry."""
+    desc, model = gpu_model(gpu_ctx, name)
+    eng = pa.Engine(model, layout)
+    B = 301
+    q, v, _, _ = inputs(desc, B, k=92)
+    if dtype == torch.float32:
+        q, v = f32_round(q, v)
+    Q, V = (to_dev(x, layout, dtype) for x in (q, v))
+    J, dJ = eng.joint_jacobians_time_variation(Q, V)
+    rJ, rdJ = oracle.joint_jacobians_time_variation(desc, q, v)
+    gJ = to_host(J, layout).reshape(B, desc.nv, 6).transpose(0, 2, 1)
+    gdJ = to_host(dJ, layout).reshape(B, desc.nv, 6).transpose(0, 2, 1)
+    rtol = 1e-12 if dtype == torch.float64 else 1e-5
+    for g, r in ((gJ, rJ), (gdJ, rdJ)):
+        scale = np.abs(r).reshape(B, -1).max(1)[:, None, None]
+        assert (np.abs(g - r) / np.maximum(scale, 1e-30)).max() < rtol
+    _, dJ2 = eng.joint_jacobians_time_variation(Q, V, jacobians=False)
+    assert torch.equal(dJ2, dJ)

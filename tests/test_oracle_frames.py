@@ -180,3 +180,19 @@ def test_frame_motions_between_reference_frames(name):
                 if k < 2:  # spatial motions: WORLD linear = LWA linear + p x angular
                     assert np.abs(wor[:3] - (lwa[:3] + np.cross(p, lwa[3:]))).max() < 1e-12 * max(
                         1.0, np.abs(lwa).max(), np.abs(wor).max())
+
+
+# ---- computeJointJacobiansTimeVariation (jacobian.hxx:673-740), pinned by central
+# differences of the WORLD joint Jacobian along q(t) = integrate(q, v t): dJ = dJ/dt.
+
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "arm7"])
+def test_jacobian_time_variation_finite_differences(name):
+    d = getattr(M, name)()
+    q, v, _, _ = M.sample_inputs(d, 2, 93)
+    J, dJ = oracle.joint_jacobians_time_variation(d, q, v)
+    assert np.abs(J - oracle.joint_jacobians(d, q)).max() < 1e-14
+    h = 1e-6
+    Jp = oracle.joint_jacobians(d, oracle.integrate(d, q, v * h))
+    Jm = oracle.joint_jacobians(d, oracle.integrate(d, q, -v * h))
+    fd = (Jp - Jm) / (2 * h)
+    assert np.abs(fd - dJ).max() < 1e-7 * max(1.0, np.abs(dJ).max())
