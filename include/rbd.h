@@ -25,6 +25,8 @@
  *                         computeJointJacobiansTimeVariation  algorithm/jacobian.hxx:673-740
  *   rbd_frames            updateFramePlacements,    algorithm/frames.hxx:15-35,
  *                         getFrameJacobian          algorithm/frames.hxx:156-175
+ *   rbd_frame_jacobians_time_variation
+ *                         getFrameJacobianTimeVariation  algorithm/frames.hxx:293-368
  *   rbd_frame_motions     getFrameVelocity,         algorithm/frames.hxx:71-99,
  *                         getFrameAcceleration,     algorithm/frames.hxx:102-130,
  *                         getFrameClassicalAcceleration  algorithm/frames.hxx:133-149
@@ -258,6 +260,15 @@ enum { RBD_WORLD = 0, RBD_LOCAL = 1, RBD_LOCAL_WORLD_ALIGNED = 2 };
 int rbd_frames(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
                int nframes, const int32_t* frame_joint, const double* frame_placement, int reference_frame,
                void* oMf, int64_t ldoMf, void* J, int64_t ldJ, uint32_t flags);
+
+/* getFrameJacobianTimeVariation (frames.hxx:293-368) after
+ * computeJointJacobiansTimeVariation(q, v), for the same frame tables as rbd_frames: the
+ * time variation of each frame's 6 x nv Jacobian in the requested reference frame (element
+ * f 6 nv + r + 6 c, zero outside the parent joint's support). */
+int rbd_frame_jacobians_time_variation(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                                       int64_t ldq, const void* v, int64_t ldv, int nframes, const int32_t* frame_joint,
+                                       const double* frame_placement, int reference_frame, void* dJ, int64_t lddJ,
+                                       uint32_t flags);
 
 /* Frame motions after forwardKinematics(q, v[, a]) for the same frame tables as rbd_frames:
  * getFrameVelocity (frames.hxx:71-99) into vf, getFrameAcceleration (frames.hxx:102-130)

@@ -143,6 +143,23 @@ def frames(model, q, parents, placements, reference_frame="world", scalar="f64",
     return oMf, J.transpose(0, 1, 3, 2).copy()
 
 
+def frame_jacobians_time_variation(model, q, v, parents, placements, reference_frame="local", scalar="f64",
+                                   nthreads=0):
+    """getFrameJacobianTimeVariation after computeJointJacobiansTimeVariation, frames as in
+    frames(): dJ (B, nf, 6, nv)."""
+    q, v = _cols(q, model.nq), _cols(v, model.nv)
+    B = q.shape[1]
+    par = np.ascontiguousarray(parents, dtype=np.int32)
+    pl = np.ascontiguousarray(placements, dtype=np.float64).reshape(len(par), 12)
+    nf = len(par)
+    dJ = np.zeros((B, nf, model.nv, 6))
+    d = model.cstruct()
+    _check(lib().orc_frame_jacobians_time_variation(ctypes.byref(d), ctypes.c_int64(B), _p(q), _p(v), int(nf), _p(par),
+                                                    _p(pl), REFERENCE_FRAMES[reference_frame], _p(dJ),
+                                                    SCALARS[scalar], nthreads))
+    return dJ.transpose(0, 1, 3, 2).copy()
+
+
 def frame_motions(model, q, v, parents, placements, a=None, reference_frame="local", scalar="f64", nthreads=0):
     """getFrameVelocity / getFrameAcceleration / getFrameClassicalAcceleration after
     forwardKinematics(q, v[, a]) for frames given as (parent joint, placement [R row-major 9,

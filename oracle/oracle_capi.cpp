@@ -201,6 +201,26 @@ template<class S> int jacDotT(const ModelDesc* d, int64_t B, const double* q, co
   });
 }
 template<class S>
+int frameJacDotT(const ModelDesc* d, int64_t B, const double* q, const double* v, int nf, const int32_t* parent,
+                 const double* placement, int rf, double* dJf, int nt) {
+  return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
+    static thread_local Buf<S> bq, bv;
+    static thread_local Mat<S> dJm;
+    static thread_local std::vector<S> out;
+    if (dJm.rows != 6 || dJm.cols != m.nv) dJm = Mat<S>(6, m.nv);
+    out.resize((size_t)6 * m.nv);
+    computeJointJacobiansTimeVariation(m, data, bq.load(q + b * m.nq, m.nq), bv.load(v + b * m.nv, m.nv), dJm);
+    for (int f = 0; f < nf; ++f) {
+      if (parent[f] < 0 || parent[f] >= m.njoints) throw std::invalid_argument("frame parent out of range");
+      SE3<S> X;
+      for (int k = 0; k < 9; ++k) X.R.m[k] = S(placement[12 * f + k]);
+      for (int k = 0; k < 3; ++k) X.p[k] = S(placement[12 * f + 9 + k]);
+      frameJacobianTimeVariation(m, data, dJm, parent[f], X, rf, out.data());
+      store(out.data(), dJf + (b * nf + f) * 6 * m.nv, 6 * m.nv);
+    }
+  });
+}
+template<class S>
 int framesT(const ModelDesc* d, int64_t B, const double* q, int nf, const int32_t* parent, const double* placement,
             int rf, double* oMf, double* J, int nt) {
   return runBatch<S>(d, B, nt, [&](const Model<S>& m, Data<S>& data, int64_t b) {
@@ -346,6 +366,20 @@ int orc_frame_motions(const ModelDesc* d, int64_t B, const double* q, const doub
 int orc_joint_jacobians_time_variation(const ModelDesc* d, int64_t B, const double* q, const double* v, double* J,
                                        double* dJ, int scalar, int nthreads) {
 #define CALL(S) jacDotT<S>(d, B, q, v, J, dJ, nthreads)
+  switch (scalar) {
+    case 0: return CALL(double);
+    case 1: return CALL(float);
+    case 2: return CALL(long double);
+  }
+#undef CALL
+  g_err = "bad scalar";
+  return 1;
+}
+
+int orc_frame_jacobians_time_variation(const ModelDesc* d, int64_t B, const double* q, const double* v, int nf,
+                                       const int32_t* parent, const double* placement, int rf, double* dJf,
+                                       int scalar, int nthreads) {
+#define CALL(S) frameJacDotT<S>(d, B, q, v, nf, parent, placement, rf, dJf, nthreads)
   switch (scalar) {
     case 0: return CALL(double);
     case 1: return CALL(float);

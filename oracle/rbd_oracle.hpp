@@ -980,6 +980,37 @@ inline void frameJacobian(const Model<S>& model, const Data<S>& data, int parent
     }
 }
 
+// getFrameJacobianTimeVariation (frames.hxx:293-368) after
+// computeJointJacobiansTimeVariation: the support columns of the joint dJ taken into the
+// reference frame like frameJacobian takes J, then for LOCAL
+//   dJ_c -= v_frame x (oMf^-1 . J_c),  v_frame = placement^-1 . v_parent,
+// and for LOCAL_WORLD_ALIGNED
+//   dJ_c.lin -= (ov_parent.lin + ov_parent.ang x oMf.p) x J_c.ang.
+template<class S>
+inline void frameJacobianTimeVariation(const Model<S>& model, const Data<S>& data, const Mat<S>& dJj, int parent,
+                                       const SE3<S>& placement, int rf, S* dJ) {
+  const SE3<S> oMf = data.oMi[parent] * placement;
+  std::fill(dJ, dJ + 6 * model.nv, S(0));
+  const Motion<S> vf = actInv(placement, data.v[parent]);
+  const Motion<S> ov = act(data.oMi[parent], data.v[parent]);
+  for (int j = parent; j > 0; j = model.parents[j])
+    for (int k = 0; k < model.nvs[j]; ++k) {
+      const int c = model.idx_v[j] + k;
+      Motion<S> m = colM(dJj, c);
+      const Motion<S> Jc = colM(data.J, c);
+      if (rf == RF_LOCAL_WORLD_ALIGNED) {
+        m.lin = m.lin - cross(oMf.p, m.ang);
+        m.lin = m.lin - cross(ov.lin + cross(ov.ang, oMf.p), Jc.ang);
+      } else if (rf == RF_LOCAL) {
+        m = actInv(oMf, m);
+        const Motion<S> x = crossMM(vf, actInv(oMf, Jc));
+        m.lin = m.lin - x.lin;
+        m.ang = m.ang - x.ang;
+      }
+      toVec(m, dJ + 6 * c);
+    }
+}
+
 // ----------------------------------------------------------------------------------------
 // Frame motions after forwardKinematics(q, v[, a]): getFrameVelocity (frames.hxx:71-99),
 // getFrameAcceleration (frames.hxx:102-130), getFrameClassicalAcceleration

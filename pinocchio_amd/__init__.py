@@ -445,6 +445,25 @@ class Engine:
                                 ctypes.c_uint32(self._flags(False))))
         return oMf, J
 
+    def frame_jacobians_time_variation(self, q, v, frame_joint, frame_placement, reference_frame="local"):
+        """getFrameJacobianTimeVariation after computeJointJacobiansTimeVariation(q, v): each
+        frame's 6 x nv dJ in the reference frame (rows f 6 nv + r + 6 c).  Synchronous."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        fj = np.ascontiguousarray(frame_joint, dtype=np.int32).ravel()
+        nf = len(fj)
+        fp = np.ascontiguousarray(frame_placement, dtype=np.float64).reshape(nf, 12)
+        dJ = self._new_vec(q, 6 * m.nv * nf, B)
+        _, ldd = self._vec(dJ, 6 * m.nv * nf, "dJ")
+        _check(lib().rbd_frame_jacobians_time_variation(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q),
+                                                        _i64(ldq), _ptr(v), _i64(ldv), int(nf),
+                                                        fj.ctypes.data_as(ctypes.c_void_p),
+                                                        fp.ctypes.data_as(ctypes.c_void_p),
+                                                        int(self.REFERENCE_FRAMES[reference_frame]), _ptr(dJ),
+                                                        _i64(ldd), ctypes.c_uint32(self._flags(False))))
+        return dJ
+
     def frame_motions(self, q, v, frame_joint, frame_placement, a=None, reference_frame="local"):
         """getFrameVelocity, getFrameAcceleration and getFrameClassicalAcceleration after
         forwardKinematics(q, v[, a]) for frames given by parent joint and placement (12 numbers

@@ -215,4 +215,101 @@ __global__ __launch_bounds__(256) void k_jacobian_dot(int64_t B, const int32_t* 
   }
 }
 
+// getFrameJacobianTimeVariation (frames.hxx:293-368) from oMi, the joints' LOCAL velocities,
+// the WORLD Jacobian J and its time variation dJ (k_jacobian_dot): the support columns of
+// dJ taken into the frame like k_frames takes J, then
+//   LOCAL                dJ_c -= v_f x (oMf^-1 . J_c),  v_f = placement^-1 . v_parent
+//   LOCAL_WORLD_ALIGNED  dJ_c.lin -= (ov.lin + ov.ang x oMf.p) x J_c.ang,  ov = oMi . v_parent
+// Output rows f 6 nv + r + 6 c, zero outside the support.
+template<class S>
+__global__ __launch_bounds__(256) void k_frame_jacobian_dot(int64_t B, int nv, const int32_t* __restrict__ fpar,
+                                                            const S* __restrict__ fpl, const uint8_t* __restrict__ support,
+                                                            CView<S> oMi, CView<S> vj, CView<S> Jw, CView<S> dJw, int rf,
+                                                            View<S> out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (b >= B) return;
+  const int p = fpar[f];
+  const S* X = fpl + 12 * f;
+  S Ri[9], ti[3], vp[6];
+  if (p > 0) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ri[k] = oMi.p[(int64_t)(12 * (p - 1) + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ti[k] = oMi.p[(int64_t)(12 * (p - 1) + 9 + k) * oMi.rs + b * oMi.bs];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vp[k] = vj.p[(int64_t)(6 * (p - 1) + k) * vj.rs + b * vj.bs];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ri[k] = (k % 4 == 0) ? S(1) : S(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ti[k] = S(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vp[k] = S(0);
+  }
+  S R[9], t[3];  // oMf = oMi * placement
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R[3 * r + c] = Ri[3 * r] * X[c] + Ri[3 * r + 1] * X[3 + c] + Ri[3 * r + 2] * X[6 + c];
+    t[r] = Ri[3 * r] * X[9] + Ri[3 * r + 1] * X[10] + Ri[3 * r + 2] * X[11] + ti[r];
+  }
+  S vf[6], u[3];  // v_f = placement^-1 . v_parent; u = ov.lin + ov.ang x oMf.p
+  frameMotion(R, ti, X, X + 9, kRfLocal, vp, vf);
+  {
+    S ov[6];
+    frameMotion(Ri, ti, X, X + 9, kRfWorld, vp, ov);
+    u[0] = ov[0] + ov[4] * t[2] - ov[5] * t[1];
+    u[1] = ov[1] + ov[5] * t[0] - ov[3] * t[2];
+    u[2] = ov[2] + ov[3] * t[1] - ov[4] * t[0];
+  }
+  const uint8_t* sup = support + (int64_t)p * nv;
+  const int64_t base = (int64_t)f * 6 * nv;
+  for (int c = 0; c < nv; ++c) {
+    S o[6] = {S(0), S(0), S(0), S(0), S(0), S(0)};
+    if (sup[c]) {
+      S m[6], J[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        m[k] = dJw.p[(int64_t)(6 * c + k) * dJw.rs + b * dJw.bs];
+        J[k] = Jw.p[(int64_t)(6 * c + k) * Jw.rs + b * Jw.bs];
+      }
+      if (rf == kRfWorld) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[k] = m[k];
+      } else if (rf == kRfLocalWorldAligned) {  // lin - t x ang - u x J.ang
+        o[0] = m[0] - (t[1] * m[5] - t[2] * m[4]) - (u[1] * J[5] - u[2] * J[4]);
+        o[1] = m[1] - (t[2] * m[3] - t[0] * m[5]) - (u[2] * J[3] - u[0] * J[5]);
+        o[2] = m[2] - (t[0] * m[4] - t[1] * m[3]) - (u[0] * J[4] - u[1] * J[3]);
+#pragma unroll
+        for (int k = 3; k < 6; ++k) o[k] = m[k];
+      } else {  // oMf^-1 . m - v_f x (oMf^-1 . J)
+        S ml[3], jl[3], ja[3];
+        ml[0] = m[0] - (t[1] * m[5] - t[2] * m[4]);
+        ml[1] = m[1] - (t[2] * m[3] - t[0] * m[5]);
+        ml[2] = m[2] - (t[0] * m[4] - t[1] * m[3]);
+        const S j0 = J[0] - (t[1] * J[5] - t[2] * J[4]);
+        const S j1 = J[1] - (t[2] * J[3] - t[0] * J[5]);
+        const S j2 = J[2] - (t[0] * J[4] - t[1] * J[3]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          o[k] = R[k] * ml[0] + R[3 + k] * ml[1] + R[6 + k] * ml[2];
+          o[3 + k] = R[k] * m[3] + R[3 + k] * m[4] + R[6 + k] * m[5];
+          jl[k] = R[k] * j0 + R[3 + k] * j1 + R[6 + k] * j2;
+          ja[k] = R[k] * J[3] + R[3 + k] * J[4] + R[6 + k] * J[5];
+        }
+        // v_f x (jl, ja) = (w x jl + vl x ja, w x ja), v_f = (vl, w)
+        o[0] -= vf[4] * jl[2] - vf[5] * jl[1] + vf[1] * ja[2] - vf[2] * ja[1];
+        o[1] -= vf[5] * jl[0] - vf[3] * jl[2] + vf[2] * ja[0] - vf[0] * ja[2];
+        o[2] -= vf[3] * jl[1] - vf[4] * jl[0] + vf[0] * ja[1] - vf[1] * ja[0];
+        o[3] -= vf[4] * ja[2] - vf[5] * ja[1];
+        o[4] -= vf[5] * ja[0] - vf[3] * ja[2];
+        o[5] -= vf[3] * ja[1] - vf[4] * ja[0];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) out.p[(base + 6 * c + r) * out.rs + b * out.bs] = o[r];
+  }
+}
+
 }  // namespace rbd

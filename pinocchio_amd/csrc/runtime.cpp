@@ -1126,6 +1126,68 @@ void jacobianDotCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q,
   finishCall(ctx, flags & ~(uint32_t)RBD_ASYNC, st);
 }
 
+// Frame Jacobian time variation: k_kin (oMi, joint velocities, WORLD Jacobian),
+// k_jacobian_dot (WORLD dJ), then k_frame_jacobian_dot.  Frame tables are host arrays,
+// copied during the call (which therefore synchronises).
+template<class S>
+void frameJacobianDotCall(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* q, int64_t ldq, const void* v,
+                          int64_t ldv, int nf, const int32_t* fjoint, const double* fplace, int rf, void* dJ,
+                          int64_t lddJ, uint32_t flags) {
+  constexpr bool f32 = std::is_same_v<S, float>;
+  const auto launch = f32 ? (void*)m->ops->kin_f32 : (void*)m->ops->kin_f64;
+  if (!launch) throw Error(RBD_EUNSUPPORTED, "frame Jacobian time variation: no kinematics kernel for this topology");
+  const int nj = m->njoints, nv = m->nv;
+  auto vq = argView<S>(m->nq, B, ldq, flags, "q");
+  auto vv = argView<S>(nv, B, ldv, flags, "v");
+  auto vd = argView<S>(6 * nv * nf, B, lddJ, flags, "dJ");
+  if (B == 0 || nf == 0) return;
+  std::vector<S> pl((size_t)12 * nf);
+  for (size_t k = 0; k < pl.size(); ++k) pl[k] = S(fplace[k]);
+  std::vector<uint8_t> sup((size_t)nj * nv, 0);
+  std::vector<int32_t> dofJoint(nv);
+  for (int j = 1; j < nj; ++j) {
+    const int nvj = m->jtype[j] == RBD_JOINT_FREEFLYER ? 6 : 1;
+    for (int k = 0; k < nvj; ++k) dofJoint[m->iv[j] + k] = j;
+    for (int a = j; a > 0; a = m->parents[a]) {
+      const int nva = m->jtype[a] == RBD_JOINT_FREEFLYER ? 6 : 1;
+      for (int k = 0; k < nva; ++k) sup[(size_t)j * nv + m->iv[a] + k] = 1;
+    }
+  }
+  setDevice(ctx);
+  Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
+  const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
+  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
+  S* dd = (S*)st.out(dJ, vd.extent * sizeof(S), (flags & RBD_LAYOUT_BATCH) != 0 || lddJ != 6 * nv * nf);
+  auto table = [&](const void* h, size_t bytes) {
+    void* d = ctx->stageBuf(st.slot++, bytes);
+    HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return d;
+  };
+  const int32_t* dpar = (const int32_t*)table(fjoint, sizeof(int32_t) * nf);
+  const S* dpl = (const S*)table(pl.data(), sizeof(S) * pl.size());
+  const uint8_t* dsup = (const uint8_t*)table(sup.data(), sup.size());
+  const int32_t* ddof = (const int32_t*)table(dofJoint.data(), sizeof(int32_t) * nv);
+  // workspace, batch-fastest: oMi, joint velocities, WORLD J, WORLD dJ
+  const size_t nO = (size_t)12 * (nj - 1) * B, nV = (size_t)6 * (nj - 1) * B, nJ = (size_t)6 * nv * B;
+  S* wO = (S*)ctx->workspace((nO + nV + 2 * nJ) * sizeof(S));
+  S* wV = wO + nO;
+  S* wJ = wV + nV;
+  S* wD = wJ + nJ;
+  const S* mc;
+  if constexpr (f32) mc = m->mc32;
+  else mc = m->mc64;
+  reinterpret_cast<LaunchKin<S>>(launch)(ctx->stream, m->gen.get(), mc, B, CView<S>{dq, vq.rs, vq.bs},
+                                        CView<S>{dv, vv.rs, vv.bs}, CView<S>{}, View<S>{wO, B, 1}, View<S>{wV, B, 1},
+                                        View<S>{}, View<S>{wJ, B, 1});
+  const unsigned gb = (unsigned)((B + 255) / 256);
+  hipLaunchKernelGGL(k_jacobian_dot<S>, dim3(gb, (unsigned)nv), dim3(256), 0, ctx->stream, B, ddof, CView<S>{wO, B, 1},
+                     CView<S>{wV, B, 1}, CView<S>{wJ, B, 1}, View<S>{wD, B, 1}, View<S>{});
+  hipLaunchKernelGGL(k_frame_jacobian_dot<S>, dim3(gb, (unsigned)nf), dim3(256), 0, ctx->stream, B, nv, dpar, dpl, dsup,
+                     CView<S>{wO, B, 1}, CView<S>{wV, B, 1}, CView<S>{wJ, B, 1}, CView<S>{wD, B, 1}, rf,
+                     View<S>{dd, vd.rs, vd.bs});
+  finishCall(ctx, flags & ~(uint32_t)RBD_ASYNC, st);
+}
+
 template<class S> __global__ void k_fill_zero(S* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = S(0);
@@ -1485,6 +1547,27 @@ int rbd_joint_jacobians_time_variation(rbd_ctx* ctx, const rbd_model* m, int64_t
     RBD_CHECK_ARG(B == 0 || (q && v && dJ), "null array argument");
     if (dtype == RBD_F32) jacobianDotCall<float>(ctx, m, B, q, ldq, v, ldv, J, ldJ, dJ, lddJ, flags);
     else jacobianDotCall<double>(ctx, m, B, q, ldq, v, ldv, J, ldJ, dJ, lddJ, flags);
+  });
+}
+
+int rbd_frame_jacobians_time_variation(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q,
+                                       int64_t ldq, const void* v, int64_t ldv, int nframes, const int32_t* frame_joint,
+                                       const double* frame_placement, int reference_frame, void* dJ, int64_t lddJ,
+                                       uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(nframes >= 0 && nframes <= 65535, "the frame count must be between 0 and 65535");
+    RBD_CHECK_ARG(nframes == 0 || (frame_joint && frame_placement), "null frame table");
+    RBD_CHECK_ARG(reference_frame >= RBD_WORLD && reference_frame <= RBD_LOCAL_WORLD_ALIGNED, "Bad reference frame.");
+    for (int f = 0; f < nframes; ++f)
+      RBD_CHECK_ARG(frame_joint[f] >= 0 && frame_joint[f] < m->njoints, "The index of the Joint is outside the bounds.");
+    RBD_CHECK_ARG(B == 0 || nframes == 0 || (q && v && dJ), "null array argument");
+    if (dtype == RBD_F32)
+      frameJacobianDotCall<float>(ctx, m, B, q, ldq, v, ldv, nframes, frame_joint, frame_placement, reference_frame, dJ,
+                                  lddJ, flags);
+    else
+      frameJacobianDotCall<double>(ctx, m, B, q, ldq, v, ldv, nframes, frame_joint, frame_placement, reference_frame,
+                                   dJ, lddJ, flags);
   });
 }
 

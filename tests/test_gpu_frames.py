@@ -161,3 +161,28 @@ def _check_raw_rf(eng, Q, V, pl):
                                          ctypes.c_int64(0), 1, fj.ctypes.data_as(ctypes.c_void_p),
                                          fp.ctypes.data_as(ctypes.c_void_p), 7, ctypes.c_void_p(out.data_ptr()), None,
                                          None, ctypes.c_int64(Q.shape[1]), ctypes.c_uint32(pa.RBD_LAYOUT_BATCH)))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("rf", ["world", "local", "local_world_aligned"])
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18"])
+def test_frame_jacobian_time_variation_parity(gpu_ctx, name, rf, layout, prec):
+    """getFrameJacobianTimeVariation (frames.hxx:293-368) against the oracle (pinned by
+    central differences of the frame Jacobian along integrate(q, v t)), same bars."""
+    desc, model = gpu_model(gpu_ctx, name)
+    B, nf = 201, 4
+    q, v, _, _ = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 882)
+    par, pl = random_frames(desc, nf, 9)
+    if prec == "f32":
+        q, v = f32_round(q, v)
+        pl = np.asarray(pl, dtype=np.float32).astype(np.float64)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    eng = pa.Engine(model, layout)
+    dev = lambda x: torch.tensor(np.ascontiguousarray(x.T if layout == "col" else x), dtype=dt, device="cuda")
+    dJ = eng.frame_jacobians_time_variation(dev(q), dev(v), par, pl, rf).double().cpu().numpy()
+    dJ = (dJ.T if layout == "batch" else dJ).reshape(B, nf, desc.nv, 6).transpose(0, 1, 3, 2)
+    ref = oracle.frame_jacobians_time_variation(desc, q, v, par, pl, rf)
+    e = rel(dJ, ref)
+    print(f"{name} {rf} {layout} {prec}: dJ {e:.2e}")
+    assert e < TOL[prec]

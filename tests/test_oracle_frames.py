@@ -196,3 +196,22 @@ def test_jacobian_time_variation_finite_differences(name):
     Jm = oracle.joint_jacobians(d, oracle.integrate(d, q, -v * h))
     fd = (Jp - Jm) / (2 * h)
     assert np.abs(fd - dJ).max() < 1e-7 * max(1.0, np.abs(dJ).max())
+
+
+# ---- getFrameJacobianTimeVariation (frames.hxx:293-368), pinned by central differences of
+# the frame Jacobian (pinned above) along q(t) = integrate(q, v t), in each reference frame.
+
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "arm7"])
+def test_frame_jacobian_time_variation_finite_differences(name):
+    d = getattr(M, name)()
+    q, v, _, _ = M.sample_inputs(d, 1, 94)
+    par, pl = random_frames(d, 5, 8)
+    h = 1e-6
+    qp, qm = oracle.integrate(d, q, v * h), oracle.integrate(d, q, -v * h)
+    for rf in ("world", "local", "local_world_aligned"):
+        dJ = oracle.frame_jacobians_time_variation(d, q, v, par, pl, rf)
+        _, Jp = oracle.frames(d, qp, par, pl, rf)
+        _, Jm = oracle.frames(d, qm, par, pl, rf)
+        fd = (Jp - Jm) / (2 * h)
+        assert np.abs(fd - dJ).max() < 1e-7 * max(1.0, np.abs(dJ).max()), rf
+    assert np.abs(dJ[:, 1]).max() == 0.0  # the universe frame
