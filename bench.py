@@ -572,6 +572,13 @@ def main():
     t = time_call(lambda: eng.frames(Q, fj, fpl, "local_world_aligned"), reps=10)
     others["frames_humanoid36_f32"] = {"batch": B, "frames": len(fj), "configs_per_s": B / t, "ms": t * 1e3,
                                        "GB_s": 4 * (desc.nq + len(fj) * (12 + 6 * nv)) * B / t / 1e9}
+    # frame velocities + spatial and classical accelerations of the same four frames (LOCAL),
+    # and the WORLD joint Jacobian with its time variation (synchronous calls)
+    t = time_call(lambda: eng.frame_motions(Q, V, fj, fpl, a=A), reps=10)
+    others["frame_motions_humanoid36_f32"] = {"batch": B, "frames": len(fj), "configs_per_s": B / t,
+                                              "ms": t * 1e3}
+    t = time_call(lambda: eng.joint_jacobians_time_variation(Q, V), reps=10)
+    others["jacobian_time_variation_humanoid36_f32"] = {"batch": B, "configs_per_s": B / t, "ms": t * 1e3}
 
     # drop-in host path: numpy matrices in and out (PCIe both ways), not the device-resident figure
     pool = pa.ModelPool(desc, size=8)
