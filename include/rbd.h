@@ -10,6 +10,8 @@
  *                         (per column: rnea         algorithm/rnea.hxx:117-161, 773)
  *   rbd_aba               abaInParallel             algorithm/parallel/aba.hpp:40-84
  *                         (per column: aba WORLD    algorithm/aba.hxx:242-293, 926-948)
+ *   rbd_nonlinear_effects nonLinearEffects          algorithm/rnea.hpp:105, rnea.hxx:302
+ *   rbd_generalized_gravity computeGeneralizedGravity algorithm/rnea.hpp:133, rnea.hxx:412
  *   rbd_rnea_fext, rbd_aba_fext
  *                         rnea / aba with fext      algorithm/rnea.hpp:60-79, aba.hpp:63-85
  *   rbd_crba              crba                      algorithm/crba.hxx:498-548, 602-617
@@ -151,6 +153,13 @@ int rbd_model_dims(const rbd_model* model, int32_t* nq, int32_t* nv, int32_t* nj
 /* tau = rnea(q, v, a) per configuration.  Replaces rneaInParallel. */
 int rbd_rnea(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
              const void* v, int64_t ldv, const void* a, int64_t lda, void* tau, int64_t ldtau, uint32_t flags);
+/* nonLinearEffects (rnea.hpp:105, rnea.hxx:302): C(q, v) v + g(q) = rnea(q, v, 0), and
+ * computeGeneralizedGravity (rnea.hpp:133, rnea.hxx:412): g(q) = rnea(q, 0, 0);
+ * the zero inputs are not read from the caller.  Layouts and flags as rbd_rnea. */
+int rbd_nonlinear_effects(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                          const void* v, int64_t ldv, void* nle, int64_t ldnle, uint32_t flags);
+int rbd_generalized_gravity(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
+                            int64_t ldq, void* g, int64_t ldg, uint32_t flags);
 
 /* ddq = aba(q, v, tau) per configuration.  Replaces abaInParallel (WORLD convention; the
  * result is convention-independent up to rounding, the engine sweeps in local frames). */

@@ -260,6 +260,28 @@ class Engine:
                              _ptr(tau), _i64(ldt), _ptr(out), _i64(ldo), ctypes.c_uint32(self._flags(async_))))
         return out
 
+    def nonlinear_effects(self, q, v, out=None, async_=False):
+        """nonLinearEffects: C(q, v) v + g(q) per configuration (rnea with a = 0)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldv = self._vec(v, m.nv, "v")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        _, ldo = self._vec(out, m.nv, "nle")
+        _check(lib().rbd_nonlinear_effects(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                           _ptr(v), _i64(ldv), _ptr(out), _i64(ldo),
+                                           ctypes.c_uint32(self._flags(async_))))
+        return out
+
+    def generalized_gravity(self, q, out=None, async_=False):
+        """computeGeneralizedGravity: g(q) per configuration (rnea with v = a = 0)."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        _, ldo = self._vec(out, m.nv, "g")
+        _check(lib().rbd_generalized_gravity(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                             _ptr(out), _i64(ldo), ctypes.c_uint32(self._flags(async_))))
+        return out
+
     def rnea_fext(self, q, v, a, fext, out=None, async_=False):
         """rnea(model, data, q, v, a, fext) per configuration (rnea.hpp:60-79): fext has
         6 (njoints - 1) rows, each joint's external force [linear; angular] in its frame."""

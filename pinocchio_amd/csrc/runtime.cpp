@@ -197,24 +197,30 @@ void checkCommon(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype) {
   RBD_CHECK_ARG(dtype == RBD_F32 || dtype == RBD_F64, "dtype must be RBD_F32 or RBD_F64");
 }
 
+// marks an input of vec3Op that is zero for every configuration
+const void* const kZeroInput = reinterpret_cast<const void*>(&kZeroInput);
+
 template<class S>
 int vec3Op(rbd_ctx* ctx, const rbd_model* m, int64_t B, const void* x0, int64_t ld0, int r0, const void* x1,
            int64_t ld1, const void* x2, int64_t ld2, void* out, int64_t ldo, uint32_t flags, LaunchVec3<S> fn,
            const char* opname) {
   return guarded([&] {
     if (!fn) throw Error(RBD_EUNSUPPORTED, std::string(opname) + ": no kernel for this dtype/topology");
+    // x1 / x2 == kZeroInput: the input is zero for every configuration (a stride-0 view of
+    // the context's zero block; nothing is read from the caller)
+    const bool z1 = x1 == kZeroInput, z2 = x2 == kZeroInput;
     RBD_CHECK_ARG(B == 0 || (x0 && x1 && x2 && out), "null array argument");
     const int nv = m->nv;
     auto v0 = argView<S>(r0, B, ld0, flags, "input 0");
-    auto v1 = argView<S>(nv, B, ld1, flags, "input 1");
-    auto v2 = argView<S>(nv, B, ld2, flags, "input 2");
+    auto v1 = z1 ? ArgView<S>{0, 0, 1} : argView<S>(nv, B, ld1, flags, "input 1");
+    auto v2 = z2 ? ArgView<S>{0, 0, 1} : argView<S>(nv, B, ld2, flags, "input 2");
     auto vo = argView<S>(nv, B, ldo, flags, "output");
     if (B == 0) return;
     setDevice(ctx);
     Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
     const S* d0 = (const S*)st.in(x0, v0.extent * sizeof(S));
-    const S* d1 = (const S*)st.in(x1, v1.extent * sizeof(S));
-    const S* d2 = (const S*)st.in(x2, v2.extent * sizeof(S));
+    const S* d1 = z1 ? (const S*)ctx->zeros : (const S*)st.in(x1, v1.extent * sizeof(S));
+    const S* d2 = z2 ? (const S*)ctx->zeros : (const S*)st.in(x2, v2.extent * sizeof(S));
     S* dout = (S*)st.out(out, vo.extent * sizeof(S), ldo != nv || (flags & RBD_LAYOUT_BATCH));
     const S* mc = std::is_same<S, float>::value ? (const S*)m->mc32 : (const S*)m->mc64;
     fn(ctx->stream, m->gen.get(), mc, B, CView<S>{d0, v0.rs, v0.bs}, CView<S>{d1, v1.rs, v1.bs}, CView<S>{d2, v2.rs, v2.bs},
@@ -274,6 +280,8 @@ int rbd_ctx_create(int device, rbd_ctx** out) {
       // where PyTorch (and most callers) enqueue the producers of the inputs and the
       // consumers of the outputs
       HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamDefault));
+      HIP_CHECK(hipMalloc(&c->zeros, 64));
+      HIP_CHECK(hipMemset(c->zeros, 0, 64));
       initScratch(device);
     } catch (...) {
       delete c;
@@ -292,6 +300,7 @@ int rbd_ctx_destroy(rbd_ctx* ctx) {
     for (void* p : ctx->stage)
       if (p) (void)hipFree(p);
     if (ctx->work) (void)hipFree(ctx->work);
+    if (ctx->zeros) (void)hipFree(ctx->zeros);
     for (void* p : ctx->retired) (void)hipFree(p);
     for (hipStream_t s : ctx->side) (void)hipStreamDestroy(s);
     for (hipEvent_t e : ctx->forkJoin) (void)hipEventDestroy(e);
@@ -593,6 +602,28 @@ int rbd_rnea(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void*
   if (dtype == RBD_F32)
     return vec3Op<float>(ctx, m, B, q, ldq, m->nq, v, ldv, a, lda, tau, ldtau, flags, m->ops->rnea_f32, "rnea");
   return vec3Op<double>(ctx, m, B, q, ldq, m->nq, v, ldv, a, lda, tau, ldtau, flags, m->ops->rnea_f64, "rnea");
+}
+
+int rbd_nonlinear_effects(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                          const void* v, int64_t ldv, void* nle, int64_t ldnle, uint32_t flags) {
+  int rc = guarded([&] { checkCommon(ctx, m, B, dtype); });
+  if (rc) return rc;
+  if (dtype == RBD_F32)
+    return vec3Op<float>(ctx, m, B, q, ldq, m->nq, v, ldv, kZeroInput, 0, nle, ldnle, flags, m->ops->rnea_f32,
+                         "nonLinearEffects");
+  return vec3Op<double>(ctx, m, B, q, ldq, m->nq, v, ldv, kZeroInput, 0, nle, ldnle, flags, m->ops->rnea_f64,
+                        "nonLinearEffects");
+}
+
+int rbd_generalized_gravity(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                            void* g, int64_t ldg, uint32_t flags) {
+  int rc = guarded([&] { checkCommon(ctx, m, B, dtype); });
+  if (rc) return rc;
+  if (dtype == RBD_F32)
+    return vec3Op<float>(ctx, m, B, q, ldq, m->nq, kZeroInput, 0, kZeroInput, 0, g, ldg, flags, m->ops->rnea_f32,
+                         "computeGeneralizedGravity");
+  return vec3Op<double>(ctx, m, B, q, ldq, m->nq, kZeroInput, 0, kZeroInput, 0, g, ldg, flags, m->ops->rnea_f64,
+                        "computeGeneralizedGravity");
 }
 
 int rbd_aba(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, const void* v,

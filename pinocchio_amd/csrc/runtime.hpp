@@ -62,6 +62,10 @@ struct rbd_ctx {
   // instead of freeing it (freed with the context)
   bool capturing = false;
   int graphs = 0;
+  // 64 zero bytes on the device (allocated with the context): a stride-0 view of them is an
+  // all-zero batch of any size, for the inputs that nonLinearEffects and
+  // computeGeneralizedGravity fix at zero
+  void* zeros = nullptr;
   std::vector<void*> retired;
   void* workspace(size_t bytes) {
     if (workBytes < bytes) {

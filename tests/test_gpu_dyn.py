@@ -172,3 +172,32 @@ def test_bad_arguments_c_abi(gpu_ctx):
     rc = L.rbd_rnea(gpu_ctx.handle, model.handle, ctypes.c_int64(-1), 0, p, ctypes.c_int64(7), p, ctypes.c_int64(7), p,
                     ctypes.c_int64(7), p, ctypes.c_int64(7), ctypes.c_uint32(0))
     assert rc == 1
+
+
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("name", ["humanoid36", "quadruped18", "arm7"])
+def test_nonlinear_effects_and_gravity(gpu_ctx, name, dtype, layout):
+    """nonLinearEffects = rnea(q, v, 0) and computeGeneralizedGravity = rnea(q, 0, 0)
+    (rnea.hpp:105, 133), with the zero inputs taken from the engine's stride-0 zero view:
+    bit-identical to rbd_rnea with explicit zero arrays, and against the FP64 oracle."""
+    desc = getattr(M, name)()
+    model = pa.Model(gpu_ctx, desc)
+    eng = pa.Engine(model, layout)
+    B = 301
+    q, v, _, _ = M.sample_inputs(desc, B, M.INPUT_SEED_BASE + 78)
+    if dtype == torch.float32:
+        q, v = f32_round(q, v)
+    dev = lambda x: torch.tensor(np.ascontiguousarray(x.T if layout == "col" else x), dtype=dtype, device="cuda")
+    Q, V = dev(q), dev(v)
+    Z = torch.zeros_like(V)
+    nle = eng.nonlinear_effects(Q, V)
+    g = eng.generalized_gravity(Q)
+    assert torch.equal(nle, eng.rnea(Q, V, Z))
+    assert torch.equal(g, eng.rnea(Q, Z, Z))
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    for out, vv in ((nle, v), (g, np.zeros_like(v))):
+        ref = oracle.rnea(desc, q, vv, np.zeros_like(v))
+        o = out.double().cpu().numpy()
+        o = o.T if layout == "col" else o
+        assert rel_err_cols(o, ref).max() < tol
