@@ -359,18 +359,18 @@ __global__ __launch_bounds__(64) void k_features(FeatMap fm, int64_t B, const fl
 }
 
 // ---- fused inference --------------------------------------------------------------------
-// The whole network for 128 configurations per workgroup: input features, every hidden
-// layer and the output layer, with the activations resident in LDS (row per configuration,
+// The whole network for R configurations per workgroup (inferRows below): input features,
+// every hidden layer and the output layer, with the activations resident in LDS (row per configuration,
 // bf16, pitch max(kin, H) + 8) and the weights streamed from L2 straight into the matrix
 // cores' operand registers -- no activation leaves the chip.  Products are computed
 // transposed, C(n, b) = sum_k W(n, k) X(b, k) (v_mfma_f32_32x32x16_bf16, A = 32 weight
 // rows, B = 32 configurations), so a lane holds four consecutive outputs of one
 // configuration: the SiLU epilogue writes them back to LDS as one 8-byte store, and the
 // output layer's stores run along b, coalesced in the batch-fastest ddq.  Eight waves; in
-// a hidden layer wave w owns 64 outputs (column group w % (H / 64)) of 128 / WR
+// a hidden layer wave w owns 64 outputs (column group w % (H / 64)) of R / WR
 // configurations (WR = 8 / (H / 64) row groups).  k runs in 16-steps from 0 to K, the
 // bias and SiLU are those of the layer-by-layer kernels.  LDS: the activation rows
-// (128 x pitch bf16), then the hidden layers' biases (L x H fp32).
+// (R x pitch bf16), then the hidden layers' biases (L x H fp32).
 constexpr int kInfThreads = 512, kInfMaxOut = 64, kInfMaxLayers = 9;
 constexpr int kFragElems = 512;  // one 32 x 16 weight fragment
 // configurations per workgroup: 64 for hidden widths 256 and 512 (two workgroups per CU,
