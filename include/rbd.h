@@ -12,6 +12,7 @@
  *                         (per column: aba WORLD    algorithm/aba.hxx:242-293, 926-948)
  *   rbd_nonlinear_effects nonLinearEffects          algorithm/rnea.hpp:105, rnea.hxx:302
  *   rbd_generalized_gravity computeGeneralizedGravity algorithm/rnea.hpp:133, rnea.hxx:412
+ *   rbd_static_torque     computeStaticTorque       algorithm/rnea.hpp:163, rnea.hxx:454-490
  *   rbd_rnea_fext, rbd_aba_fext
  *                         rnea / aba with fext      algorithm/rnea.hpp:60-79, aba.hpp:63-85
  *   rbd_crba              crba                      algorithm/crba.hxx:498-548, 602-617
@@ -160,6 +161,11 @@ int rbd_nonlinear_effects(rbd_ctx* ctx, const rbd_model* model, int64_t batch, i
                           const void* v, int64_t ldv, void* nle, int64_t ldnle, uint32_t flags);
 int rbd_generalized_gravity(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q,
                             int64_t ldq, void* g, int64_t ldg, uint32_t flags);
+/* computeStaticTorque (rnea.hpp:163, rnea.hxx:454-490): g(q) with the external forces,
+ * rnea(q, 0, 0, fext); fext as rbd_rnea_fext (6 (njoints - 1) rows per configuration,
+ * [linear; angular] in each joint's frame).  Run-time-topology kernels for every tree. */
+int rbd_static_torque(rbd_ctx* ctx, const rbd_model* model, int64_t batch, int dtype, const void* q, int64_t ldq,
+                      const void* fext, int64_t ldfext, void* tau, int64_t ldtau, uint32_t flags);
 
 /* ddq = aba(q, v, tau) per configuration.  Replaces abaInParallel (WORLD convention; the
  * result is convention-independent up to rounding, the engine sweeps in local frames). */

@@ -291,6 +291,19 @@ class Engine:
         """aba(model, data, q, v, tau, fext) per configuration (aba.hpp:63-85)."""
         return self._fext("rbd_aba_fext", q, v, tau, fext, out, async_)
 
+    def static_torque(self, q, fext, out=None, async_=False):
+        """computeStaticTorque (rnea.hpp:163): g(q) with external forces, rnea(q, 0, 0, fext);
+        fext as in rnea_fext."""
+        m = self.model
+        B, ldq = self._vec(q, m.nq, "q")
+        _, ldf = self._vec(fext, 6 * (m.njoints - 1), "fext")
+        out = self._new_vec(q, m.nv, B) if out is None else out
+        _, ldo = self._vec(out, m.nv, "tau")
+        _check(lib().rbd_static_torque(m.ctx.handle, m.handle, _i64(B), _dtype_code(q), _ptr(q), _i64(ldq),
+                                       _ptr(fext), _i64(ldf), _ptr(out), _i64(ldo),
+                                       ctypes.c_uint32(self._flags(async_))))
+        return out
+
     def _fext(self, fn, q, v, x, fext, out, async_):
         m = self.model
         B, ldq = self._vec(q, m.nq, "q")

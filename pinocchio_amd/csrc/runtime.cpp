@@ -238,17 +238,18 @@ void fextCall(rbd_ctx* ctx, const rbd_model* m, int alg, int64_t B, const void* 
               int64_t ldv, const void* x, int64_t ldx, const void* fext, int64_t ldf, void* out, int64_t ldo,
               uint32_t flags) {
   const int nv = m->nv;
+  const bool zv = v == kZeroInput, zx = x == kZeroInput;  // zero for every configuration
   auto vq = argView<S>(m->nq, B, ldq, flags, "q");
-  auto vv = argView<S>(nv, B, ldv, flags, "v");
-  auto vx = argView<S>(nv, B, ldx, flags, alg == 0 ? "a" : "tau");
+  auto vv = zv ? ArgView<S>{0, 0, 1} : argView<S>(nv, B, ldv, flags, "v");
+  auto vx = zx ? ArgView<S>{0, 0, 1} : argView<S>(nv, B, ldx, flags, alg == 0 ? "a" : "tau");
   auto vf = argView<S>(6 * (m->njoints - 1), B, ldf, flags, "fext");
   auto vo = argView<S>(nv, B, ldo, flags, alg == 0 ? "tau" : "ddq");
   if (B == 0) return;
   setDevice(ctx);
   Staged st{ctx, (flags & RBD_HOST_PTRS) != 0};
   const S* dq = (const S*)st.in(q, vq.extent * sizeof(S));
-  const S* dv = (const S*)st.in(v, vv.extent * sizeof(S));
-  const S* dx = (const S*)st.in(x, vx.extent * sizeof(S));
+  const S* dv = zv ? (const S*)ctx->zeros : (const S*)st.in(v, vv.extent * sizeof(S));
+  const S* dx = zx ? (const S*)ctx->zeros : (const S*)st.in(x, vx.extent * sizeof(S));
   const S* df = (const S*)st.in(fext, vf.extent * sizeof(S));
   S* dout = (S*)st.out(out, vo.extent * sizeof(S), ldo != nv || (flags & RBD_LAYOUT_BATCH));
   const S* mc = std::is_same<S, float>::value ? (const S*)m->mc32 : (const S*)m->mc64;
@@ -624,6 +625,18 @@ int rbd_generalized_gravity(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dty
                          "computeGeneralizedGravity");
   return vec3Op<double>(ctx, m, B, q, ldq, m->nq, kZeroInput, 0, kZeroInput, 0, g, ldg, flags, m->ops->rnea_f64,
                         "computeGeneralizedGravity");
+}
+
+int rbd_static_torque(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq,
+                      const void* fext, int64_t ldfext, void* tau, int64_t ldtau, uint32_t flags) {
+  return guarded([&] {
+    checkCommon(ctx, m, B, dtype);
+    RBD_CHECK_ARG(B == 0 || (q && fext && tau), "null array argument");
+    if (dtype == RBD_F32)
+      fextCall<float>(ctx, m, 0, B, q, ldq, kZeroInput, 0, kZeroInput, 0, fext, ldfext, tau, ldtau, flags);
+    else
+      fextCall<double>(ctx, m, 0, B, q, ldq, kZeroInput, 0, kZeroInput, 0, fext, ldfext, tau, ldtau, flags);
+  });
 }
 
 int rbd_aba(rbd_ctx* ctx, const rbd_model* m, int64_t B, int dtype, const void* q, int64_t ldq, const void* v,

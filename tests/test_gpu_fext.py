@@ -52,3 +52,24 @@ def test_rnea_aba_fext(gpu_ctx, name, layout):
             # ABA's FP64 bar follows the conditioning the oracle's FP32 run shows
             amp = rel_err_cols(oracle.aba_fext(desc, qq, vv, tt, ff, scalar="f32"), ra).max() / 2 ** -24
             assert et.max() < 1e-12 and ea.max() < max(1e-12, 4 * amp * 2 ** -53 * 2 ** 24)
+
+
+@pytest.mark.parametrize("layout", ["col", "batch"])
+@pytest.mark.parametrize("name", list(TREES))
+def test_static_torque(gpu_ctx, name, layout):
+    """computeStaticTorque (rnea.hpp:163) = rnea(q, 0, 0, fext): against the oracle's RNEA
+    with external forces at v = a = 0, and bit-identical to rbd_rnea_fext given explicit
+    zero arrays."""
+    desc = TREES[name]()
+    eng = pa.Engine(pa.Model(gpu_ctx, desc), layout)
+    B = 257
+    q, _, _, _ = inputs(desc, B, k=97)
+    f = np.random.default_rng(7).normal(size=(6 * (desc.njoints - 1), B))
+    z = np.zeros((desc.nv, B))
+    for dtype in (torch.float32, torch.float64):
+        qq, ff = f32_round(q, f) if dtype == torch.float32 else (q, f)
+        Q, F, Z = dev(qq, layout, dtype), dev(ff, layout, dtype), dev(z, layout, dtype)
+        g = eng.static_torque(Q, F)
+        assert torch.equal(g, eng.rnea_fext(Q, Z, Z, F))
+        e = rel_err_cols(host(g, layout), oracle.rnea_fext(desc, qq, z, z, ff)).max()
+        assert e < (1e-5 if dtype == torch.float32 else 1e-12), e
